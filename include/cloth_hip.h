@@ -1,0 +1,223 @@
+/*
+ * cloth_hip.h — C ABI of the MI355X implicit cloth step (HipSolver backend).
+ *
+ * This is the drop-in boundary for closim's `ClothSolver` backend interface
+ * (src/backend/solver.ts:5-13).  Every `cs_*` entry point below replaces one
+ * member of that interface as implemented by `CpuSolver`
+ * (src/backend/cpu-solver.ts:7-49); the citation is on each declaration.
+ *
+ * Conventions
+ *   - Plain pointers + sizes, no C++/torch types.  All arrays are host memory
+ *     owned by the caller; the library copies what it needs.
+ *   - Every function returns an int status (CS_OK == 0).  On failure the
+ *     message for the calling thread is available from cs_last_error().
+ *     No C++ exception ever crosses the ABI.
+ *   - A solver is not re-entrant: one thread at a time per cs_solver, exactly
+ *     like the synchronous single-threaded CpuSolver.
+ *   - Layouts follow the reference ClothScene / ClothMeshData
+ *     (src/physics/scene.ts:5-14, src/mesh/mesh.ts:3-14): interleaved xyz
+ *     (3n doubles), triangle indices (3m u32), invDm as [a,b,c,d] per triangle
+ *     and areas/masses as float32 exactly as the reference stores them.
+ */
+#ifndef CLOTH_HIP_H
+#define CLOTH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_ABI_VERSION 1
+
+/* Opaque handle: one simulated ClothScene resident on one GPU. */
+typedef struct cs_solver cs_solver;
+
+/* ---- status codes (thrown JS Errors in the reference) ------------------ */
+enum {
+  CS_OK = 0,
+  CS_E_NOT_INIT = 1,    /* "CpuSolver not initialized" (cpu-solver.ts:16,22,35,41,46) */
+  CS_E_INVALID = 2,     /* bad argument / malformed scene */
+  CS_E_NONFINITE = 3,   /* "non-finite gradient — blow up; reduce dt/stiffness" (newton.ts:129) */
+  CS_E_HIP = 4,         /* HIP runtime error (no device, launch failure, ...) */
+  CS_E_OVERFLOW = 5,    /* a bounded device buffer could not be grown */
+  CS_E_NOMEM = 6,
+  CS_E_UNSUPPORTED = 7
+};
+
+/* ---- solver configuration ------------------------------------------------ */
+enum { CS_PREC_FP64 = 0, CS_PREC_MIXED = 1 };
+/* CS_HVP_FD: the reference CpuSolver operator (central FD of the membrane
+ * gradient, src/physics/fem.ts:108-130).  CS_HVP_ANALYTIC: the analytic
+ * membrane HVP (src/physics/membrane-hvp.ts:18-108). */
+enum { CS_HVP_FD = 0, CS_HVP_ANALYTIC = 1 };
+/* pcg() preconditioner (src/math/pcg.ts:11-95 `applyPreconditioner`). */
+enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 2 };
+
+#define CS_FLAG_TIMING 1u   /* record per-kernel HIP event timings (cs_kernel_timing) */
+
+/* ClothMaterial (src/physics/types.ts:1-14). */
+typedef struct cs_material {
+  double areal_density;    /* arealDensityKgM2 */
+  double thickness;
+  double stretch_warp;     /* C00 */
+  double stretch_weft;     /* C11 */
+  double stretch_coupling; /* C01 */
+  double shear;            /* G */
+  double bend_warp;
+  double bend_weft;
+  double damping;
+} cs_material;
+
+/* ContactParams (src/collision/types.ts:27-39). */
+typedef struct cs_contact_params {
+  double d_hat;         /* dHatM */
+  double kappa;         /* kappaJ */
+  double d_min;         /* dMinM */
+  double friction_mu;   /* frictionMu */
+  double friction_eps;  /* frictionEpsM */
+} cs_contact_params;
+
+/* ClothScene (src/physics/scene.ts:5-14) + ClothMeshData (src/mesh/mesh.ts:3-14)
+ * + the ContactSystem configuration it carries (contact-assembly.ts:46-73). */
+typedef struct cs_scene_desc {
+  uint32_t n_verts;
+  uint32_t n_tris;
+  uint32_t n_hinges;
+  const double* positions;        /* 3n, scene.positions */
+  const double* velocities;       /* 3n, scene.velocities */
+  const uint32_t* indices;        /* 3m */
+  const float* inv_dm;            /* 4m  [a,b,c,d] */
+  const float* areas;             /* m */
+  const float* masses;            /* n  lumped */
+  const float* rest_positions;    /* 3n mesh.restPositions (NULL: use positions) */
+  const uint32_t* hinge_verts;    /* 4h (v0,v1 shared edge; v2,v3 opposite) */
+  const double* hinge_rest_angle; /* h */
+  const double* hinge_edge_len;   /* h */
+  const double* hinge_area_sum;   /* h */
+  cs_material material;
+  double gravity[3];
+  uint32_t n_pins;
+  const uint32_t* pin_ids;        /* scene.pinned keys (insertion order) */
+  const double* pin_pos;          /* 3 * n_pins */
+  int32_t contact_enabled;        /* scene.contact != null */
+  cs_contact_params contact;
+  int32_t has_floor;              /* ContactSystem.floorY != null */
+  double floor_y;
+  uint32_t n_static_verts;        /* ContactSystem.setStaticMesh */
+  uint32_t n_static_tris;
+  const float* static_pos;        /* 3 * n_static_verts */
+  const uint32_t* static_idx;     /* 3 * n_static_tris */
+} cs_scene_desc;
+
+/* StepperOptions (src/solver/newton.ts:20-25) + backend selection. */
+typedef struct cs_options {
+  int32_t newton_iters;       /* default 10 */
+  int32_t pcg_max_iters;      /* default 60 */
+  int32_t line_search_iters;  /* default 10 */
+  double grad_tol;            /* default 1e-5 (absolute) */
+  int32_t precision;          /* CS_PREC_* (default FP64) */
+  int32_t hvp_mode;           /* CS_HVP_* (default FD = CpuSolver operator) */
+  int32_t precond;            /* CS_PRECOND_* (default Jacobi = CpuSolver) */
+  int32_t schwarz_target;     /* buildSchwarzDomains(target, min, max) */
+  int32_t schwarz_min;
+  int32_t schwarz_max;
+  int32_t device;             /* HIP device ordinal */
+  uint32_t flags;             /* CS_FLAG_* */
+} cs_options;
+
+/* ContactDiagnostics (src/collision/types.ts:49-61). */
+typedef struct cs_contact_diag {
+  uint64_t candidate_pairs;
+  uint64_t active_pairs;
+  uint64_t vt_pairs;
+  uint64_t ee_pairs;
+  uint64_t floor_pairs;
+  double min_distance;
+  double min_toi;
+  double barrier_energy;
+  double friction_work;
+  uint64_t rejected_line_search_steps;
+  uint64_t ccd_failures;
+} cs_contact_diag;
+
+/* StepStats (src/solver/newton.ts:10-18) plus a few counters the reference
+ * does not report (newton_used, line_search_trials). */
+typedef struct cs_step_stats {
+  int32_t newton_iters;       /* configured maximum, as the reference reports (newton.ts:231) */
+  int32_t pcg_iters;          /* total PCG iterations in the step */
+  double grad_norm;
+  double energy;
+  double max_strain;
+  int32_t converged;
+  int32_t has_contact;
+  cs_contact_diag contact;
+  int32_t newton_used;        /* Newton iterations actually entered */
+  int32_t line_search_trials; /* Armijo trials evaluated */
+} cs_step_stats;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+
+/* Fill `opt` with the CpuSolver defaults (newton.ts:30-33; FP64, FD HVP, Jacobi). */
+void cs_options_default(cs_options* opt);
+
+/* ClothSolver.initialize(scene) — cpu-solver.ts:11-13.  Copies the scene
+ * (mesh, material, pins, contact configuration) to the device. */
+int cs_create(const cs_scene_desc* scene, const cs_options* opt, cs_solver** out);
+/* (no reference counterpart: releases device memory) */
+void cs_destroy(cs_solver* s);
+
+/* ClothSolver.step(dt) — cpu-solver.ts:15-18 → implicitStep (newton.ts:27-232).
+ * Synchronous.  CS_E_NONFINITE mirrors the reference's non-finite gradient throw. */
+int cs_step(cs_solver* s, double dt);
+
+/* ClothSolver.setMaterial — cpu-solver.ts:20-23 (copies the material). */
+int cs_set_material(cs_solver* s, const cs_material* mat);
+
+/* ClothSolver.pinVertex — cpu-solver.ts:25-33.  pos == NULL pins at the current
+ * position; pins are enforced on the positions immediately. */
+int cs_pin_vertex(cs_solver* s, uint32_t vertex_id, const double* pos_or_null);
+
+/* ClothSolver.unpinVertex — cpu-solver.ts:35-37. */
+int cs_unpin_vertex(cs_solver* s, uint32_t vertex_id);
+
+/* ClothSolver.getPositions / getVelocities — cpu-solver.ts:39-48.  Copy-out of
+ * 3n doubles (the host wrapper writes them into the live scene arrays). */
+int cs_get_positions(cs_solver* s, double* out, size_t n3);
+int cs_get_velocities(cs_solver* s, double* out, size_t n3);
+
+/* Overwrite scene.positions / scene.velocities (checkpoint restore; the
+ * reference mutates these arrays directly, scene.ts:10-11). */
+int cs_set_state(cs_solver* s, const double* x, const double* v, size_t n3);
+
+/* CpuSolver.lastStats — cpu-solver.ts:9. */
+int cs_last_stats(const cs_solver* s, cs_step_stats* out);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* cs_last_error(void);
+
+/* Device name / ABI version (diagnostics only). */
+int cs_device_info(int device, char* name, size_t cap);
+int cs_abi_version(void);
+
+/* ---- measurement hooks (no reference counterpart) ------------------------ */
+enum {
+  CS_KERNEL_HVP = 0,       /* one membrane(+mass+contact) Hessian-vector product */
+  CS_KERNEL_GRADIENT = 1,  /* Newton gradient assembly */
+  CS_KERNEL_BROADPHASE = 2,/* candidate generation (LBVH build + traverse) */
+  CS_KERNEL_CCD = 3,       /* trial validation (distances + CCD) */
+  CS_KERNEL_PRECOND = 4,   /* preconditioner apply */
+  CS_KERNEL_COUNT = 5
+};
+/* Accumulated HIP-event time (ms) and launch count of a kernel family since the
+ * last reset; only recorded when CS_FLAG_TIMING is set.  `bytes` receives the
+ * algorithmic bytes per launch for that family (0 when not defined). */
+int cs_kernel_timing(const cs_solver* s, int kernel, double* total_ms, uint64_t* count, double* bytes);
+int cs_reset_timing(cs_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOTH_HIP_H */
