@@ -35,10 +35,54 @@ def js_hypot3(a, b, c):
     return np.where(np.isinf(mx), np.inf, out)
 
 
+_PS = (1.66666666666666657415e-01, -3.25565818622400915405e-01, 2.01212532134862925881e-01,
+       -4.00555345006794114027e-02, 7.91534994289814532176e-04, 3.47933107596021167570e-05)
+_QS = (-2.40339491173441421878e+00, 2.02094576023350569471e+00, -6.88283971605453293030e-01,
+       7.70381505559019352791e-02)
+
+
+def _pq(z):
+    p = z * (_PS[0] + z * (_PS[1] + z * (_PS[2] + z * (_PS[3] + z * (_PS[4] + z * _PS[5])))))
+    q = 1.0 + z * (_QS[0] + z * (_QS[1] + z * (_QS[2] + z * _QS[3])))
+    return p / q
+
+
 def _acos(x):
-    # C-library acos per element (the same routine the oracle calls), not a
-    # vectorised approximation.
-    return np.fromiter((math.acos(v) for v in x.tolist()), np.float64, len(x))
+    """Math.acos as V8 computes it (fdlibm e_acos.c), vectorised with the same
+    IEEE operations as the device code, so rest angles are bit-identical."""
+    x = np.ascontiguousarray(x, np.float64)
+    u = x.view(np.uint64)
+    hx = (u >> np.uint64(32)).astype(np.int64)
+    hx = np.where(hx >= 2 ** 31, hx - 2 ** 32, hx)
+    ix = hx & 0x7fffffff
+    lo = (u & np.uint64(0xffffffff)).astype(np.int64)
+    pi, pio2_hi, pio2_lo = 3.14159265358979311600e+00, 1.57079632679489655800e+00, 6.12323399573676603587e-17
+    out = np.full_like(x, np.nan)
+    with np.errstate(all="ignore"):
+        one = (ix == 0x3ff00000) & (lo == 0)
+        out[one & (hx > 0)] = 0.0
+        out[one & (hx <= 0)] = pi + 2.0 * pio2_lo
+        small = ix < 0x3fe00000
+        tiny = small & (ix <= 0x3c600000)
+        out[tiny] = pio2_hi + pio2_lo
+        s1 = small & ~tiny
+        xs = x[s1]
+        r = _pq(xs * xs)
+        out[s1] = pio2_hi - (xs - (pio2_lo - xs * r))
+        mid = (ix >= 0x3fe00000) & (ix < 0x3ff00000)
+        neg = mid & (hx < 0)
+        z = (1.0 + x[neg]) * 0.5
+        s = np.sqrt(z)
+        w = _pq(z) * s - pio2_lo
+        out[neg] = pi - 2.0 * (s + w)
+        pos = mid & (hx >= 0)
+        z = (1.0 - x[pos]) * 0.5
+        s = np.sqrt(z)
+        df = (s.view(np.uint64) & np.uint64(0xffffffff00000000)).view(np.float64)
+        c = (z - df * df) / (s + df)
+        w = _pq(z) * s + c
+        out[pos] = 2.0 * (df + w)
+    return out
 
 
 def dihedral(x, v0, v1, v2, v3):

@@ -8,15 +8,19 @@
 // written in the same operand order as the TypeScript (JS evaluates + and *
 // left to right exactly like C++), compiled without FP contraction or
 // reassociation, so every elementwise result is the IEEE-754 double the
-// reference computes.  Math.hypot is restated with V8's scaled Kahan sum;
-// log/acos/cbrt/cos come from the C library (V8 uses fdlibm ports of the same
-// functions; they may differ by one ulp).
+// reference computes.  Math.hypot/log/acos/cbrt/cos are restated from the
+// algorithms V8 implements them with (jsmath.h).  Every reduction over a whole
+// array (dot products, norms, energy totals) is returned correctly rounded
+// (double-double accumulation, one final rounding): the reference's naive
+// left-to-right sum differs from it by at most its own rounding error, and
+// the correctly rounded value does not depend on summation order.
 //
 // Parity status of each part is recorded in tests/test_oracle_*.py, which pin
 // it against the reference's own tests (known answers, FD checks, invariants)
 // and against brute force / numpy where those tests leave it open.
 
 #include "cloth_oracle.h"
+#include "jsmath.h"
 
 #include <algorithm>
 #include <array>
@@ -48,34 +52,7 @@ inline double jmin(double a, double b) {
   return a < b ? a : b;
 }
 
-// V8 Math.hypot (src/builtins/math.tq): scale by the largest magnitude, Kahan
-// summation of the squares, sqrt(sum) * max.
-double hypot3(double a, double b, double c) {
-  const double v[3] = {a, b, c};
-  double ab[3] = {0, 0, 0};
-  bool nan = false;
-  double mx = 0;
-  for (int i = 0; i < 3; i++) {
-    if (std::isnan(v[i])) {
-      nan = true;
-    } else {
-      ab[i] = std::fabs(v[i]);
-      if (ab[i] > mx) mx = ab[i];
-    }
-  }
-  if (mx == INF) return INF;
-  if (nan) return NaN;
-  if (mx == 0) return 0;
-  double sum = 0, comp = 0;
-  for (int i = 0; i < 3; i++) {
-    const double n = ab[i] / mx;
-    const double summand = n * n - comp;
-    const double prelim = sum + summand;
-    comp = (prelim - sum) - summand;
-    sum = prelim;
-  }
-  return std::sqrt(sum) * mx;
-}
+inline double hypot3(double a, double b, double c) { return orcm::hypot3(a, b, c); }
 
 struct Mat {
   double rho, thick, c00, c11, c01, g, bw, bf, damp;
@@ -121,7 +98,7 @@ double dihedral(const T* x, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) 
   const double l1 = hypot3(n1[0], n1[1], n1[2]) + 1e-30;
   const double l2 = hypot3(n2[0], n2[1], n2[2]) + 1e-30;
   const double cosT = jmin(1, jmax(-1, (n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2]) / (l1 * l2)));
-  return std::acos(cosT);
+  return orcm::acos(cosT);
 }
 
 // buildHinges() mesh.ts:99-126.  Map insertion order == first occurrence of
@@ -370,13 +347,14 @@ void eval_internal(const Mesh& mesh, const Mat& mat, const double* x, double* gr
                    double* max_strain_out) {
   const size_t n3 = (size_t)mesh.n * 3;
   std::fill(grad, grad + n3, 0.0);
-  double energy = 0, maxStrain = 0;
+  orcm::dd energy = orcm::dd_zero();
+  double maxStrain = 0;
   for (size_t t = 0; t < mesh.m; t++) {
     const uint32_t i0 = mesh.idx[t * 3], i1 = mesh.idx[t * 3 + 1], i2 = mesh.idx[t * 3 + 2];
     double x0[3], x1[3], x2[3], inv[4];
     load3(x, i0, x0); load3(x, i1, x1); load3(x, i2, x2); inv4(mesh, t, inv);
     const TriGrad r = tri_energy_grad(x0, x1, x2, inv, mesh.areas[t], mat);
-    energy += r.energy;
+    orcm::dd_add(energy, r.energy);
     maxStrain = jmax(jmax(jmax(maxStrain, std::fabs(r.E00)), std::fabs(r.E11)), std::fabs(r.E01));
     const uint32_t vs[3] = {i0, i1, i2};
     for (int v = 0; v < 3; v++)
@@ -385,26 +363,27 @@ void eval_internal(const Mesh& mesh, const Mat& mat, const double* x, double* gr
   const double kb = 0.5 * (mat.bw + mat.bf);
   if (kb > 0) {
     std::vector<double> tmp(x, x + n3);
-    for (const Hinge& h : mesh.hinges) energy += add_hinge_gradient(tmp, grad, h, kb);
+    for (const Hinge& h : mesh.hinges) orcm::dd_add(energy, add_hinge_gradient(tmp, grad, h, kb));
   }
-  if (energy_out) *energy_out = energy;
+  if (energy_out) *energy_out = orcm::dd_value(energy);
   if (max_strain_out) *max_strain_out = maxStrain;
 }
 
 // internalEnergyOnly() fem.ts:56-75
 double internal_energy(const Mesh& mesh, const Mat& mat, const double* x) {
-  double e = 0;
+  orcm::dd e = orcm::dd_zero();
   for (size_t t = 0; t < mesh.m; t++) {
     const uint32_t i0 = mesh.idx[t * 3], i1 = mesh.idx[t * 3 + 1], i2 = mesh.idx[t * 3 + 2];
     double x0[3], x1[3], x2[3], inv[4];
     load3(x, i0, x0); load3(x, i1, x1); load3(x, i2, x2); inv4(mesh, t, inv);
-    e += tri_energy_grad(x0, x1, x2, inv, mesh.areas[t], mat).energy;
+    orcm::dd_add(e, tri_energy_grad(x0, x1, x2, inv, mesh.areas[t], mat).energy);
   }
   const double kb = 0.5 * (mat.bw + mat.bf);
   if (kb > 0) {
-    for (const Hinge& h : mesh.hinges) e += hinge_energy(x, h.v0, h.v1, h.v2, h.v3, h.rest, h.len, h.asum, kb);
+    for (const Hinge& h : mesh.hinges)
+      orcm::dd_add(e, hinge_energy(x, h.v0, h.v1, h.v2, h.v3, h.rest, h.len, h.asum, kb));
   }
-  return e;
+  return orcm::dd_value(e);
 }
 
 // evalMembrane() fem.ts:78-104 (gradient only; energy unused by callers here).
@@ -425,9 +404,9 @@ void eval_membrane(const Mesh& mesh, const Mat& mat, const double* x, double* gr
 void membrane_hvp_fd(const Mesh& mesh, const Mat& mat, const double* x, const double* p, double* out) {
   const size_t n3 = (size_t)mesh.n * 3;
   const double eps = 1e-8;
-  double pNorm = 0;
-  for (size_t i = 0; i < n3; i++) pNorm += p[i] * p[i];
-  pNorm = std::sqrt(pNorm);
+  orcm::dd pn = orcm::dd_zero();
+  for (size_t i = 0; i < n3; i++) orcm::dd_add_prod(pn, p[i], p[i]);
+  const double pNorm = std::sqrt(orcm::dd_value(pn));
   const double h = eps * (1 + pNorm);
   std::vector<double> xp(n3), xm(n3), gp(n3), gm(n3);
   for (size_t i = 0; i < n3; i++) {
@@ -1020,20 +999,20 @@ int solve_cubic01(double a3, double a2, double a1, double a0, double* res) {
   const double shift = A / 3;
   if (disc > eps) {
     const double sd = std::sqrt(disc);
-    const double u = std::cbrt(-q / 2 + sd);
-    const double v = std::cbrt(-q / 2 - sd);
+    const double u = orcm::cbrt(-q / 2 + sd);
+    const double v = orcm::cbrt(-q / 2 - sd);
     roots[nr++] = u + v - shift;
   } else if (std::fabs(disc) <= eps) {
-    const double u = std::cbrt(-q / 2);
+    const double u = orcm::cbrt(-q / 2);
     roots[nr++] = 2 * u - shift;
     roots[nr++] = -u - shift;
   } else {
     const double r = std::sqrt(-(p * p * p) / 27);
-    const double phi = std::acos(jmin(1, jmax(-1, -q / (2 * r))));
-    const double s = 2 * std::cbrt(r);
-    roots[nr++] = s * std::cos(phi / 3) - shift;
-    roots[nr++] = s * std::cos((phi + 2 * M_PI) / 3) - shift;
-    roots[nr++] = s * std::cos((phi + 4 * M_PI) / 3) - shift;
+    const double phi = orcm::acos(jmin(1, jmax(-1, -q / (2 * r))));
+    const double s = 2 * orcm::cbrt(r);
+    roots[nr++] = s * orcm::cos(phi / 3) - shift;
+    roots[nr++] = s * orcm::cos((phi + 2 * M_PI) / 3) - shift;
+    roots[nr++] = s * orcm::cos((phi + 4 * M_PI) / 3) - shift;
   }
   double out[3];
   int no = 0;
@@ -1210,13 +1189,13 @@ double barrier_value(double d, double dHat) {
   if (d >= dHat) return 0;
   const double dc = jmax(d, 1e-12);
   const double t = dc - dHat;
-  return -t * t * std::log(dc / dHat);
+  return -t * t * orcm::log(dc / dHat);
 }
 double barrier_grad_scalar(double d, double dHat, double kappa, double w) {
   if (d >= dHat) return 0;
   const double dc = jmax(d, 1e-12);
   const double t = dc - dHat;
-  const double db = -2 * t * std::log(dc / dHat) - (t * t) / dc;
+  const double db = -2 * t * orcm::log(dc / dHat) - (t * t) / dc;
   const double r = kappa * w * db;
   return g_fault == ORC_FAULT_BARRIER_SIGN ? -r : r;
 }
@@ -1548,11 +1527,11 @@ struct Contact {
   double energy_grad(const double* x, double* grad, size_t n3) {
     const double dHat = params.d_hat, kappa = params.kappa, mu = params.friction_mu, eps = params.friction_eps;
     std::fill(grad, grad + n3, 0.0);
-    double energy = 0, fwork = 0;
+    orcm::dd energy = orcm::dd_zero(), fwork = orcm::dd_zero();
     for (const Frozen& c : active) {
       const double d = jmax(frozen_dist(x, c), 1e-12);
       if (d >= dHat) continue;
-      energy += kappa * barrier_value(d, dHat);
+      orcm::dd_add(energy, kappa * barrier_value(d, dHat));
       const double g = barrier_grad_scalar(d, dHat, kappa, 1);
       add_dist_grad(grad, c, g);
       auto it = lagged.find(c.key);
@@ -1564,11 +1543,11 @@ struct Contact {
       relative_slip(x, c, u);
       coulomb_force(u[0], u[1], u[2], nx, ny, nz, lambdaN, mu, eps, f);
       add_friction_force(grad, c, -f[0], -f[1], -f[2]);
-      fwork += f[0] * u[0] + f[1] * u[1] + f[2] * u[2];
+      orcm::dd_add(fwork, f[0] * u[0] + f[1] * u[1] + f[2] * u[2]);
     }
-    diag.barrier_energy = energy;
-    diag.friction_work += fwork;
-    return energy;
+    diag.barrier_energy = orcm::dd_value(energy);
+    diag.friction_work += orcm::dd_value(fwork);
+    return diag.barrier_energy;
   }
 
   // distJvp() :394-414
@@ -1606,7 +1585,7 @@ struct Contact {
       const Frozen& c = active[i];
       const double d = jmax(frozen_dist(x, c), 1e-12);
       if (d >= dHat) { dd[i] = 0; continue; }
-      const double L = std::log(d / dHat);
+      const double L = orcm::log(d / dHat);
       const double u = d - dHat;
       const double b2 = -(2 * L + (4 * u) / d - (u * u) / (d * d));
       const double coeff = kappa * jmax(b2, 0);
@@ -1621,13 +1600,13 @@ struct Contact {
   // meritEnergy() :419-428
   double merit_energy(const double* x) const {
     const double dHat = params.d_hat, kappa = params.kappa;
-    double e = 0;
+    orcm::dd e = orcm::dd_zero();
     for (const Frozen& c : active) {
       const double d = jmax(frozen_dist(x, c), 1e-12);
       if (d >= dHat) continue;
-      e += kappa * barrier_value(d, dHat);
+      orcm::dd_add(e, kappa * barrier_value(d, dHat));
     }
-    return e;
+    return orcm::dd_value(e);
   }
 
   // addDiagEstimate() :436-471
@@ -1636,7 +1615,7 @@ struct Contact {
     for (const Frozen& c : active) {
       const double d = jmax(frozen_dist(x, c), 1e-12);
       if (d >= dHat) continue;
-      const double L = std::log(d / dHat);
+      const double L = orcm::log(d / dHat);
       const double u = d - dHat;
       const double b2 = jmax(-(2 * L + (4 * u) / d - (u * u) / (d * d)), 0);
       const double k = kappa * b2;
@@ -1791,9 +1770,14 @@ struct PinFilter {
 };
 
 double dot_self(const std::vector<double>& a) {
-  double s = 0;
-  for (double v : a) s += v * v;
-  return s;
+  orcm::dd s = orcm::dd_zero();
+  for (double v : a) orcm::dd_add_prod(s, v, v);
+  return orcm::dd_value(s);
+}
+double dot_cr(const double* a, const double* b, size_t n) {
+  orcm::dd s = orcm::dd_zero();
+  for (size_t i = 0; i < n; i++) orcm::dd_add_prod(s, a[i], b[i]);
+  return orcm::dd_value(s);
 }
 
 // pcg() pcg.ts:11-95 (x0 = 0).  `precond` null => Jacobi on diag.
@@ -1819,8 +1803,7 @@ PcgResult pcg(const std::vector<double>& b, Hvp hvp, const std::vector<double>& 
   };
   apply_m();
   p = z;
-  double rz = 0;
-  for (size_t i = 0; i < n; i++) rz += r[i] * z[i];
+  double rz = dot_cr(r.data(), z.data(), n);
   const double bNorm = std::sqrt(dot_self(b));
   const double absTol = jmax(tol * jmax(bNorm, 1e-12), 1e-12);
   double residual = std::sqrt(dot_self(r));
@@ -1829,8 +1812,7 @@ PcgResult pcg(const std::vector<double>& b, Hvp hvp, const std::vector<double>& 
     filter(p.data());
     hvp(p.data(), Ap.data());
     filter(Ap.data());
-    double pAp = 0;
-    for (size_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+    const double pAp = dot_cr(p.data(), Ap.data(), n);
     if (!(pAp > 1e-30)) return {x, k, residual, residual < absTol};
     const double alpha = rz / pAp;
     for (size_t i = 0; i < n; i++) {
@@ -1842,8 +1824,7 @@ PcgResult pcg(const std::vector<double>& b, Hvp hvp, const std::vector<double>& 
     residual = std::sqrt(dot_self(r));
     if (residual < absTol) return {x, k + 1, residual, true};
     apply_m();
-    double rzNew = 0;
-    for (size_t i = 0; i < n; i++) rzNew += r[i] * z[i];
+    const double rzNew = dot_cr(r.data(), z.data(), n);
     const double beta = rzNew / rz;
     for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
     rz = rzNew;
@@ -1964,14 +1945,15 @@ cs_step_stats implicit_step(Solver& sc, double h) {
     }
   }
   auto totalEnergy = [&](const double* xx) {
-    double e = 0;
+    orcm::dd in = orcm::dd_zero();
     for (size_t i = 0; i < n; i++) {
       const double m = M[i];
       for (int k = 0; k < 3; k++) {
         const double d = xx[i * 3 + k] - yHat[i * 3 + k];
-        e += 0.5 * m * d * d / (h * h);
+        orcm::dd_add(in, 0.5 * m * d * d / (h * h));
       }
     }
+    double e = orcm::dd_value(in);
     e += internal_energy(sc.mesh, sc.mat, xx);
     if (contact) e += contact->merit_energy(xx);
     return e;
@@ -2009,11 +1991,7 @@ cs_step_stats implicit_step(Solver& sc, double h) {
       contact->add_diag_estimate(diag.data(), x.data());
     }
     filter(g.data());
-    {
-      double s = 0;
-      for (double v : g) s += v * v;
-      gradNorm = std::sqrt(s);
-    }
+    gradNorm = std::sqrt(dot_self(g));
     if (!std::isfinite(gradNorm)) throw NonFinite{};
     if (gradNorm < gradTol) { converged = true; break; }
     std::vector<double> b(n3);
@@ -2046,12 +2024,10 @@ cs_step_stats implicit_step(Solver& sc, double h) {
     }
     std::vector<double>& dx = res.x;
     pcgItersTotal += res.iters;
-    double gtdx = 0;
-    for (size_t i = 0; i < n3; i++) gtdx += g[i] * dx[i];
+    double gtdx = dot_cr(g.data(), dx.data(), n3);
     if (!std::isfinite(gtdx) || gtdx >= 0) {
       for (size_t i = 0; i < n3; i++) dx[i] = -g[i] / diag[i];
-      gtdx = 0;
-      for (size_t i = 0; i < n3; i++) gtdx += g[i] * dx[i];
+      gtdx = dot_cr(g.data(), dx.data(), n3);
     }
     {
       double maxDx = 0;
