@@ -1,0 +1,42 @@
+// contact.h — device-resident ContactSystem (contact-assembly.ts:46-595).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/cloth_hip.h"
+#include "pcg.h"
+
+namespace cloth {
+
+struct OverflowError : std::runtime_error {
+  explicit OverflowError(const std::string& s) : std::runtime_error(s) {}
+};
+
+struct ContactImpl;
+
+class ContactSystem {
+ public:
+  ContactSystem(const cs_scene_desc& d, hipStream_t st);
+  ~ContactSystem();
+  void begin_step(const double* x0);                       // beginStep :75-81
+  bool check_trial(const double* xt);                      // checkTrial :489-553
+  void update_active_set(const double* x);                 // updateActiveSet :164-220
+  void energy_grad(const double* x, double* ceg);          // energyGrad :262-285
+  void add_diag_estimate(double* diag, const double* x);   // addDiagEstimate :436-471
+  double merit_energy(const double* x);                    // meritEnergy :419-428
+  void add_rejection();                                    // :83
+  void commit(const double* x);                            // commit :475-485
+  void kill_approach_velocity(const double* x, double* v); // newton.ts:238-303
+  cs_contact_diag diagnostics() const;
+  DContactHvp hvp_hook();
+  // dd[c] = coeff[c] * (J_c . p) for the current PCG direction.
+  void launch_hvp_coeffs(const double* p, const int32_t* done);
+  uint64_t active_count() const;
+  ContactImpl* impl() { return im_; }
+
+ private:
+  ContactImpl* im_;
+};
+
+}  // namespace cloth

@@ -1,0 +1,75 @@
+// fem.h — element kernels of the cloth step (device side) and their launchers.
+//
+// Layout in HBM (one cloth instance):
+//   x, p, ...      interleaved xyz doubles (3n), the reference's layout
+//   idx            u32 [3m] triangle corners
+//   inv_dm         f32 [4m] [a,b,c,d] (the reference stores Float32)
+//   area           f32 [m];  mass f32 [n]
+//   vt_off/vt_rec  CSR vertex -> (triangle << 2 | corner), ascending triangle
+//   hv, h*         hinge corners u32 [4h] + rest/len/areaSum doubles
+//   vh_off/vh_rec  CSR vertex -> (hinge << 2 | corner), ascending hinge
+// Every per-vertex sum walks its CSR list in element order, which reproduces
+// the reference's element-order scatter (fem.ts:38-41, bending.ts:29-39)
+// bit for bit without scratch arrays or atomics.
+#pragma once
+
+#include "common.h"
+
+namespace cloth {
+
+struct DMesh {
+  uint32_t n, m, nh;
+  const uint32_t* idx;
+  const float* inv_dm;
+  const float* area;
+  const float* mass;
+  const uint32_t* vt_off;
+  const uint32_t* vt_rec;
+  const uint32_t* hv;
+  const double* hrest;
+  const double* hlen;
+  const double* hasum;
+  const uint32_t* vh_off;
+  const uint32_t* vh_rec;
+  const uint8_t* pinmask;  // 1 = vertex is pinned (makePinFilter, scene.ts:46-55)
+};
+
+struct DMat {
+  double thick, c00, c11, c01, g, kb;  // kb = 0.5 * (bendWarp + bendWeft)
+};
+
+// Float32 mesh mirror for the mixed-precision inner solve.
+struct DMesh32 {
+  uint32_t n, m;
+  const uint32_t* idx;
+  const float* inv_dm;
+  const float* area;
+  const float* mass;
+  const uint32_t* vt_off;
+  const uint32_t* vt_rec;
+  const uint8_t* pinmask;
+};
+
+struct Scalars;  // device scalar block (solver.hip)
+
+// g = M (x - yHat) / h^2 + grad_membrane + grad_bending (+ ceg), pins zeroed;
+// result[slot] = correctly rounded sum of g^2.
+void launch_gradient(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
+                     double inv_h2, const double* ceg, double* g, jsm::dd* partials, unsigned* ticket,
+                     double* out_sumsq);
+
+// out = M p / h^2 + H_membrane p (+ contact), pins zeroed, for PCG iteration;
+// the FD operator reads h from *h_fd; partial sums of p.out are reduced into
+// the PCG scalars.  See pcg.hip for the driver.
+enum HvpMode { HVP_FD = 0, HVP_ANALYTIC = 1 };
+
+// Energies at x: inertia sum and internal (membrane + bending) sum, each
+// correctly rounded; written to out2[0], out2[1].
+void launch_energy(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
+                   double h, jsm::dd* partials, unsigned* tickets, double* out2);
+
+// maxStrain over triangles (JS Math.max semantics).
+void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, unsigned* ticket,
+                       double* out);
+
+}  // namespace cloth

@@ -1,0 +1,79 @@
+// pcg.h — device-resident preconditioned CG (pcg.ts:11-95) for the cloth step.
+#pragma once
+
+#include "fem.h"
+
+namespace cloth {
+
+// Frozen active contact (contact-assembly.ts:22-30) as stored on the device,
+// in the reference's active-list order.
+struct Frozen {
+  int32_t kind, p, a, b, c, d;  // 0 VT, 1 EE, 2 floor; negative ids = static vertices
+  double w0, w1, w2, w3, nx, ny, nz;
+};
+
+// Contact Hessian-vector hook: dd[c] = kappa max(b'', 0) (J_c . v) per active
+// contact, gathered per vertex through a CSR in active-list order.
+struct DContactHvp {
+  uint32_t C = 0;
+  const Frozen* act = nullptr;
+  const double* coeff = nullptr;    // kappa * max(b''(d), 0), 0 when d >= dHat
+  double* dd = nullptr;             // per contact, per PCG iteration
+  const uint32_t* v_off = nullptr;  // n+1
+  const uint32_t* v_ent = nullptr;  // contact << 3 | slot
+};
+
+// Device scalars of one PCG solve.
+struct PcgScalars {
+  double rz, pAp, alpha, beta, residual, abs_tol, h_fd, bnorm;
+  int32_t done, iters;
+  unsigned tickets[8];
+};
+
+enum PrecondKind { PRE_JACOBI = 0, PRE_BLOCK = 1, PRE_EXTERNAL = 2 };
+
+struct PcgArgs {
+  DMesh me;
+  DMat mat;
+  double inv_h2;
+  const double* x;     // Newton iterate (HVP linearisation point)
+  const double* b;     // right-hand side (-g)
+  const double* diag;  // Jacobi diagonal
+  const double* binv;  // block-Jacobi inverses [9n] (PRE_BLOCK)
+  double* sol;         // PCG x (the Newton step)
+  double* r;
+  double* z;
+  double* p;
+  double* Ap;
+  int hvp_mode;        // HVP_FD / HVP_ANALYTIC
+  int precond;         // PrecondKind
+  double tol;
+  PcgScalars* s;
+  jsm::dd* partials;   // >= 4 * 1024
+  DContactHvp contact;
+};
+
+void pcg_init(hipStream_t st, const PcgArgs& a);
+// One PCG iteration k (0-based): HVP + alpha, update + preconditioner +
+// residual test + beta, direction update (+ |p| for the FD operator).
+void pcg_iteration(hipStream_t st, const PcgArgs& a, int k);
+// External preconditioner hook (Schwarz): z = M^-1 r must be produced between
+// pcg_update_x and pcg_finish_z.
+void pcg_iteration_split_a(hipStream_t st, const PcgArgs& a, int k);
+void pcg_iteration_split_b(hipStream_t st, const PcgArgs& a, int k);
+
+// Block-Jacobi factors (block-jacobi.ts:30-87) from the membrane vertex
+// blocks at x and the Jacobi diagonal.
+void launch_block_factors(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* diag,
+                          double* binv, uint8_t* flag);
+
+// Vector helpers used by the Newton driver.
+void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, unsigned* ticket,
+                double* out);
+void launch_max_abs(hipStream_t st, const double* a, size_t n, double* partials, unsigned* ticket, double* out);
+void launch_fallback_dir(hipStream_t st, const double* g, const double* diag, double* dx, size_t n);
+void launch_scale(hipStream_t st, double* v, double s, size_t n);
+void launch_trial(hipStream_t st, const double* x, const double* dx, double alpha, double* xt, size_t n);
+void launch_neg(hipStream_t st, const double* g, double* b, size_t n);
+
+}  // namespace cloth

@@ -1,0 +1,740 @@
+// solver.hip — HipSolver: the implicit cloth step on one MI355X behind the
+// C ABI of include/cloth_hip.h (drop-in for CpuSolver, cpu-solver.ts:7-49).
+//
+// Control flow follows implicitStep (newton.ts:27-232) on the host; all data
+// stays resident in HBM and only scalars cross PCIe (gradient norm, PCG
+// completion, g.dx, max |dx|, trial energies, contact counts/validity).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cloth_hip.h"
+#include "contact.h"
+#include "fem.h"
+#include "mixed.h"
+#include "pcg.h"
+
+namespace cloth {
+
+thread_local std::string g_last_error;
+
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    if (n <= count && ptr) return;
+    release();
+    if (n == 0) n = 1;
+    CS_HIP(hipMalloc(&ptr, n * sizeof(T)));
+    count = n;
+  }
+  void upload(const T* h, size_t n, hipStream_t st) {
+    alloc(n);
+    if (n) CS_HIP(hipMemcpyAsync(ptr, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void zero(hipStream_t st) { CS_HIP(hipMemsetAsync(ptr, 0, count * sizeof(T), st)); }
+};
+
+struct NewtonScalars {
+  PcgScalars pcg;
+  double grad_sumsq;
+  double e2[2];
+  double merit;
+  double gtdx;
+  double maxdx;
+  double max_strain;
+  unsigned tickets[16];
+};
+
+struct NonFinite {};
+
+__global__ void k_predictor(const double* x0, const double* v0, double h, double gx, double gy, double gz,
+                            double* yhat, double* x, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // yHat = x + h v + h^2 g (newton.ts:42-47)
+  const double g3[3] = {gx, gy, gz};
+  for (int k = 0; k < 3; k++) {
+    const size_t j = (size_t)i * 3 + k;
+    const double y = x0[j] + h * v0[j] + h * h * g3[k];
+    yhat[j] = y;
+    x[j] = y;
+  }
+}
+
+__global__ void k_enforce_pins(const uint32_t* ids, const double* pos, uint32_t np, double* x) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const size_t v = ids[k];
+  x[v * 3] = pos[k * 3];
+  x[v * 3 + 1] = pos[k * 3 + 1];
+  x[v * 3 + 2] = pos[k * 3 + 2];
+}
+
+__global__ void k_init_diag(const float* mass, double inv_h2, double beta, double* diag, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = mass[i] * inv_h2 + beta;  // newton.ts:103-109
+  diag[(size_t)i * 3] = d;
+  diag[(size_t)i * 3 + 1] = d;
+  diag[(size_t)i * 3 + 2] = d;
+}
+
+__global__ void k_velocity(const double* x, const double* x0, double h, double damp, double* v, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) v[i] = ((x[i] - x0[i]) / h) * damp;  // newton.ts:222-226
+}
+
+struct Solver {
+  int device = 0;
+  hipStream_t st = nullptr;
+  cs_options opt;
+  uint32_t n = 0, m = 0, nh = 0;
+  cs_material mat;
+  double gravity[3];
+  bool has_contact = false;
+  cs_step_stats last;
+  bool has_last = false;
+
+  // mesh
+  DevBuf<uint32_t> idx, vt_off, vt_rec, hv, vh_off, vh_rec;
+  DevBuf<float> inv_dm, area, mass;
+  DevBuf<double> hrest, hlen, hasum;
+  // state
+  DevBuf<double> pos, vel, x0, yhat, x, xt, g, b, dx, diag, ceg;
+  // pcg
+  DevBuf<double> r, z, p, Ap, binv;
+  DevBuf<jsm::dd> partials;
+  DevBuf<double> mpartials;
+  DevBuf<NewtonScalars> scal;
+  NewtonScalars* hscal = nullptr;  // pinned host mirror
+  // pins (Map insertion order)
+  std::vector<uint32_t> pin_ids;
+  std::vector<double> pin_pos;
+  DevBuf<uint32_t> d_pin_ids;
+  DevBuf<double> d_pin_pos;
+  DevBuf<uint8_t> pinmask;
+  bool pins_dirty = true;
+  // subsystems
+  std::unique_ptr<ContactSystem> contact;
+  std::unique_ptr<MixedSolve> mixed;
+  // timing
+  hipEvent_t ev_pool[2 * 4096];
+  int ev_used = 0;
+  bool ev_init = false;
+  double kt_ms[CS_KERNEL_COUNT] = {0};
+  uint64_t kt_count[CS_KERNEL_COUNT] = {0};
+  std::vector<std::pair<int, int>> ev_pending[CS_KERNEL_COUNT];
+
+  ~Solver() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (hscal) (void)hipHostFree(hscal);
+    if (ev_init)
+      for (auto& e : ev_pool) (void)hipEventDestroy(e);
+    contact.reset();
+    mixed.reset();
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  DMesh dmesh() const {
+    DMesh d;
+    d.n = n; d.m = m; d.nh = nh;
+    d.idx = idx.ptr; d.inv_dm = inv_dm.ptr; d.area = area.ptr; d.mass = mass.ptr;
+    d.vt_off = vt_off.ptr; d.vt_rec = vt_rec.ptr;
+    d.hv = hv.ptr; d.hrest = hrest.ptr; d.hlen = hlen.ptr; d.hasum = hasum.ptr;
+    d.vh_off = vh_off.ptr; d.vh_rec = vh_rec.ptr;
+    d.pinmask = pinmask.ptr;
+    return d;
+  }
+  DMat dmat() const {
+    return DMat{mat.thickness, mat.stretch_warp, mat.stretch_weft, mat.stretch_coupling, mat.shear,
+                0.5 * (mat.bend_warp + mat.bend_weft)};
+  }
+
+  template <typename T>
+  T read(const T* dptr) {
+    T v;
+    CS_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    return v;
+  }
+  void sync_scalars() {
+    CS_HIP(hipMemcpyAsync(hscal, scal.ptr, sizeof(NewtonScalars), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+  }
+  NewtonScalars* ds() { return scal.ptr; }
+
+  // ---- timing --------------------------------------------------------------
+  int ev_mark() {
+    if (!(opt.flags & CS_FLAG_TIMING)) return -1;
+    if (!ev_init) {
+      for (auto& e : ev_pool) CS_HIP(hipEventCreate(&e));
+      ev_init = true;
+    }
+    if (ev_used >= (int)(sizeof(ev_pool) / sizeof(ev_pool[0]))) flush_timing();
+    const int k = ev_used++;
+    CS_HIP(hipEventRecord(ev_pool[k], st));
+    return k;
+  }
+  void ev_span(int kernel, int a, int b) {
+    if (a >= 0 && b >= 0) ev_pending[kernel].push_back({a, b});
+  }
+  void flush_timing() {
+    if (!ev_init) return;
+    CS_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < CS_KERNEL_COUNT; k++) {
+      for (auto& pr : ev_pending[k]) {
+        float ms = 0;
+        CS_HIP(hipEventElapsedTime(&ms, ev_pool[pr.first], ev_pool[pr.second]));
+        kt_ms[k] += ms;
+        kt_count[k]++;
+      }
+      ev_pending[k].clear();
+    }
+    ev_used = 0;
+  }
+
+  // ---- setup ---------------------------------------------------------------
+  void build_csr(const uint32_t* elems, uint32_t count, int corners, uint32_t nv, std::vector<uint32_t>& off,
+                 std::vector<uint32_t>& rec) {
+    // incidence.ts:20-53: records appended in ascending element order.
+    off.assign(nv + 1, 0);
+    for (uint32_t e = 0; e < count; e++)
+      for (int c = 0; c < corners; c++) off[elems[(size_t)e * corners + c] + 1]++;
+    for (uint32_t v = 0; v < nv; v++) off[v + 1] += off[v];
+    rec.assign((size_t)count * corners, 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t e = 0; e < count; e++)
+      for (int c = 0; c < corners; c++) rec[cur[elems[(size_t)e * corners + c]]++] = (e << 2) | (uint32_t)c;
+  }
+
+  void init(const cs_scene_desc* d, const cs_options* o) {
+    opt = *o;
+    device = opt.device;
+    CS_HIP(hipSetDevice(device));
+    CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    n = d->n_verts; m = d->n_tris; nh = d->n_hinges;
+    mat = d->material;
+    for (int i = 0; i < 3; i++) gravity[i] = d->gravity[i];
+    for (uint32_t k = 0; k < (size_t)m * 3; k++)
+      if (d->indices[k] >= n) throw std::invalid_argument("triangle index out of range");
+    for (uint32_t k = 0; k < (size_t)nh * 4; k++)
+      if (d->hinge_verts[k] >= n) throw std::invalid_argument("hinge vertex out of range");
+    idx.upload(d->indices, (size_t)m * 3, st);
+    inv_dm.upload(d->inv_dm, (size_t)m * 4, st);
+    area.upload(d->areas, m, st);
+    mass.upload(d->masses, n, st);
+    std::vector<uint32_t> off, rec;
+    build_csr(d->indices, m, 3, n, off, rec);
+    vt_off.upload(off.data(), off.size(), st);
+    vt_rec.upload(rec.data(), rec.size(), st);
+    if (nh) {
+      hv.upload(d->hinge_verts, (size_t)nh * 4, st);
+      hrest.upload(d->hinge_rest_angle, nh, st);
+      hlen.upload(d->hinge_edge_len, nh, st);
+      hasum.upload(d->hinge_area_sum, nh, st);
+    } else {
+      hv.alloc(4); hrest.alloc(1); hlen.alloc(1); hasum.alloc(1);
+    }
+    build_csr(nh ? d->hinge_verts : nullptr, nh, 4, n, off, rec);
+    vh_off.upload(off.data(), off.size(), st);
+    vh_rec.upload(rec.data(), rec.size(), st);
+    const size_t n3 = (size_t)n * 3;
+    pos.upload(d->positions, n3, st);
+    vel.upload(d->velocities, n3, st);
+    for (auto* bf : {&x0, &yhat, &x, &xt, &g, &b, &dx, &diag, &ceg, &r, &z, &p, &Ap}) bf->alloc(n3);
+    partials.alloc(4 * 1024);
+    mpartials.alloc(4 * 1024);
+    scal.alloc(1);
+    scal.zero(st);
+    CS_HIP(hipHostMalloc(&hscal, sizeof(NewtonScalars)));
+    std::memset(hscal, 0, sizeof(NewtonScalars));
+    pin_ids.assign(d->pin_ids, d->pin_ids + d->n_pins);
+    pin_pos.assign(d->pin_pos, d->pin_pos + (size_t)d->n_pins * 3);
+    for (uint32_t id : pin_ids)
+      if (id >= n) throw std::invalid_argument("pin id out of range");
+    pinmask.alloc(n);
+    if (opt.precond == CS_PRECOND_BLOCK_JACOBI) binv.alloc((size_t)n * 9);
+    has_contact = d->contact_enabled != 0;
+    if (has_contact) contact.reset(new ContactSystem(*d, st));
+    if (opt.precision == CS_PREC_MIXED) mixed.reset(new MixedSolve(*d, opt, st));
+    CS_HIP(hipStreamSynchronize(st));
+  }
+
+  void upload_pins() {
+    if (!pins_dirty) return;
+    std::vector<uint8_t> mask(n, 0);
+    for (uint32_t id : pin_ids) mask[id] = 1;
+    CS_HIP(hipMemcpyAsync(pinmask.ptr, mask.data(), n, hipMemcpyHostToDevice, st));
+    if (!pin_ids.empty()) {
+      d_pin_ids.upload(pin_ids.data(), pin_ids.size(), st);
+      d_pin_pos.upload(pin_pos.data(), pin_pos.size(), st);
+    }
+    CS_HIP(hipStreamSynchronize(st));
+    pins_dirty = false;
+  }
+
+  void enforce_pins(double* xv) {
+    if (pin_ids.empty()) return;
+    hipLaunchKernelGGL(k_enforce_pins, dim3(grid_for(pin_ids.size())), dim3(kBlock), 0, st, d_pin_ids.ptr,
+                       d_pin_pos.ptr, (uint32_t)pin_ids.size(), xv);
+    CS_LAUNCH_CHECK();
+  }
+
+  // totalEnergy (newton.ts:76-88).
+  double total_energy(const double* xx, double h) {
+    launch_energy(st, dmesh(), dmat(), xx, yhat.ptr, h, partials.ptr, &ds()->tickets[0], ds()->e2);
+    double merit = 0;
+    if (contact) merit = contact->merit_energy(xx);
+    sync_scalars();
+    double e = hscal->e2[0];
+    e += hscal->e2[1];
+    if (contact) e += merit;
+    return e;
+  }
+
+  // ---- the step --------------------------------------------------------------
+  cs_step_stats step(double h) {
+    upload_pins();
+    const size_t n3 = (size_t)n * 3;
+    const int newtonIters = opt.newton_iters, pcgMaxIters = opt.pcg_max_iters, lsIters = opt.line_search_iters;
+    const double gradTol = opt.grad_tol;
+    CS_HIP(hipMemcpyAsync(x0.ptr, pos.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_predictor, dim3(grid_for(n)), dim3(kBlock), 0, st, x0.ptr, vel.ptr, h, gravity[0],
+                       gravity[1], gravity[2], yhat.ptr, x.ptr, n);
+    CS_LAUNCH_CHECK();
+    enforce_pins(x.ptr);
+    int lsTrials = 0;
+    if (contact) {
+      contact->begin_step(x0.ptr);
+      if (!contact->check_trial(x.ptr)) {
+        // predictor bisection (newton.ts:53-74)
+        double lo = 0, hi = 1;
+        for (int bb = 0; bb < 8; bb++) {
+          const double mid = 0.5 * (lo + hi);
+          launch_lerp(mid, xt.ptr);
+          enforce_pins(xt.ptr);
+          if (contact->check_trial(xt.ptr)) lo = mid;
+          else hi = mid;
+          if (hi - lo < 1e-4) break;
+        }
+        launch_lerp(lo, xt.ptr);
+        std::swap(x.ptr, xt.ptr);
+        enforce_pins(x.ptr);
+      }
+      contact->update_active_set(x.ptr);
+    }
+    double energy = total_energy(x.ptr, h);
+    double gradNorm = INFINITY, maxStrain = 0;
+    int pcgItersTotal = 0, used = 0;
+    bool converged = false;
+    const double invH2 = 1 / (h * h);
+    const double beta = jsm::jmax(jsm::jmax(mat.stretch_warp, mat.stretch_weft), mat.shear) * mat.thickness * 0.1 + 1e-6;
+    hipLaunchKernelGGL(k_init_diag, dim3(grid_for(n)), dim3(kBlock), 0, st, mass.ptr, invH2, beta, diag.ptr, n);
+    CS_LAUNCH_CHECK();
+    for (int ni = 0; ni < newtonIters; ni++) {
+      used++;
+      const double* cg = nullptr;
+      if (contact) {
+        contact->update_active_set(x.ptr);
+        contact->energy_grad(x.ptr, ceg.ptr);
+        contact->add_diag_estimate(diag.ptr, x.ptr);
+        cg = ceg.ptr;
+      }
+      const int e0 = ev_mark();
+      launch_gradient(st, dmesh(), dmat(), x.ptr, yhat.ptr, invH2, cg, g.ptr, partials.ptr, &ds()->tickets[0],
+                      &ds()->grad_sumsq);
+      ev_span(CS_KERNEL_GRADIENT, e0, ev_mark());
+      launch_max_strain(st, dmesh(), x.ptr, mpartials.ptr, &ds()->tickets[1], &ds()->max_strain);
+      sync_scalars();
+      maxStrain = hscal->max_strain;
+      gradNorm = std::sqrt(hscal->grad_sumsq);
+      if (!std::isfinite(gradNorm)) throw NonFinite{};
+      if (gradNorm < gradTol) {
+        converged = true;
+        break;
+      }
+      launch_neg(st, g.ptr, b.ptr, n3);
+      int iters = 0;
+      if (mixed) {
+        iters = mixed->solve(*this, invH2);
+      } else {
+        iters = pcg_solve(invH2, pcgMaxIters);
+      }
+      pcgItersTotal += iters;
+      // descent safeguard (newton.ts:154-160)
+      launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->tickets[2], &ds()->gtdx);
+      double gtdx = read(&ds()->gtdx);
+      if (!std::isfinite(gtdx) || gtdx >= 0) {
+        launch_fallback_dir(st, g.ptr, diag.ptr, dx.ptr, n3);
+        launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->tickets[2], &ds()->gtdx);
+        gtdx = read(&ds()->gtdx);
+      }
+      // trust region (newton.ts:167-179)
+      launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->tickets[3], &ds()->maxdx);
+      const double maxDx = read(&ds()->maxdx);
+      const double TRUST = 0.002;
+      if (maxDx > TRUST) {
+        const double s = TRUST / maxDx;
+        launch_scale(st, dx.ptr, s, n3);
+        gtdx *= s;
+      }
+      // Armijo (newton.ts:184-219)
+      double alpha = 1;
+      bool accepted = false;
+      for (int li = 0; li < lsIters; li++) {
+        lsTrials++;
+        launch_trial(st, x.ptr, dx.ptr, alpha, xt.ptr, n3);
+        enforce_pins(xt.ptr);
+        if (contact && !contact->check_trial(xt.ptr)) {
+          contact->add_rejection();
+          alpha *= 0.5;
+          continue;
+        }
+        const double eNew = total_energy(xt.ptr, h);
+        if (!std::isfinite(eNew)) {
+          if (contact) contact->add_rejection();
+          alpha *= 0.5;
+          continue;
+        }
+        if (eNew <= energy + 1e-4 * alpha * gtdx) {
+          std::swap(x.ptr, xt.ptr);
+          energy = eNew;
+          accepted = true;
+          if (contact) contact->commit(x.ptr);
+          break;
+        }
+        if (contact) contact->add_rejection();
+        alpha *= 0.5;
+      }
+      if (!accepted) break;
+      if (alpha < 1e-6) break;
+    }
+    const double damp = 1 - jsm::jmin(jsm::jmax(mat.damping, 0), 0.1);
+    hipLaunchKernelGGL(k_velocity, dim3(grid_for(n3)), dim3(kBlock), 0, st, x.ptr, x0.ptr, h, damp, vel.ptr, n3);
+    CS_LAUNCH_CHECK();
+    if (contact) contact->kill_approach_velocity(x.ptr, vel.ptr);
+    CS_HIP(hipMemcpyAsync(pos.ptr, x.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    enforce_pins(pos.ptr);
+    CS_HIP(hipStreamSynchronize(st));
+    flush_timing();
+    cs_step_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.newton_iters = newtonIters;
+    s.pcg_iters = pcgItersTotal;
+    s.grad_norm = gradNorm;
+    s.energy = energy;
+    s.max_strain = maxStrain;
+    s.converged = converged ? 1 : 0;
+    s.has_contact = contact ? 1 : 0;
+    if (contact) s.contact = contact->diagnostics();
+    else {
+      s.contact.min_distance = INFINITY;
+      s.contact.min_toi = INFINITY;
+    }
+    s.newton_used = used;
+    s.line_search_trials = lsTrials;
+    return s;
+  }
+
+  // xt = x0 + t (x - x0) (predictor bisection probe, newton.ts:62-63)
+  void launch_lerp(double t, double* out);
+
+  PcgArgs pcg_args(double invH2) {
+    PcgArgs a;
+    a.me = dmesh();
+    a.mat = dmat();
+    a.inv_h2 = invH2;
+    a.x = x.ptr;
+    a.b = b.ptr;
+    a.diag = diag.ptr;
+    a.binv = binv.ptr;
+    a.sol = dx.ptr;
+    a.r = r.ptr; a.z = z.ptr; a.p = p.ptr; a.Ap = Ap.ptr;
+    a.hvp_mode = opt.hvp_mode == CS_HVP_ANALYTIC ? HVP_ANALYTIC : HVP_FD;
+    a.precond = opt.precond == CS_PRECOND_BLOCK_JACOBI ? PRE_BLOCK : PRE_JACOBI;
+    a.tol = 1e-3;  // newton.ts:148
+    a.s = &ds()->pcg;
+    a.partials = partials.ptr;
+    if (contact) a.contact = contact->hvp_hook();
+    return a;
+  }
+
+  int pcg_solve(double invH2, int maxIters) {
+    PcgArgs a = pcg_args(invH2);
+    if (a.precond == PRE_BLOCK) launch_block_factors(st, a.me, a.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
+    pcg_init(st, a);
+    const int chunk = 8;
+    for (int k = 0; k < maxIters; k++) {
+      if (contact) contact->launch_hvp_coeffs(p.ptr, &ds()->pcg.done);
+      const int e0 = ev_mark();
+      pcg_iteration(st, a, k);
+      ev_span(CS_KERNEL_HVP, e0, ev_mark());
+      if ((k + 1) % chunk == 0 || k + 1 == maxIters) {
+        const PcgScalars ps = read(&ds()->pcg);
+        if (ps.done) break;
+      }
+    }
+    const PcgScalars ps = read(&ds()->pcg);
+    if (k_first_done_zero(ps)) return maxIters;
+    return ps.iters;
+  }
+  static bool k_first_done_zero(const PcgScalars& ps) { return ps.done == 0; }
+};
+
+__global__ void k_lerp(const double* x0, const double* x1, double t, double* out, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = x0[i] + t * (x1[i] - x0[i]);
+}
+
+void Solver::launch_lerp(double t, double* out) {
+  const size_t n3 = (size_t)n * 3;
+  hipLaunchKernelGGL(k_lerp, dim3(grid_for(n3)), dim3(kBlock), 0, st, x0.ptr, x.ptr, t, out, n3);
+  CS_LAUNCH_CHECK();
+}
+
+}  // namespace cloth
+
+// ============================================================== C ABI
+using cloth::g_last_error;
+
+struct cs_solver {
+  cloth::Solver s;
+};
+
+namespace {
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const cloth::NonFinite&) {
+    return fail(CS_E_NONFINITE, "non-finite gradient \xe2\x80\x94 blow up; reduce dt/stiffness");
+  } catch (const cloth::OverflowError& e) {
+    return fail(CS_E_OVERFLOW, e.what());
+  } catch (const cloth::HipError& e) {
+    return fail(CS_E_HIP, e.what());
+  } catch (const std::invalid_argument& e) {
+    return fail(CS_E_INVALID, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CS_E_NOMEM, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(CS_E_HIP, e.what());
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void cs_options_default(cs_options* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->newton_iters = 10;
+  o->pcg_max_iters = 60;
+  o->line_search_iters = 10;
+  o->grad_tol = 1e-5;
+  o->precision = CS_PREC_FP64;
+  o->hvp_mode = CS_HVP_FD;
+  o->precond = CS_PRECOND_JACOBI;
+  o->schwarz_target = 16;
+  o->schwarz_min = 8;
+  o->schwarz_max = 32;
+  o->device = 0;
+  o->flags = 0;
+}
+
+int cs_abi_version(void) { return CS_ABI_VERSION; }
+
+const char* cs_last_error(void) { return g_last_error.c_str(); }
+
+int cs_device_info(int device, char* name, size_t cap) {
+  return guarded([&]() -> int {
+    int count = 0;
+    CS_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(CS_E_HIP, "no such HIP device");
+    hipDeviceProp_t prop;
+    CS_HIP(hipGetDeviceProperties(&prop, device));
+    if (name && cap) {
+      std::snprintf(name, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    }
+    return CS_OK;
+  });
+}
+
+int cs_create(const cs_scene_desc* scene, const cs_options* opt, cs_solver** out) {
+  if (!scene || !out) return fail(CS_E_INVALID, "null argument");
+  cs_options def;
+  cs_options_default(&def);
+  if (!opt) opt = &def;
+  if (scene->n_verts == 0 || scene->n_tris == 0 || !scene->positions || !scene->velocities || !scene->indices ||
+      !scene->inv_dm || !scene->areas || !scene->masses)
+    return fail(CS_E_INVALID, "scene is missing mesh arrays");
+  if (scene->n_hinges && (!scene->hinge_verts || !scene->hinge_rest_angle || !scene->hinge_edge_len ||
+                          !scene->hinge_area_sum))
+    return fail(CS_E_INVALID, "scene is missing hinge arrays");
+  if (scene->n_pins && (!scene->pin_ids || !scene->pin_pos)) return fail(CS_E_INVALID, "scene is missing pin arrays");
+  if (opt->newton_iters < 0 || opt->pcg_max_iters < 0 || opt->line_search_iters < 0)
+    return fail(CS_E_INVALID, "negative iteration count");
+  if (opt->precision != CS_PREC_FP64 && opt->precision != CS_PREC_MIXED)
+    return fail(CS_E_INVALID, "unknown precision");
+  if (opt->precond == CS_PRECOND_SCHWARZ) return fail(CS_E_UNSUPPORTED, "additive Schwarz is not built in this library");
+  *out = nullptr;
+  auto* h = new cs_solver;
+  const int rc = guarded([&]() -> int {
+    h->s.init(scene, opt);
+    return CS_OK;
+  });
+  if (rc != CS_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CS_OK;
+}
+
+void cs_destroy(cs_solver* s) { delete s; }
+
+int cs_step(cs_solver* s, double dt) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CS_E_INVALID, "dt must be positive and finite");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(s->s.device));
+    s->s.last = s->s.step(dt);
+    s->s.has_last = true;
+    return CS_OK;
+  });
+}
+
+int cs_set_material(cs_solver* s, const cs_material* mat) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!mat) return fail(CS_E_INVALID, "null material");
+  s->s.mat = *mat;
+  if (s->s.mixed) s->s.mixed->set_material(*mat);
+  return CS_OK;
+}
+
+int cs_pin_vertex(cs_solver* s, uint32_t vid, const double* pos) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  cloth::Solver& S = s->s;
+  if (vid >= S.n) return fail(CS_E_INVALID, "vertex id out of range");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(S.device));
+    double p[3];
+    if (pos) {
+      p[0] = pos[0]; p[1] = pos[1]; p[2] = pos[2];
+    } else {
+      CS_HIP(hipMemcpyAsync(p, S.pos.ptr + (size_t)vid * 3, 3 * sizeof(double), hipMemcpyDeviceToHost, S.st));
+      CS_HIP(hipStreamSynchronize(S.st));
+    }
+    auto it = std::find(S.pin_ids.begin(), S.pin_ids.end(), vid);
+    if (it != S.pin_ids.end()) {
+      const size_t k = it - S.pin_ids.begin();
+      std::copy(p, p + 3, S.pin_pos.begin() + k * 3);
+    } else {
+      S.pin_ids.push_back(vid);
+      S.pin_pos.insert(S.pin_pos.end(), p, p + 3);
+    }
+    S.pins_dirty = true;
+    S.upload_pins();
+    S.enforce_pins(S.pos.ptr);
+    CS_HIP(hipStreamSynchronize(S.st));
+    return CS_OK;
+  });
+}
+
+int cs_unpin_vertex(cs_solver* s, uint32_t vid) {
+  if (!s) return CS_OK;  // unpinVertex uses optional chaining (cpu-solver.ts:36)
+  cloth::Solver& S = s->s;
+  auto it = std::find(S.pin_ids.begin(), S.pin_ids.end(), vid);
+  if (it != S.pin_ids.end()) {
+    const size_t k = it - S.pin_ids.begin();
+    S.pin_ids.erase(it);
+    S.pin_pos.erase(S.pin_pos.begin() + k * 3, S.pin_pos.begin() + k * 3 + 3);
+    S.pins_dirty = true;
+  }
+  return CS_OK;
+}
+
+int cs_get_positions(cs_solver* s, double* out, size_t n3) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!out || n3 != (size_t)s->s.n * 3) return fail(CS_E_INVALID, "positions buffer must hold 3n doubles");
+  return guarded([&]() -> int {
+    CS_HIP(hipMemcpyAsync(out, s->s.pos.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, s->s.st));
+    CS_HIP(hipStreamSynchronize(s->s.st));
+    return CS_OK;
+  });
+}
+
+int cs_get_velocities(cs_solver* s, double* out, size_t n3) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!out || n3 != (size_t)s->s.n * 3) return fail(CS_E_INVALID, "velocities buffer must hold 3n doubles");
+  return guarded([&]() -> int {
+    CS_HIP(hipMemcpyAsync(out, s->s.vel.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, s->s.st));
+    CS_HIP(hipStreamSynchronize(s->s.st));
+    return CS_OK;
+  });
+}
+
+int cs_set_state(cs_solver* s, const double* x, const double* v, size_t n3) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (n3 != (size_t)s->s.n * 3) return fail(CS_E_INVALID, "state must hold 3n doubles");
+  return guarded([&]() -> int {
+    if (x) CS_HIP(hipMemcpyAsync(s->s.pos.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, s->s.st));
+    if (v) CS_HIP(hipMemcpyAsync(s->s.vel.ptr, v, n3 * sizeof(double), hipMemcpyHostToDevice, s->s.st));
+    CS_HIP(hipStreamSynchronize(s->s.st));
+    return CS_OK;
+  });
+}
+
+int cs_last_stats(const cs_solver* s, cs_step_stats* out) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!out) return fail(CS_E_INVALID, "null stats");
+  if (!s->s.has_last) {
+    std::memset(out, 0, sizeof(*out));
+    return CS_OK;
+  }
+  *out = s->s.last;
+  return CS_OK;
+}
+
+int cs_kernel_timing(const cs_solver* s, int kernel, double* total_ms, uint64_t* count, double* bytes) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (kernel < 0 || kernel >= CS_KERNEL_COUNT) return fail(CS_E_INVALID, "unknown kernel family");
+  if (total_ms) *total_ms = s->s.kt_ms[kernel];
+  if (count) *count = s->s.kt_count[kernel];
+  if (bytes) {
+    const double n = s->s.n, m = s->s.m;
+    double b = 0;
+    const bool mixed = s->s.opt.precision == CS_PREC_MIXED;
+    const double C = s->s.contact ? (double)s->s.contact->active_count() : 0.0;
+    if (kernel == CS_KERNEL_HVP) b = mixed ? 40 * n + 32 * m + 64 * C : 80 * n + 52 * m + 96 * C;
+    *bytes = b;
+  }
+  return CS_OK;
+}
+
+int cs_reset_timing(cs_solver* s) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  for (int k = 0; k < CS_KERNEL_COUNT; k++) {
+    s->s.kt_ms[k] = 0;
+    s->s.kt_count[k] = 0;
+  }
+  return CS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+"""HipSolver: the ClothSolver backend (src/backend/solver.ts:5-13) on one
+MI355X, over the C ABI of include/cloth_hip.h.
+
+Mirrors CpuSolver (src/backend/cpu-solver.ts:7-49): `initialize` keeps the
+caller's scene by reference, `step` advances it and writes the new state back
+into `scene.positions` / `scene.velocities` in place, so `getPositions()`
+returns the same live array every tick.  Errors raise like the reference's
+thrown Errors.  There is no CPU fallback: without the HIP library or a device
+every call fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from .scene import ClothMaterial, ClothScene, to_desc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libclothhip.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class ClothError(RuntimeError):
+    """A non-zero status from the C ABI (message from cs_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"HIP cloth library not built: {path} (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    sig = {
+        "cs_options_default": (None, [C.POINTER(_abi.CsOptions)]),
+        "cs_create": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.POINTER(_abi.CsOptions), C.POINTER(C.c_void_p)]),
+        "cs_destroy": (None, [C.c_void_p]),
+        "cs_step": (C.c_int, [C.c_void_p, C.c_double]),
+        "cs_set_material": (C.c_int, [C.c_void_p, C.POINTER(_abi.CsMaterial)]),
+        "cs_pin_vertex": (C.c_int, [C.c_void_p, C.c_uint32, _dp]),
+        "cs_unpin_vertex": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "cs_get_positions": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
+        "cs_get_velocities": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
+        "cs_set_state": (C.c_int, [C.c_void_p, _dp, _dp, C.c_size_t]),
+        "cs_last_stats": (C.c_int, [C.c_void_p, C.POINTER(_abi.CsStepStats)]),
+        "cs_last_error": (C.c_char_p, []),
+        "cs_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_size_t]),
+        "cs_abi_version": (C.c_int, []),
+        "cs_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_uint64), _dp]),
+        "cs_reset_timing": (C.c_int, [C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != _abi.CS_OK:
+        raise ClothError(rc, load_library().cs_last_error().decode(errors="replace"))
+
+
+def device_info(device=0):
+    buf = C.create_string_buffer(256)
+    _check(load_library().cs_device_info(device, buf, 256))
+    return buf.value.decode()
+
+
+class HipSolver:
+    """ClothSolver on the GPU.  `precision` is "fp64" (the reference operator,
+    bit-faithful to CpuSolver) or "mixed" (FP32 inner solve, FP64 Newton)."""
+
+    def __init__(self, precision="fp64", options=None, **kw):
+        self._lib = load_library()
+        if options is None:
+            prec = _abi.CS_PREC_MIXED if precision == "mixed" else _abi.CS_PREC_FP64
+            options = _abi.options(precision=prec, **kw)
+        self._opts = options
+        self._h = None
+        self.scene = None
+
+    # ---- ClothSolver ---------------------------------------------------------
+    def initialize(self, scene: ClothScene):
+        self.close()
+        desc, keep = to_desc(scene)
+        h = C.c_void_p()
+        _check(self._lib.cs_create(C.byref(desc), C.byref(self._opts), C.byref(h)))
+        self._h = h
+        self.scene = scene
+        del keep
+
+    def _need(self):
+        if self._h is None:
+            raise ClothError(_abi.CS_E_NOT_INIT, "HipSolver not initialized")
+
+    def step(self, dt):
+        self._need()
+        _check(self._lib.cs_step(self._h, float(dt)))
+        self._copy_out()
+
+    def setMaterial(self, material: ClothMaterial):
+        self._need()
+        c = material.to_c()
+        _check(self._lib.cs_set_material(self._h, C.byref(c)))
+        from dataclasses import replace
+        self.scene.material = replace(material)
+
+    def pinVertex(self, vertexId, position=None):
+        self._need()
+        if position is None:
+            _check(self._lib.cs_pin_vertex(self._h, vertexId, None))
+        else:
+            p = np.ascontiguousarray(position, np.float64)
+            _check(self._lib.cs_pin_vertex(self._h, vertexId, p.ctypes.data_as(_dp)))
+        self._copy_out(velocities=False)
+        i = int(vertexId)
+        self.scene.pinned[i] = tuple(float(v) for v in self.scene.positions[i * 3:i * 3 + 3])
+
+    def unpinVertex(self, vertexId):
+        if self._h is None:
+            return
+        _check(self._lib.cs_unpin_vertex(self._h, vertexId))
+        self.scene.pinned.pop(int(vertexId), None)
+
+    def getPositions(self):
+        self._need()
+        return self.scene.positions
+
+    def getVelocities(self):
+        self._need()
+        return self.scene.velocities
+
+    @property
+    def lastStats(self):
+        if self._h is None:
+            return None
+        st = _abi.CsStepStats()
+        _check(self._lib.cs_last_stats(self._h, C.byref(st)))
+        return _abi.stats_to_dict(st)
+
+    # ---- extras --------------------------------------------------------------
+    def set_state(self, x, v):
+        self._need()
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        _check(self._lib.cs_set_state(self._h, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp), len(x)))
+        self.scene.positions[:] = x
+        self.scene.velocities[:] = v
+
+    def kernel_timing(self):
+        self._need()
+        out = {}
+        for k, name in enumerate(_abi.KERNEL_NAMES):
+            ms = C.c_double()
+            cnt = C.c_uint64()
+            b = C.c_double()
+            _check(self._lib.cs_kernel_timing(self._h, k, C.byref(ms), C.byref(cnt), C.byref(b)))
+            out[name] = dict(total_ms=ms.value, count=cnt.value, bytes=b.value)
+        return out
+
+    def reset_timing(self):
+        self._need()
+        _check(self._lib.cs_reset_timing(self._h))
+
+    def _copy_out(self, velocities=True):
+        sc = self.scene
+        n3 = len(sc.positions)
+        if not sc.positions.flags.c_contiguous or sc.positions.dtype != np.float64:
+            raise ValueError("scene.positions must be a contiguous float64 array")
+        _check(self._lib.cs_get_positions(self._h, sc.positions.ctypes.data_as(_dp), n3))
+        if velocities:
+            _check(self._lib.cs_get_velocities(self._h, sc.velocities.ctypes.data_as(_dp), n3))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.cs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,75 @@
+"""C ABI: the library loads on the CPU, exports exactly what include/ declares
+and rejects bad arguments before touching a device."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from closim_amd import _abi
+from closim_amd.solver import load_library, LIB_PATH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char\s*\*|const char\s*\*)\s*\**\s*(cs_\w+)\s*\(",
+                                    txt, re.M))
+    return names
+
+
+def test_header_declares_the_clothsolver_surface():
+    names = declared_symbols()
+    for n in ("cs_create", "cs_step", "cs_set_material", "cs_pin_vertex", "cs_unpin_vertex",
+              "cs_get_positions", "cs_get_velocities", "cs_last_stats", "cs_last_error", "cs_destroy"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    missing = declared_symbols() - exported
+    assert not missing, missing
+    for n in declared_symbols():
+        assert hasattr(L, n)
+
+
+def test_struct_sizes_match_header(tmp_path):
+    src = str(tmp_path / "abi_sizes.c")
+    exe = src[:-2]
+    with open(src, "w") as f:
+        f.write('#include <stdio.h>\n#include "cloth_hip.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu\\n",'
+                'sizeof(cs_material),sizeof(cs_contact_params),sizeof(cs_scene_desc),sizeof(cs_options),'
+                'sizeof(cs_contact_diag),sizeof(cs_step_stats));return 0;}\n')
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+    sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    py = [C.sizeof(t) for t in (_abi.CsMaterial, _abi.CsContactParams, _abi.CsSceneDesc, _abi.CsOptions,
+                                _abi.CsContactDiag, _abi.CsStepStats)]
+    assert sizes == py
+
+
+def test_defaults_match_reference_stepper_options():
+    L = load_library()
+    o = _abi.CsOptions()
+    L.cs_options_default(C.byref(o))
+    assert (o.newton_iters, o.pcg_max_iters, o.line_search_iters, o.grad_tol) == (10, 60, 10, 1e-5)
+    assert o.precision == _abi.CS_PREC_FP64 and o.hvp_mode == _abi.CS_HVP_FD and o.precond == _abi.CS_PRECOND_JACOBI
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    L = load_library()
+    h = C.c_void_p()
+    assert L.cs_create(None, None, C.byref(h)) == _abi.CS_E_INVALID
+    d = _abi.CsSceneDesc()
+    assert L.cs_create(C.byref(d), None, C.byref(h)) == _abi.CS_E_INVALID
+    assert b"missing" in L.cs_last_error()
+    assert L.cs_step(None, 1 / 60) == _abi.CS_E_NOT_INIT
+    assert L.cs_last_error() == b"CpuSolver not initialized"
+    assert L.cs_get_positions(None, None, 0) == _abi.CS_E_NOT_INIT
+    assert L.cs_abi_version() == 1
