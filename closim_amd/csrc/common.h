@@ -25,6 +25,34 @@ struct HipError : std::runtime_error {
 
 #define CS_LAUNCH_CHECK() CS_HIP(hipGetLastError())
 
+// Device buffer that grows on demand (contents are not preserved on growth).
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    if (n <= count && ptr) return;
+    release();
+    if (n == 0) n = 1;
+    CS_HIP(hipMalloc(&ptr, n * sizeof(T)));
+    count = n;
+  }
+  void upload(const T* h, size_t n, hipStream_t st) {
+    alloc(n);
+    if (n) CS_HIP(hipMemcpyAsync(ptr, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void zero(hipStream_t st) { CS_HIP(hipMemsetAsync(ptr, 0, count * sizeof(T), st)); }
+};
+
 constexpr int kBlock = 256;       // threads per block for 1-D kernels (4 wave64)
 constexpr int kWave = 64;
 

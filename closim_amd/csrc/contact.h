@@ -33,10 +33,19 @@ class ContactSystem {
   // dd[c] = coeff[c] * (J_c . p) for the current PCG direction.
   void launch_hvp_coeffs(const double* p, const int32_t* done);
   uint64_t active_count() const;
+  // Copy the active list (kind,p,a,b,c,d + w0..w3,nx,ny,nz) to host arrays.
+  uint64_t copy_active(int32_t* ids6, double* vals7, uint64_t cap);
+  uint64_t candidate_count() const;
   ContactImpl* impl() { return im_; }
 
  private:
   ContactImpl* im_;
 };
+
+// Batched device primitives for parity probes.
+void probe_ccd(hipStream_t st, const double* x0, const double* x1, uint32_t n, const int32_t* kinds,
+               const int32_t* ids4, uint64_t count, double thickness, double* toi);
+void probe_closest(hipStream_t st, const double* x0, uint32_t n, const int32_t* kinds, const int32_t* ids4,
+                   uint64_t count, double* out3);
 
 }  // namespace cloth

@@ -21,33 +21,6 @@ namespace cloth {
 
 thread_local std::string g_last_error;
 
-template <typename T>
-struct DevBuf {
-  T* ptr = nullptr;
-  size_t count = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    count = 0;
-  }
-  void alloc(size_t n) {
-    if (n <= count && ptr) return;
-    release();
-    if (n == 0) n = 1;
-    CS_HIP(hipMalloc(&ptr, n * sizeof(T)));
-    count = n;
-  }
-  void upload(const T* h, size_t n, hipStream_t st) {
-    alloc(n);
-    if (n) CS_HIP(hipMemcpyAsync(ptr, h, n * sizeof(T), hipMemcpyHostToDevice, st));
-  }
-  void zero(hipStream_t st) { CS_HIP(hipMemsetAsync(ptr, 0, count * sizeof(T), st)); }
-};
-
 struct NewtonScalars {
   PcgScalars pcg;
   double grad_sumsq;
@@ -735,6 +708,67 @@ int cs_reset_timing(cs_solver* s) {
     s->s.kt_count[k] = 0;
   }
   return CS_OK;
+}
+
+int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t* ids6, double* vals7,
+                        uint64_t cap, uint64_t* count, uint64_t* n_candidates) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (!x0 || !x) return fail(CS_E_INVALID, "null positions");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x0.ptr, x0, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.contact->begin_step(S.x0.ptr);
+    S.contact->update_active_set(S.x.ptr);
+    const uint64_t c = S.contact->copy_active(ids6, vals7, cap);
+    if (count) *count = c;
+    if (n_candidates) *n_candidates = S.contact->candidate_count();
+    return CS_OK;
+  });
+}
+
+int cs_probe_check_trial(cs_solver* s, const double* x0, const double* xt, int32_t* valid, double* min_dist,
+                         double* min_toi) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (!x0 || !xt) return fail(CS_E_INVALID, "null positions");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x0.ptr, x0, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.xt.ptr, xt, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.contact->begin_step(S.x0.ptr);
+    const bool ok = S.contact->check_trial(S.xt.ptr);
+    const cs_contact_diag d = S.contact->diagnostics();
+    if (valid) *valid = ok ? 1 : 0;
+    if (min_dist) *min_dist = d.min_distance;
+    if (min_toi) *min_toi = d.min_toi;
+    return CS_OK;
+  });
+}
+
+int cs_probe_ccd(int device, const double* x0, const double* x1, uint32_t n_verts, const int32_t* kinds,
+                 const int32_t* ids4, uint64_t count, double thickness, double* toi) {
+  if (!x0 || !x1 || !kinds || !ids4 || !toi) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    cloth::probe_ccd(nullptr, x0, x1, n_verts, kinds, ids4, count, thickness, toi);
+    return CS_OK;
+  });
+}
+
+int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32_t* kinds, const int32_t* ids4,
+                     uint64_t count, double* out3) {
+  if (!x0 || !kinds || !ids4 || !out3) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    cloth::probe_closest(nullptr, x0, n_verts, kinds, ids4, count, out3);
+    return CS_OK;
+  });
 }
 
 }  // extern "C"

@@ -55,6 +55,13 @@ def load_library(path=LIB_PATH):
         "cs_abi_version": (C.c_int, []),
         "cs_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_uint64), _dp]),
         "cs_reset_timing": (C.c_int, [C.c_void_p]),
+        "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "cs_probe_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, _dp]),
+        "cs_probe_ccd": (C.c_int, [C.c_int, _dp, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.c_uint64, C.c_double, _dp]),
+        "cs_probe_closest": (C.c_int, [C.c_int, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                       C.c_uint64, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -73,6 +80,32 @@ def device_info(device=0):
     buf = C.create_string_buffer(256)
     _check(load_library().cs_device_info(device, buf, 256))
     return buf.value.decode()
+
+
+def probe_ccd(x0, x1, kinds, ids4, thickness, device=0):
+    """vtCCD / eeCCD on the device for a batch of primitives."""
+    L = load_library()
+    x0 = np.ascontiguousarray(x0, np.float64)
+    x1 = np.ascontiguousarray(x1, np.float64)
+    kinds = np.ascontiguousarray(kinds, np.int32)
+    ids4 = np.ascontiguousarray(ids4, np.int32)
+    out = np.zeros(len(kinds))
+    _check(L.cs_probe_ccd(device, x0.ctypes.data_as(_dp), x1.ctypes.data_as(_dp), len(x0) // 3,
+                          kinds.ctypes.data_as(C.POINTER(C.c_int32)), ids4.ctypes.data_as(C.POINTER(C.c_int32)),
+                          len(kinds), thickness, out.ctypes.data_as(_dp)))
+    return out
+
+
+def probe_closest(x0, kinds, ids4, device=0):
+    """Closest point (s, t, dist) on the device for a batch of VT / EE primitives."""
+    L = load_library()
+    x0 = np.ascontiguousarray(x0, np.float64)
+    kinds = np.ascontiguousarray(kinds, np.int32)
+    ids4 = np.ascontiguousarray(ids4, np.int32)
+    out = np.zeros((len(kinds), 3))
+    _check(L.cs_probe_closest(device, x0.ctypes.data_as(_dp), len(x0) // 3, kinds.ctypes.data_as(C.POINTER(C.c_int32)),
+                              ids4.ctypes.data_as(C.POINTER(C.c_int32)), len(kinds), out.ctypes.data_as(_dp)))
+    return out
 
 
 class HipSolver:
@@ -170,6 +203,34 @@ class HipSolver:
     def reset_timing(self):
         self._need()
         _check(self._lib.cs_reset_timing(self._h))
+
+    # ---- component probes (parity tests) ------------------------------------
+    def probe_active_set(self, x0, x, cap=1 << 20):
+        self._need()
+        x0 = np.ascontiguousarray(x0, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        ids = np.zeros((cap, 6), np.int32)
+        vals = np.zeros((cap, 7))
+        cnt = C.c_uint64()
+        nc = C.c_uint64()
+        _check(self._lib.cs_probe_active_set(self._h, x0.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                             ids.ctypes.data_as(C.POINTER(C.c_int32)), vals.ctypes.data_as(_dp),
+                                             cap, C.byref(cnt), C.byref(nc)))
+        k = cnt.value
+        if k > cap:
+            return self.probe_active_set(x0, x, cap=int(k))
+        return ids[:k].copy(), vals[:k].copy(), nc.value
+
+    def probe_check_trial(self, x0, xt):
+        self._need()
+        x0 = np.ascontiguousarray(x0, np.float64)
+        xt = np.ascontiguousarray(xt, np.float64)
+        v = C.c_int32()
+        md = C.c_double()
+        mt = C.c_double()
+        _check(self._lib.cs_probe_check_trial(self._h, x0.ctypes.data_as(_dp), xt.ctypes.data_as(_dp), C.byref(v),
+                                              C.byref(md), C.byref(mt)))
+        return bool(v.value), md.value, mt.value
 
     def _copy_out(self, velocities=True):
         sc = self.scene

@@ -216,6 +216,25 @@ enum {
 int cs_kernel_timing(const cs_solver* s, int kernel, double* total_ms, uint64_t* count, double* bytes);
 int cs_reset_timing(cs_solver* s);
 
+/* ---- component probes for parity tests (no reference counterpart) ------- */
+/* Begin a step at x0 and build the frozen active set at x (updateActiveSet,
+ * contact-assembly.ts:164-220).  Writes min(count, cap) records in active-list
+ * order: ids6 = (kind, p, a, b, c, d), vals7 = (w0, w1, w2, w3, nx, ny, nz). */
+int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t* ids6, double* vals7,
+                        uint64_t cap, uint64_t* count, uint64_t* n_candidates);
+/* Begin a step at x0 and validate trial xt (checkTrial, contact-assembly.ts:489-553).
+ * min_dist / min_toi are the minima recorded when the trial is valid. */
+int cs_probe_check_trial(cs_solver* s, const double* x0, const double* xt, int32_t* valid, double* min_dist,
+                         double* min_toi);
+/* Batched primitives on device `device`: kinds[i] 0 = VT (p,a,b,c), 1 = EE
+ * (a,b,c,d); ids4 index interleaved xyz arrays x0/x1 of n_verts vertices.
+ * cs_probe_ccd -> vtCCD/eeCCD time of impact; cs_probe_closest -> (s, t, dist)
+ * of the closest point at x0. */
+int cs_probe_ccd(int device, const double* x0, const double* x1, uint32_t n_verts, const int32_t* kinds,
+                 const int32_t* ids4, uint64_t count, double thickness, double* toi);
+int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32_t* kinds, const int32_t* ids4,
+                     uint64_t count, double* out3);
+
 #ifdef __cplusplus
 }
 #endif
