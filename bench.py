@@ -1,0 +1,193 @@
+"""Benchmark: implicit cloth step on one MI355X, mixed precision vs FP64.
+
+Workload (default): sphere + floor drape, 256x256-vertex sheet (130,050
+triangles, 1 m side) at 0.5 m with seeded +-1e-4 m jitter, static 64x32
+sphere r = 0.25 m, floor, d_hat 1e-3, mu 0.3, h = 1/60, block-Jacobi PCG.
+A "step" is one ClothSolver.step(h) of the whole scene.
+
+Prints one JSON line:
+  value        mixed-precision HipSolver steps/s (state resident in HBM;
+               host copy-out of positions is excluded and reported apart)
+  fp64         the FP64 HipSolver (reference operator) on the same scene
+  roofline     HVP kernel: algorithmic bytes per launch / mean launch time
+               (HIP events on the solver's stream) vs the 8 TB/s HBM peak
+  cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hip_sync():
+    try:
+        lib = C.CDLL("libamdhip64.so")
+        lib.hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def make_scene(name):
+    from closim_amd import scenes as S
+    if name == "sphere":
+        return S.sphere_drape(255)
+    if name == "fold":
+        return S.crumple_fold(511)
+    if name == "strip":
+        return S.strip_pull()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def solver_for(scene, precision, precond, timing):
+    from closim_amd import _abi
+    from closim_amd.solver import HipSolver
+    flags = _abi.CS_FLAG_TIMING if timing else 0
+    g = HipSolver(precision, precond=precond, flags=flags)
+    g.initialize(scene)
+    return g
+
+
+def raw_step(g, h):
+    from closim_amd.solver import _check
+    _check(g._lib.cs_step(g._h, h))
+
+
+def timed_steps(g, h, steps):
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        raw_step(g, h)
+    hip_sync()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(name, precond, budget_s):
+    """Oracle (single thread) on a bounded sample: whole steps from the
+    initial state until `budget_s` seconds have elapsed (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as O
+    from closim_amd import _abi
+    scene = make_scene(name)
+    o = O.OracleSolver(scene, _abi.options(precond=precond))
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1 / 60)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="steps/s", cores=1, kind="port",
+                sample=f"first {n} step(s) of the same workload from its initial state on one host thread "
+                       f"(CPU oracle, FP64, same preconditioner), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "fold", "strip"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the cloth step is single-device); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    from closim_amd import _abi
+    from closim_amd.solver import device_info
+    h = 1.0 / 60.0
+    precond = _abi.CS_PRECOND_BLOCK_JACOBI if args.workload == "sphere" else _abi.CS_PRECOND_JACOBI
+    print(f"[bench] device: {device_info(0)}", file=sys.stderr, flush=True)
+    sc_m = make_scene(args.workload)
+    mesh = sc_m.mesh
+    print(f"[bench] workload {args.workload}: {mesh.count} vertices, {mesh.tri_count} triangles", file=sys.stderr,
+          flush=True)
+    gm = solver_for(sc_m, "mixed", precond, timing=True)
+    gd = None
+    if not args.no_fp64:
+        sc_d = make_scene(args.workload)
+        gd = solver_for(sc_d, "fp64", precond, timing=True)
+    # warmup (both solvers advance from the same initial state)
+    for k in range(args.warmup):
+        raw_step(gm, h)
+        if gd:
+            raw_step(gd, h)
+        print(f"[bench] warmup step {k} done", file=sys.stderr, flush=True)
+    gm.reset_timing()
+    t_m = timed_steps(gm, h, args.steps)
+    print(f"[bench] mixed: {args.steps} steps in {t_m:.3f} s", file=sys.stderr, flush=True)
+    kt = gm.kernel_timing()
+    st_m = gm.lastStats
+    res = {}
+    if gd:
+        gd.reset_timing()
+        t_d = timed_steps(gd, h, args.steps)
+        print(f"[bench] fp64: {args.steps} steps in {t_d:.3f} s", file=sys.stderr, flush=True)
+        kt_d = gd.kernel_timing()
+        gm._copy_out()
+        gd._copy_out()
+        pm, pd = gm.getPositions(), gd.getPositions()
+        err = float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd)))
+        res["fp64"] = dict(steps_per_s=args.steps / t_d, ms_per_step=1000 * t_d / args.steps,
+                           hvp_mean_us=1000 * kt_d["hvp"]["total_ms"] / max(kt_d["hvp"]["count"], 1),
+                           hvp_launches=kt_d["hvp"]["count"])
+        res["mixed_over_fp64"] = t_d / t_m
+        res["pos_rel_err_mixed_vs_fp64"] = err
+    hvp = kt["hvp"]
+    mean_s = hvp["total_ms"] / max(hvp["count"], 1) / 1000.0
+    achieved = hvp["bytes"] / mean_s / 1e9 if mean_s > 0 else 0.0
+    # PCIe-inclusive rate (positions + velocities copied out each step)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        raw_step(gm, h)
+        gm._copy_out()
+    pcie_rate = 2 / (time.perf_counter() - t0)
+    out = {
+        "metric": "sim steps/s + HVP HBM GB/s, mixed vs FP64 GPU at <=1e-5 rel pos err",
+        "value": args.steps / t_m,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000 * t_m / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner solve / f64 Newton",
+        "data": "synthetic",
+        "config": {"workload": make_scene.__name__ and sc_m.meta.get("name", args.workload),
+                   "vertices": mesh.count, "triangles": mesh.tri_count, "h": h,
+                   "preconditioner": "block-jacobi" if precond == _abi.CS_PRECOND_BLOCK_JACOBI else "jacobi",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_mx_hvp (FP32 analytic HVP + mass + contact)",
+                     "bytes_per_launch": hvp["bytes"], "mean_launch_us": mean_s * 1e6, "launches": hvp["count"]},
+        "newton_used_last_step": st_m["newton_used"],
+        "pcg_iters_last_step": st_m["pcg_iters"],
+        "active_contacts_last_step": st_m["contact"]["active_pairs"] if st_m["contact"] else 0,
+        "pcie_inclusive_steps_per_s": pcie_rate,
+    }
+    out.update(res)
+    if not args.no_cpu:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.workload, precond, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,5 +1,6 @@
 // mixed.h — mixed-precision inner solve: FP32 analytic HVP, preconditioner
-// and PCG vectors inside the FP64 Newton loop.
+// apply and PCG vectors with FP32 accumulation, inside the FP64 Newton loop
+// (the Newton gradient, energies, CCD and the Armijo test stay FP64).
 #pragma once
 
 #include "../../include/cloth_hip.h"
@@ -7,18 +8,41 @@
 
 namespace cloth {
 
-struct Solver;
+struct MixedInputs {
+  DMesh me;
+  DMat mat;
+  double inv_h2;
+  const double* x;     // FP64 Newton iterate
+  const double* b;     // FP64 right-hand side -g
+  const double* diag;  // FP64 Jacobi diagonal
+  const double* binv;  // FP64 block-Jacobi inverses [9n] or null
+  double* dx;          // FP64 Newton step (output)
+  DContactHvp contact; // FP64 frozen contacts (act + coeff)
+  int max_iters;
+  double tol;
+};
+
 struct MixedImpl;
 
 class MixedSolve {
  public:
   MixedSolve(const cs_scene_desc& d, const cs_options& o, hipStream_t st);
   ~MixedSolve();
-  int solve(Solver& s, double inv_h2);  // writes the FP64 Newton step into s.dx
-  void set_material(const cs_material& m);
+  // Returns the PCG iteration count; `hvp_timer(k, begin)` brackets the HVP
+  // launch of iteration k for kernel timing.
+  template <typename F>
+  int solve(const MixedInputs& in, F&& hvp_timer);
+  int solve_plain(const MixedInputs& in, void (*timer)(void*, int, bool), void* ctx);
+  void set_material(const cs_material&) {}
 
  private:
   MixedImpl* im_;
 };
+
+template <typename F>
+int MixedSolve::solve(const MixedInputs& in, F&& hvp_timer) {
+  auto thunk = [](void* ctx, int k, bool begin) { (*static_cast<F*>(ctx))(k, begin); };
+  return solve_plain(in, thunk, &hvp_timer);
+}
 
 }  // namespace cloth

@@ -57,6 +57,8 @@ void pcg_init(hipStream_t st, const PcgArgs& a);
 // One PCG iteration k (0-based): HVP + alpha, update + preconditioner +
 // residual test + beta, direction update (+ |p| for the FD operator).
 void pcg_iteration(hipStream_t st, const PcgArgs& a, int k);
+void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k);   // k_pcg_hvp only
+void pcg_iteration_rest(hipStream_t st, const PcgArgs& a, int k);  // update + direction
 // External preconditioner hook (Schwarz): z = M^-1 r must be produced between
 // pcg_update_x and pcg_finish_z.
 void pcg_iteration_split_a(hipStream_t st, const PcgArgs& a, int k);

@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k) {
   if (grid_reduce_dd<kBlock>(rr, a.partials, &s->tickets[5], t)) {
     s->residual = jsm::sqrt_(jsm::dd_value(t));
     if (s->residual < s->abs_tol) {
-      s->done = 2;  // converged: finished after this iteration
+      s->done = 2;  // converged in this iteration (any non-zero value stops the solve)
       s->iters = k + 1;
     }
   }
@@ -270,12 +270,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a) {
   }
 }
 
-// The converged latch set inside k_pcg_update must stop the direction update
-// of the same iteration too; done == 2 marks "converged this iteration".
-__global__ void k_pcg_settle(PcgScalars* s) {
-  if (s->done == 2) s->done = 1;
-}
-
 void pcg_init(hipStream_t st, const PcgArgs& a) {
   hipLaunchKernelGGL(k_pcg_init, dim3(red_grid(a.me.n)), dim3(kBlock), 0, st, a);
   CS_LAUNCH_CHECK();
@@ -284,11 +278,19 @@ void pcg_init(hipStream_t st, const PcgArgs& a) {
 }
 
 void pcg_iteration(hipStream_t st, const PcgArgs& a, int k) {
+  pcg_iteration_hvp(st, a, k);
+  pcg_iteration_rest(st, a, k);
+}
+
+void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k) {
+  hipLaunchKernelGGL(k_pcg_hvp, dim3(red_grid(a.me.n)), dim3(kBlock), 0, st, a, k);
+  CS_LAUNCH_CHECK();
+}
+
+void pcg_iteration_rest(hipStream_t st, const PcgArgs& a, int k) {
   const dim3 g(red_grid(a.me.n)), b(kBlock);
-  hipLaunchKernelGGL(k_pcg_hvp, g, b, 0, st, a, k);
   hipLaunchKernelGGL(k_pcg_update, g, b, 0, st, a, k);
   hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a);
-  hipLaunchKernelGGL(k_pcg_settle, dim3(1), dim3(1), 0, st, a.s);
   CS_LAUNCH_CHECK();
 }
 
@@ -303,7 +305,6 @@ void pcg_iteration_split_b(hipStream_t st, const PcgArgs& a, int k) {
   const dim3 g(red_grid(a.me.n)), b(kBlock);
   hipLaunchKernelGGL(k_pcg_rz, g, b, 0, st, a);
   hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a);
-  hipLaunchKernelGGL(k_pcg_settle, dim3(1), dim3(1), 0, st, a.s);
   CS_LAUNCH_CHECK();
 }
 

@@ -343,7 +343,7 @@ struct Solver {
       launch_neg(st, g.ptr, b.ptr, n3);
       int iters = 0;
       if (mixed) {
-        iters = mixed->solve(*this, invH2);
+        iters = mixed_solve(invH2, pcgMaxIters);
       } else {
         iters = pcg_solve(invH2, pcgMaxIters);
       }
@@ -446,16 +446,47 @@ struct Solver {
     return a;
   }
 
+  int mixed_solve(double invH2, int maxIters) {
+    MixedInputs in;
+    in.me = dmesh();
+    in.mat = dmat();
+    in.inv_h2 = invH2;
+    in.x = x.ptr;
+    in.b = b.ptr;
+    in.diag = diag.ptr;
+    in.binv = nullptr;
+    if (opt.precond == CS_PRECOND_BLOCK_JACOBI) {
+      const int e0 = ev_mark();
+      launch_block_factors(st, in.me, in.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
+      ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
+      in.binv = binv.ptr;
+    }
+    in.dx = dx.ptr;
+    if (contact) in.contact = contact->hvp_hook();
+    in.max_iters = maxIters;
+    in.tol = 1e-3;
+    int mark = -1;
+    return mixed->solve(in, [&](int, bool begin) {
+      if (begin) mark = ev_mark();
+      else ev_span(CS_KERNEL_HVP, mark, ev_mark());
+    });
+  }
+
   int pcg_solve(double invH2, int maxIters) {
     PcgArgs a = pcg_args(invH2);
-    if (a.precond == PRE_BLOCK) launch_block_factors(st, a.me, a.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
+    if (a.precond == PRE_BLOCK) {
+      const int e0 = ev_mark();
+      launch_block_factors(st, a.me, a.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
+      ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
+    }
     pcg_init(st, a);
     const int chunk = 8;
     for (int k = 0; k < maxIters; k++) {
       if (contact) contact->launch_hvp_coeffs(p.ptr, &ds()->pcg.done);
       const int e0 = ev_mark();
-      pcg_iteration(st, a, k);
+      pcg_iteration_hvp(st, a, k);
       ev_span(CS_KERNEL_HVP, e0, ev_mark());
+      pcg_iteration_rest(st, a, k);
       if ((k + 1) % chunk == 0 || k + 1 == maxIters) {
         const PcgScalars ps = read(&ds()->pcg);
         if (ps.done) break;
@@ -597,7 +628,6 @@ int cs_set_material(cs_solver* s, const cs_material* mat) {
   if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
   if (!mat) return fail(CS_E_INVALID, "null material");
   s->s.mat = *mat;
-  if (s->s.mixed) s->s.mixed->set_material(*mat);
   return CS_OK;
 }
 

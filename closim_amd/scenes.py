@@ -68,10 +68,15 @@ def strip_pull_targets(scene, step):
     return out
 
 
-def sphere_drape(nx=255, seed=0x5EED01, d_hat=1e-3, mu=0.3):
+def sphere_drape(nx=255, seed=0x5EED01, d_hat=1e-3, mu=0.3, sphere_y=0.2):
     """Sphere + floor drape: (nx+1)^2 sheet, 1 m side at y = 0.5 with seeded
     +-1e-4 m jitter per coordinate, static sphere r = 0.25 centred
-    (0.5, 0.25, 0.5) at 64x32 resolution, floor y = 0."""
+    (0.5, sphere_y, 0.5) at 64x32 resolution, floor y = 0.
+
+    The sphere centre sits at y = 0.2 (top 5 cm below the sheet): a sphere
+    resting on the floor would touch the sheet at y = 0.5 and the jitter
+    would start vertices inside d_min, where every line-search trial of the
+    reference's checkTrial is rejected and the sheet never moves."""
     pos, uv, idx = build_grid(nx, nx, 1.0, 1.0)
     P = pos.reshape(-1, 3).copy()
     P[:, 1] = 0.5
@@ -81,7 +86,7 @@ def sphere_drape(nx=255, seed=0x5EED01, d_hat=1e-3, mu=0.3):
     scene.positions += (jit * 2 - 1) * 1e-4
     c = ContactSystem(ContactParams(dHatM=d_hat, kappaJ=50.0, dMinM=1e-4, frictionMu=mu, frictionEpsM=1e-4))
     c.setFloor(0.0)
-    sp, si = sphere_mesh(0.25, 64, 32, center=(0.5, 0.25, 0.5))
+    sp, si = sphere_mesh(0.25, 64, 32, center=(0.5, sphere_y, 0.5))
     c.setStaticMesh(sp, si)
     scene.contact = c
     scene.meta = dict(steps=120, name=f"sphere_floor_drape_{nx + 1}x{nx + 1}")
