@@ -9,8 +9,10 @@ Prints one JSON line:
   value        mixed-precision HipSolver steps/s (state resident in HBM;
                host copy-out of positions is excluded and reported apart)
   fp64         the FP64 HipSolver (reference operator) on the same scene
-  roofline     HVP kernel: algorithmic bytes per launch / mean launch time
-               (HIP events on the solver's stream) vs the 8 TB/s HBM peak
+  roofline     HVP kernel (k_mx_hvp): algorithmic bytes per launch / mean
+               launch time of back-to-back launches on the operands the
+               timed steps left resident (HIP events on the solver's
+               stream), vs the 8 TB/s HBM peak
   cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample
 """
 import argparse
@@ -100,6 +102,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--hvp-reps", type=int, default=200)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per HVP launch from the FETCH_SIZE/WRITE_SIZE counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the cloth step is single-device); got --gpus {args.gpus}",
@@ -114,40 +119,40 @@ def main():
     mesh = sc_m.mesh
     print(f"[bench] workload {args.workload}: {mesh.count} vertices, {mesh.tri_count} triangles", file=sys.stderr,
           flush=True)
-    gm = solver_for(sc_m, "mixed", precond, timing=True)
+    gm = solver_for(sc_m, "mixed", precond, timing=False)
     gd = None
     if not args.no_fp64:
         sc_d = make_scene(args.workload)
-        gd = solver_for(sc_d, "fp64", precond, timing=True)
+        gd = solver_for(sc_d, "fp64", precond, timing=False)
     # warmup (both solvers advance from the same initial state)
     for k in range(args.warmup):
         raw_step(gm, h)
         if gd:
             raw_step(gd, h)
         print(f"[bench] warmup step {k} done", file=sys.stderr, flush=True)
-    gm.reset_timing()
     t_m = timed_steps(gm, h, args.steps)
     print(f"[bench] mixed: {args.steps} steps in {t_m:.3f} s", file=sys.stderr, flush=True)
-    kt = gm.kernel_timing()
     st_m = gm.lastStats
+    hvp_ms, hvp_bytes = gm.time_hvp(args.hvp_reps)
+    print(f"[bench] k_mx_hvp: {1000 * hvp_ms:.2f} us x {args.hvp_reps} launches", file=sys.stderr, flush=True)
     res = {}
     if gd:
-        gd.reset_timing()
         t_d = timed_steps(gd, h, args.steps)
         print(f"[bench] fp64: {args.steps} steps in {t_d:.3f} s", file=sys.stderr, flush=True)
-        kt_d = gd.kernel_timing()
+        st_d = gd.lastStats
+        hvp_ms_d, hvp_bytes_d = gd.time_hvp(args.hvp_reps)
         gm._copy_out()
         gd._copy_out()
         pm, pd = gm.getPositions(), gd.getPositions()
         err = float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd)))
         res["fp64"] = dict(steps_per_s=args.steps / t_d, ms_per_step=1000 * t_d / args.steps,
-                           hvp_mean_us=1000 * kt_d["hvp"]["total_ms"] / max(kt_d["hvp"]["count"], 1),
-                           hvp_launches=kt_d["hvp"]["count"])
+                           hvp_kernel="k_pcg_hvp (FP64 finite-difference membrane HVP, the CpuSolver operator)",
+                           hvp_mean_us=1000 * hvp_ms_d, hvp_gbs=hvp_bytes_d / (hvp_ms_d / 1000) / 1e9,
+                           newton_used_last_step=st_d["newton_used"], pcg_iters_last_step=st_d["pcg_iters"])
         res["mixed_over_fp64"] = t_d / t_m
         res["pos_rel_err_mixed_vs_fp64"] = err
-    hvp = kt["hvp"]
-    mean_s = hvp["total_ms"] / max(hvp["count"], 1) / 1000.0
-    achieved = hvp["bytes"] / mean_s / 1e9 if mean_s > 0 else 0.0
+    mean_s = hvp_ms / 1000.0
+    achieved = hvp_bytes / mean_s / 1e9 if mean_s > 0 else 0.0
     # PCIe-inclusive rate (positions + velocities copied out each step)
     t0 = time.perf_counter()
     for _ in range(2):
@@ -167,17 +172,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 inner solve / f64 Newton",
         "data": "synthetic",
-        "config": {"workload": make_scene.__name__ and sc_m.meta.get("name", args.workload),
+        "config": {"workload": sc_m.meta.get("name", args.workload),
                    "vertices": mesh.count, "triangles": mesh.tri_count, "h": h,
                    "preconditioner": "block-jacobi" if precond == _abi.CS_PRECOND_BLOCK_JACOBI else "jacobi",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_mx_hvp (FP32 analytic HVP + mass + contact)",
-                     "bytes_per_launch": hvp["bytes"], "mean_launch_us": mean_s * 1e6, "launches": hvp["count"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                     "kernel": "k_mx_hvp (FP32 analytic membrane HVP + mass + frozen-contact HVP)",
+                     "bytes_per_launch": hvp_bytes, "mean_launch_us": mean_s * 1e6, "launches": args.hvp_reps},
         "newton_used_last_step": st_m["newton_used"],
         "pcg_iters_last_step": st_m["pcg_iters"],
         "active_contacts_last_step": st_m["contact"]["active_pairs"] if st_m["contact"] else 0,
+        "candidate_prims_last_step": st_m["contact"]["candidate_pairs"] if st_m["contact"] else 0,
         "pcie_inclusive_steps_per_s": pcie_rate,
     }
     out.update(res)

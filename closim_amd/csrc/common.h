@@ -63,9 +63,10 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
 
 // Bounded grid for reduction kernels: a fixed count per size keeps the
 // partial layout (and thus the reduction tree) deterministic.
+constexpr unsigned kMaxRedBlocks = 1024;  // partial slots per reduction
 inline unsigned red_grid(size_t n, int block = kBlock) {
   size_t g = (n + block - 1) / block;
-  if (g > 1024) g = 1024;
+  if (g > kMaxRedBlocks) g = kMaxRedBlocks;
   return (unsigned)(g ? g : 1);
 }
 
@@ -94,38 +95,35 @@ __device__ __forceinline__ jsm::dd block_reduce_dd(jsm::dd v) {
   return r;
 }
 
-// Grid reduction with the "last block finishes" ticket: every block writes its
-// partial, the block that takes the last ticket merges all partials in block
-// order and returns true (in thread 0) with the total in `out`.
-// `ticket` must be zero on entry; it is reset by the last block.
+// Grid reductions are two-phase and fence-free: each block writes its
+// partial to partials[blockIdx.x]; a later kernel (the consumer, or a
+// one-block finish kernel when the host reads the value) merges the
+// partials in a fixed order.  On gfx950 a per-block release fence costs an
+// L2 write-back each, which serialises the blocks; the kernel boundary
+// provides the same visibility once per launch.
 template <int BLOCK>
-__device__ __forceinline__ bool grid_reduce_dd(jsm::dd v, jsm::dd* partials, unsigned* ticket, jsm::dd& out) {
-  jsm::dd b = block_reduce_dd<BLOCK>(v);
-  __shared__ bool s_last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = b;
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  jsm::dd acc = jsm::dd_zero();
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK) {
-    acc = jsm::dd_merge(acc, partials[i]);
-  }
-  acc = block_reduce_dd<BLOCK>(acc);
-  if (threadIdx.x == 0) {
-    *ticket = 0;
-    out = acc;
-  }
-  return threadIdx.x == 0;
+__device__ __forceinline__ void block_partial_dd(jsm::dd v, jsm::dd* partials) {
+  const jsm::dd b = block_reduce_dd<BLOCK>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
-// Max / min reductions (exact, order independent) with the same ticket
-// pattern.  Op picks the JS semantics: JsMax propagates NaN like Math.max;
-// GtMax / LtMin ignore NaN like `if (v > best) best = v` loops.
+// Merge partials[0..count) (fixed tree); every thread of the block gets the total.
+template <int BLOCK>
+__device__ __forceinline__ jsm::dd merge_partials_dd(const jsm::dd* partials, unsigned count) {
+  __shared__ double s_tot[2];
+  jsm::dd acc = jsm::dd_zero();
+  for (unsigned i = threadIdx.x; i < count; i += BLOCK) acc = jsm::dd_merge(acc, partials[i]);
+  acc = block_reduce_dd<BLOCK>(acc);
+  if (threadIdx.x == 0) { s_tot[0] = acc.hi; s_tot[1] = acc.lo; }
+  __syncthreads();
+  const jsm::dd r{s_tot[0], s_tot[1]};
+  __syncthreads();
+  return r;
+}
+
+// Max / min reductions (exact, order independent).  Op picks the JS
+// semantics: JsMax propagates NaN like Math.max; GtMax / LtMin ignore NaN
+// like `if (v > best) best = v` loops (their inputs are never NaN here).
 struct JsMax {
   __device__ static double ident() { return -__builtin_inf(); }
   __device__ static double op(double a, double b) { return jsm::jmax(a, b); }
@@ -139,10 +137,11 @@ struct LtMin {
   __device__ static double op(double a, double b) { return b < a ? b : a; }
 };
 
+// Block-wide Op reduction; every thread gets the result.
 template <int BLOCK, typename Op>
-__device__ __forceinline__ bool grid_reduce_ext(double v, double* partials, unsigned* ticket, double& out) {
+__device__ __forceinline__ double block_reduce_ext(double v) {
   __shared__ double s[BLOCK / kWave];
-  __shared__ bool s_last;
+  __shared__ double s_res;
   for (int off = kWave / 2; off > 0; off >>= 1) v = Op::op(v, __shfl_down(v, off, kWave));
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   if (lane == 0) s[wid] = v;
@@ -150,22 +149,49 @@ __device__ __forceinline__ bool grid_reduce_ext(double v, double* partials, unsi
   if (threadIdx.x == 0) {
     double b = s[0];
     for (int w = 1; w < BLOCK / kWave; w++) b = Op::op(b, s[w]);
-    partials[blockIdx.x] = b;
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    s_last = (t == gridDim.x - 1);
+    s_res = b;
   }
   __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    double m = Op::ident();
-    for (unsigned i = 0; i < gridDim.x; i++) m = Op::op(m, partials[i]);
-    *ticket = 0;
-    out = m;
-    return true;
-  }
-  return false;
+  const double r = s_res;
+  __syncthreads();
+  return r;
 }
+
+template <int BLOCK, typename Op>
+__device__ __forceinline__ void block_partial_ext(double v, double* partials) {
+  const double b = block_reduce_ext<BLOCK, Op>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+template <int BLOCK, typename Op>
+__device__ __forceinline__ double merge_partials_ext(const double* partials, unsigned count) {
+  double acc = Op::ident();
+  for (unsigned i = threadIdx.x; i < count; i += BLOCK) acc = Op::op(acc, partials[i]);
+  return block_reduce_ext<BLOCK, Op>(acc);
+}
+
+// A latch read once per block: thread 0 loads it and the block shares the
+// value, so all threads of a block take the same early exit even when
+// block 0 of the same launch sets the latch meanwhile.
+__device__ __forceinline__ bool block_latched(const int32_t* flag) {
+  __shared__ int32_t s_flag;
+  if (threadIdx.x == 0) s_flag = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return s_flag != 0;
+}
+
+// Host-read reductions: one block per job merges its partials.
+struct FinishJob {
+  const void* partials;
+  unsigned count;
+  int kind;  // FIN_DD, FIN_JSMAX, FIN_GTMAX, FIN_LTMIN
+  int clamp0;  // result = Op(0, result) (max(0, .) for the max kinds)
+  double* out;
+};
+enum { FIN_DD = 0, FIN_JSMAX = 1, FIN_GTMAX = 2, FIN_LTMIN = 3 };
+struct FinishJobs {
+  FinishJob j[4];
+};
+void launch_finish(hipStream_t st, const FinishJobs& jobs, int njobs);
 
 }  // namespace cloth

@@ -30,12 +30,11 @@ class ContactSystem {
   void kill_approach_velocity(const double* x, double* v); // newton.ts:238-303
   cs_contact_diag diagnostics() const;
   DContactHvp hvp_hook();
-  // dd[c] = coeff[c] * (J_c . p) for the current PCG direction.
-  void launch_hvp_coeffs(const double* p, const int32_t* done);
   uint64_t active_count() const;
   // Copy the active list (kind,p,a,b,c,d + w0..w3,nx,ny,nz) to host arrays.
   uint64_t copy_active(int32_t* ids6, double* vals7, uint64_t cap);
   uint64_t candidate_count() const;
+  uint64_t incidence_count() const;
   ContactImpl* impl() { return im_; }
 
  private:

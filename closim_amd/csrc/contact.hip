@@ -45,6 +45,7 @@ struct DGeom {
 struct CScalars {
   double red[16];
   unsigned tickets[16];
+  uint64_t mink[16];   // okey images of LtMin reductions (all ones = no value)
   uint32_t pair_count, spair_count, active_count, entry_count, nonfinite, floor_fail;
   uint32_t kind_count[4];
   uint64_t nan_key, cross_key;
@@ -64,40 +65,185 @@ __device__ __forceinline__ bool box_overlap(const double* a, const double* b) {
 }
 
 // ---------------------------------------------------------------- LBVH
+//
+// The candidate set of selfPairs (bvh.ts:79-124) is defined by the inclusive
+// swept-AABB predicate and the shared-vertex exclusion alone, so any
+// hierarchy whose inner boxes contain their leaves finds it exactly.  Here the
+// hierarchy is implicit: leaves are the triangles in 30-bit Morton order and
+// level l node k covers leaves [k 8^l, (k+1) 8^l).  Inner boxes are exact
+// double unions (NaN-ignoring), leaf tests use the reference's own box values.
+// A refit is one fused kernel (leaf boxes + three levels by wave shuffles) and
+// one small block for the levels above; the Morton order is reused until the
+// owner asks for a re-sort, so most builds skip the sort.
 
-// sweptTriAabb (aabb.ts:9-31) + box centre bounds for the Morton frame.
-__global__ void __launch_bounds__(kBlock) k_tri_boxes(DGeom G, const double* __restrict__ x0,
-                                                      const double* __restrict__ x1, double pad, double* boxes,
-                                                      double* partials, unsigned* tickets, double* bounds) {
-  double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-  double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < G.m; t += gridDim.x * blockDim.x) {
-    double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
-    double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
-    for (int c = 0; c < 3; c++) {
-      const size_t v = G.idx[(size_t)t * 3 + c];
-      const double* xs[2] = {x0, x1};
-      for (int q = 0; q < 2; q++) {
-        for (int k = 0; k < 3; k++) {
-          const double pk = xs[q][v * 3 + k];
-          if (pk < mn[k]) mn[k] = pk;
-          if (pk > mx[k]) mx[k] = pk;
-        }
+constexpr int kFan = 8;
+constexpr int kMaxLevels = 9;   // stack bound 7 L + 1 <= 64 entries per wave
+constexpr int kTravStack = 64;
+
+struct Leaf {     // 64 B
+  double lo[3], hi[3];
+  int32_t v[3];   // cloth vertex ids of the triangle
+  uint32_t tri;   // triangle id
+};
+
+struct Levels {   // level l (1..L) holds cnt[l] inner boxes at nodes + off[l] * 6
+  int L;
+  uint32_t cnt[kMaxLevels + 1];
+  uint32_t off[kMaxLevels + 1];
+  uint32_t span[kMaxLevels + 1];  // leaves under one level-l node (8^l)
+};
+
+static Levels make_levels(uint32_t m, size_t* total_nodes) {
+  Levels LV;
+  std::memset(&LV, 0, sizeof(LV));
+  LV.cnt[0] = m;
+  LV.span[0] = 1;
+  uint32_t off = 0;
+  int L = 0;
+  uint32_t c = m;
+  while (c > 1) {
+    if (L == kMaxLevels) throw std::invalid_argument("mesh too large for the 8-ary LBVH (more than 8^9 triangles)");
+    c = (c + kFan - 1) / kFan;
+    L++;
+    LV.cnt[L] = c;
+    LV.off[L] = off;
+    LV.span[L] = LV.span[L - 1] * kFan;
+    off += c;
+  }
+  LV.L = L;
+  *total_nodes = off;
+  return LV;
+}
+
+// Order-preserving u64 image of a double (for atomic min / max of non-NaN values).
+__device__ __forceinline__ uint64_t okey(double d) {
+  const uint64_t u = (uint64_t)__double_as_longlong(d);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double okey_dec(uint64_t k) {
+  const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+
+// Block-wide `if (v < best) best = v` (NaN never taken) folded into a global
+// okey slot with one atomic per block.  Every thread of the block must call it.
+__device__ __forceinline__ void block_min_to(double v, uint64_t* slot) {
+  __shared__ double s_min[kBlock / kWave];
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off, kWave);
+    if (o < v) v = o;
+  }
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0) s_min[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s_min[0];
+    for (int w = 1; w < kBlock / kWave; w++)
+      if (s_min[w] < r) r = s_min[w];
+    if (r < __builtin_inf()) atomicMin((unsigned long long*)slot, (unsigned long long)okey(r));
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool box_overlap6(const double* a, const double* lo, const double* hi) {
+  return a[0] <= hi[0] && a[3] >= lo[0] && a[1] <= hi[1] && a[4] >= lo[1] && a[2] <= hi[2] && a[5] >= lo[2];
+}
+
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+// Wave-aggregated allocation: the lanes with `want` get consecutive slots
+// from one atomic per wave.  Every lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_alloc(bool want, uint32_t* counter) {
+  const uint64_t mask = __ballot(want);
+  if (!mask) return 0;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if ((int)(threadIdx.x % kWave) == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, kWave);
+  return base + rank_in(mask);
+}
+
+// Per-wave staging of found pairs in LDS; one global atomic per flush.
+constexpr int kPairStage = 256;
+struct PairStage {
+  uint2* buf;  // kPairStage entries of this wave
+  int count;   // wave-uniform
+  __device__ __forceinline__ void push(bool hit, uint2 v, uint32_t* counter, uint2* out, uint32_t cap) {
+    const uint64_t mask = __ballot(hit);
+    if (!mask) return;
+    if (hit) buf[count + rank_in(mask)] = v;
+    count += __popcll(mask);
+    if (count > kPairStage - kWave) flush(counter, out, cap);
+  }
+  __device__ __forceinline__ void flush(uint32_t* counter, uint2* out, uint32_t cap) {
+    if (count == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    const int lane = threadIdx.x % kWave;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(counter, (uint32_t)count);
+    base = __shfl(base, 0, kWave);
+    for (int t = lane; t < count; t += kWave)
+      if (base + t < cap) out[base + t] = buf[t];
+    __builtin_amdgcn_wave_barrier();
+    count = 0;
+  }
+};
+
+// sweptTriAabb (aabb.ts:9-31): min / max over the three vertices at x0 and
+// x1 in the reference's visiting order, then -/+ pad.
+__device__ __forceinline__ void swept_box(const DGeom& G, const double* x0, const double* x1, uint32_t t, double pad,
+                                          double* lo, double* hi, int32_t* v) {
+  double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+  double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+  for (int c = 0; c < 3; c++) {
+    const uint32_t vi = G.idx[(size_t)t * 3 + c];
+    v[c] = (int32_t)vi;
+    const double* xs[2] = {x0, x1};
+    for (int q = 0; q < 2; q++) {
+      for (int k = 0; k < 3; k++) {
+        const double pk = xs[q][(size_t)vi * 3 + k];
+        if (pk < mn[k]) mn[k] = pk;
+        if (pk > mx[k]) mx[k] = pk;
       }
     }
-    double* b = boxes + (size_t)t * 6;
-    for (int k = 0; k < 3; k++) {
-      b[k] = mn[k] - pad;
-      b[3 + k] = mx[k] + pad;
-      const double c = 0.5 * (b[k] + b[3 + k]);
-      if (c < lo[k]) lo[k] = c;
-      if (c > hi[k]) hi[k] = c;
-    }
   }
-  double r;
   for (int k = 0; k < 3; k++) {
-    if (grid_reduce_ext<kBlock, LtMin>(lo[k], partials + k * 1024, &tickets[k], r)) bounds[k] = r;
-    if (grid_reduce_ext<kBlock, GtMax>(hi[k], partials + (3 + k) * 1024, &tickets[3 + k], r)) bounds[3 + k] = r;
+    lo[k] = mn[k] - pad;
+    hi[k] = mx[k] + pad;
+  }
+}
+
+// Morton frame: bounds of the vertex positions (NaN ignored), one atomic per
+// block and axis.  bkeys = {lo x,y,z (min), hi x,y,z (max)} as okey images.
+__global__ void __launch_bounds__(kBlock) k_vert_bounds(uint32_t n, const double* __restrict__ x, uint64_t* bkeys) {
+  double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+  double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+    for (int k = 0; k < 3; k++) {
+      const double p = x[(size_t)v * 3 + k];
+      lo[k] = fmin(lo[k], p);
+      hi[k] = fmax(hi[k], p);
+    }
+  __shared__ double s[6][kBlock / kWave];
+  for (int k = 0; k < 3; k++)
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      lo[k] = fmin(lo[k], __shfl_down(lo[k], off, kWave));
+      hi[k] = fmax(hi[k], __shfl_down(hi[k], off, kWave));
+    }
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) { s[k][wid] = lo[k]; s[3 + k][wid] = hi[k]; }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double r = s[k][0];
+    for (int w = 1; w < kBlock / kWave; w++) r = k < 3 ? fmin(r, s[k][w]) : fmax(r, s[k][w]);
+    if (r == r && (k < 3 ? r < __builtin_inf() : r > -__builtin_inf())) {
+      if (k < 3) atomicMin((unsigned long long*)&bkeys[k], (unsigned long long)okey(r));
+      else atomicMax((unsigned long long*)&bkeys[k], (unsigned long long)okey(r));
+    }
   }
 }
 
@@ -109,16 +255,24 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   return v;
 }
 
-__global__ void k_morton(uint32_t m, const double* __restrict__ boxes, const double* __restrict__ bounds,
-                         uint64_t* keys, uint32_t* vals) {
+// 30-bit Morton code of each triangle centroid in the cube spanned by the
+// largest extent of the vertex bounds (equal cells on every axis, so a thin
+// sheet's jitter across its plane does not take over the high bits).
+__global__ void k_morton(DGeom G, const double* __restrict__ x, const uint64_t* __restrict__ bkeys, uint64_t* keys,
+                         uint32_t* vals) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  const double* b = boxes + (size_t)t * 6;
+  if (t >= G.m) return;
+  double ext = 0;
+  for (int k = 0; k < 3; k++) {
+    const double e = okey_dec(bkeys[3 + k]) - okey_dec(bkeys[k]);
+    if (e > ext) ext = e;
+  }
   uint32_t q[3];
   for (int k = 0; k < 3; k++) {
-    const double ext = bounds[3 + k] - bounds[k];
-    const double c = 0.5 * (b[k] + b[3 + k]);
-    double u = ext > 0 ? (c - bounds[k]) / ext : 0.0;
+    const double lo = okey_dec(bkeys[k]);
+    double c = 0;
+    for (int j = 0; j < 3; j++) c += x[(size_t)G.idx[(size_t)t * 3 + j] * 3 + k];
+    double u = ext > 0 ? (c * (1.0 / 3.0) - lo) / ext : 0.0;
     if (!(u >= 0)) u = 0;
     if (u > 1) u = 1;
     q[k] = (uint32_t)fmin(1023.0, u * 1024.0);
@@ -127,166 +281,199 @@ __global__ void k_morton(uint32_t m, const double* __restrict__ boxes, const dou
   vals[t] = t;
 }
 
-__device__ __forceinline__ int lcp(const uint64_t* codes, int m, int i, int j) {
-  if (j < 0 || j >= m) return -1;
-  const uint32_t ci = (uint32_t)codes[i], cj = (uint32_t)codes[j];
-  if (ci == cj) return 32 + __clz((uint32_t)(i ^ j));
-  return __clz(ci ^ cj);
+__device__ __forceinline__ void store_box(double* dst, const double* lo, const double* hi) {
+  for (int k = 0; k < 3; k++) { dst[k] = lo[k]; dst[3 + k] = hi[k]; }
 }
 
-// Karras 2012 hierarchy over sorted Morton codes: internal node i covers the
-// sorted leaf range [first, last]; children carry LEAF for leaves.
-__global__ void k_karras(int m, const uint64_t* __restrict__ codes, uint32_t* left, uint32_t* right,
-                         uint32_t* first, uint32_t* last, uint32_t* parent_int, uint32_t* parent_leaf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m - 1) return;
-  const int d = (lcp(codes, m, i, i + 1) - lcp(codes, m, i, i - 1)) >= 0 ? 1 : -1;
-  const int dmin = lcp(codes, m, i, i - d);
-  int lmax = 2;
-  while (lcp(codes, m, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (lcp(codes, m, i, i + (l + t) * d) > dmin) l += t;
-  const int j = i + l * d;
-  const int dnode = lcp(codes, m, i, j);
-  int s = 0, div = 2, t;
-  do {
-    t = (l + div - 1) / div;
-    if (lcp(codes, m, i, i + (s + t) * d) > dnode) s += t;
-    div *= 2;
-  } while (t > 1);
-  const int gamma = i + s * d + (d < 0 ? d : 0);
-  const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const uint32_t L = (lo == gamma) ? (LEAF | (uint32_t)gamma) : (uint32_t)gamma;
-  const uint32_t R = (hi == gamma + 1) ? (LEAF | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
-  left[i] = L;
-  right[i] = R;
-  first[i] = lo;
-  last[i] = hi;
-  if (L & LEAF) parent_leaf[L & ~LEAF] = i; else parent_int[L] = i;
-  if (R & LEAF) parent_leaf[R & ~LEAF] = i; else parent_int[R] = i;
-}
-
-__device__ __forceinline__ const double* child_box(uint32_t c, const uint32_t* sorted_tri, const double* boxes,
-                                                   const double* nodebox) {
-  return (c & LEAF) ? boxes + (size_t)sorted_tri[c & ~LEAF] * 6 : nodebox + (size_t)c * 6;
-}
-
-// Bottom-up refit: the second child to arrive at a node writes its union.
-__global__ void k_refit(int m, const uint32_t* __restrict__ sorted_tri, const double* __restrict__ boxes,
-                        const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
-                        const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
-                        double* nodebox, unsigned* flags) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint32_t node = parent_leaf[i];
-  while (node != NONE) {
-    __threadfence();
-    if (atomicAdd(&flags[node], 1u) == 0) return;
-    __threadfence();
-    const double* a = child_box(left[node], sorted_tri, boxes, nodebox);
-    const double* b = child_box(right[node], sorted_tri, boxes, nodebox);
-    double* o = nodebox + (size_t)node * 6;
+// Refit: one wave per 512 sorted leaves.  Lane L owns leaves 8L..8L+7 of its
+// wave's range: writes their records and the level-1 box; levels 2 and 3
+// come from xor-shuffles over 8 and 64 lanes.
+__global__ void __launch_bounds__(kBlock) k_refit8(DGeom G, const double* __restrict__ x0,
+                                                   const double* __restrict__ x1, double pad,
+                                                   const uint32_t* __restrict__ order, Leaf* leaves, double* nodes,
+                                                   Levels LV) {
+  const uint32_t lane = threadIdx.x % kWave;
+  const uint32_t wbase = (blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * kFan);
+  double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+  double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+  for (int c = 0; c < kFan; c++) {
+    const uint32_t j = wbase + lane * kFan + c;
+    if (j >= G.m) break;
+    Leaf R;
+    const uint32_t t = order[j];
+    swept_box(G, x0, x1, t, pad, R.lo, R.hi, R.v);
+    R.tri = t;
+    leaves[j] = R;
     for (int k = 0; k < 3; k++) {
-      o[k] = jsm::jmin(a[k], b[k]);
-      o[3 + k] = jsm::jmax(a[3 + k], b[3 + k]);
+      lo[k] = fmin(lo[k], R.lo[k]);
+      hi[k] = fmax(hi[k], R.hi[k]);
     }
-    node = parent_int[node];
+  }
+  const uint32_t n1 = wbase / kFan + lane;
+  if (LV.L >= 1 && n1 < LV.cnt[1]) store_box(nodes + ((size_t)LV.off[1] + n1) * 6, lo, hi);
+  for (int o = 1; o < kFan; o <<= 1)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, kWave));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, kWave));
+    }
+  const uint32_t n2 = wbase / (kFan * kFan) + lane / kFan;
+  if (LV.L >= 2 && lane % kFan == 0 && n2 < LV.cnt[2]) store_box(nodes + ((size_t)LV.off[2] + n2) * 6, lo, hi);
+  for (int o = kFan; o < kWave; o <<= 1)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fmin(lo[k], __shfl_xor(lo[k], o, kWave));
+      hi[k] = fmax(hi[k], __shfl_xor(hi[k], o, kWave));
+    }
+  const uint32_t n3 = wbase / (kFan * kFan * kFan);
+  if (LV.L >= 3 && lane == 0 && n3 < LV.cnt[3]) store_box(nodes + ((size_t)LV.off[3] + n3) * 6, lo, hi);
+}
+
+// Levels 4..L in one block (level 3 has m/512 nodes).
+__global__ void __launch_bounds__(1024) k_refit_top(double* nodes, Levels LV) {
+  for (int l = 4; l <= LV.L; l++) {
+    for (uint32_t k = threadIdx.x; k < LV.cnt[l]; k += blockDim.x) {
+      double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+      double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+      const uint32_t c0 = k * kFan, c1 = min(c0 + kFan, LV.cnt[l - 1]);
+      for (uint32_t c = c0; c < c1; c++) {
+        const double* b = nodes + ((size_t)LV.off[l - 1] + c) * 6;
+        for (int q = 0; q < 3; q++) {
+          lo[q] = fmin(lo[q], b[q]);
+          hi[q] = fmax(hi[q], b[3 + q]);
+        }
+      }
+      store_box(nodes + ((size_t)LV.off[l] + k) * 6, lo, hi);
+    }
+    __syncthreads();
   }
 }
 
-constexpr int kStack = 64;
+__device__ __forceinline__ uint32_t stk_enc(int l, uint32_t k) { return ((uint32_t)l << 28) | k; }
 
 // selfPairs (bvh.ts:79-124): every unordered pair of overlapping triangles
 // sharing no vertex, found once from the leaf with the smaller sorted index.
-__global__ void __launch_bounds__(kBlock) k_traverse_self(DGeom G, const uint32_t* __restrict__ sorted_tri,
-                                                          const double* __restrict__ boxes,
-                                                          const uint32_t* __restrict__ left,
-                                                          const uint32_t* __restrict__ right,
-                                                          const uint32_t* __restrict__ last,
-                                                          const double* __restrict__ nodebox, uint2* pairs,
+// Wave-packet traversal: a wave queries its 64 consecutive leaves together;
+// a node is entered when any lane's query box overlaps it and a lane can
+// still pair with a later leaf under it.  The node stack is wave-uniform in LDS.
+__global__ void __launch_bounds__(kBlock) k_traverse_self(const Leaf* __restrict__ leaves,
+                                                          const double* __restrict__ nodes, Levels LV, uint2* pairs,
                                                           uint32_t cap, CScalars* cs) {
-  const int m = (int)G.m;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || m < 2) return;
-  const uint32_t ti = sorted_tri[i];
+  __shared__ uint32_t stk[kBlock / kWave][kTravStack];
+  __shared__ uint2 stage[kBlock / kWave][kPairStage];
+  const int wid = threadIdx.x / kWave;
+  const uint32_t m = LV.cnt[0];
+  PairStage ps{stage[wid], 0};
+  const uint32_t i = (blockIdx.x * (kBlock / kWave) + wid) * kWave + threadIdx.x % kWave;
+  const bool act = i < m;
   double Q[6];
-  for (int k = 0; k < 6; k++) Q[k] = boxes[(size_t)ti * 6 + k];
-  const uint32_t A0 = G.idx[(size_t)ti * 3], A1 = G.idx[(size_t)ti * 3 + 1], A2 = G.idx[(size_t)ti * 3 + 2];
-  uint32_t stack[kStack];
+  int32_t A[3] = {-1, -1, -1};
+  uint32_t ti = 0;
+  if (act) {
+    const Leaf& R = leaves[i];
+    for (int k = 0; k < 3; k++) { Q[k] = R.lo[k]; Q[3 + k] = R.hi[k]; A[k] = R.v[k]; }
+    ti = R.tri;
+  } else {
+    for (int k = 0; k < 6; k++) Q[k] = __builtin_nan("");
+  }
+  if (LV.L == 0) return;
+  uint32_t* S = stk[wid];
   int sp = 0;
-  stack[sp++] = 0;
+  S[sp++] = stk_enc(LV.L, 0);
+  bool overflow = false;
   while (sp > 0) {
-    const uint32_t node = stack[--sp];
-    const uint32_t ch[2] = {left[node], right[node]};
-    for (int c = 0; c < 2; c++) {
-      const uint32_t cc = ch[c];
-      if (cc & LEAF) {
-        const int j = (int)(cc & ~LEAF);
-        if (j <= i) continue;
-        const uint32_t tj = sorted_tri[j];
-        if (!box_overlap(Q, boxes + (size_t)tj * 6)) continue;
-        const uint32_t B0 = G.idx[(size_t)tj * 3], B1 = G.idx[(size_t)tj * 3 + 1], B2 = G.idx[(size_t)tj * 3 + 2];
-        if (A0 == B0 || A0 == B1 || A0 == B2 || A1 == B0 || A1 == B1 || A1 == B2 || A2 == B0 || A2 == B1 ||
-            A2 == B2)
-          continue;
-        const uint32_t k = atomicAdd(&cs->pair_count, 1u);
-        if (k < cap) pairs[k] = make_uint2(ti < tj ? ti : tj, ti < tj ? tj : ti);
-      } else {
-        if ((int)last[cc] <= i) continue;
-        if (!box_overlap(Q, nodebox + (size_t)cc * 6)) continue;
-        if (sp < kStack) stack[sp++] = cc;
-        else atomicOr(&cs->nonfinite, 2u);  // traversal stack overflow
+    const uint32_t e = __builtin_amdgcn_readfirstlane(S[--sp]);
+    const int cl = (int)(e >> 28) - 1;
+    const uint32_t k = e & 0x0fffffffu;
+    const uint32_t c0 = k * kFan, c1 = min(c0 + kFan, LV.cnt[cl]);
+    if (cl == 0) {
+      for (uint32_t j = c0; j < c1; j++) {
+        const Leaf& R = leaves[j];
+        const bool hit = act && j > i && box_overlap6(Q, R.lo, R.hi) &&
+                         !(A[0] == R.v[0] || A[0] == R.v[1] || A[0] == R.v[2] || A[1] == R.v[0] ||
+                           A[1] == R.v[1] || A[1] == R.v[2] || A[2] == R.v[0] || A[2] == R.v[1] || A[2] == R.v[2]);
+        const uint32_t tj = R.tri;
+        ps.push(hit, make_uint2(ti < tj ? ti : tj, ti < tj ? tj : ti), &cs->pair_count, pairs, cap);
+      }
+    } else {
+      for (uint32_t c = c1; c-- > c0;) {
+        const double* b = nodes + ((size_t)LV.off[cl] + c) * 6;
+        const uint64_t last = min((uint64_t)(c + 1) * LV.span[cl], (uint64_t)m) - 1;
+        const bool hit = act && last > i && box_overlap6(Q, b, b + 3);
+        if (__any(hit)) {
+          if (sp < kTravStack) S[sp++] = stk_enc(cl, c);
+          else overflow = true;
+        }
       }
     }
   }
+  ps.flush(&cs->pair_count, pairs, cap);
+  if (overflow && (threadIdx.x % kWave) == 0) atomicOr(&cs->nonfinite, 2u);
 }
 
-// queryBvhAll (contact-assembly.ts:124-140): cloth triangles overlapping each
-// padded static triangle box (staticTriAabb :597-605).
-__global__ void __launch_bounds__(kBlock) k_traverse_static(DGeom G, double pad, const uint32_t* __restrict__ sorted_tri,
-                                                            const double* __restrict__ boxes,
-                                                            const uint32_t* __restrict__ left,
-                                                            const uint32_t* __restrict__ right,
-                                                            const double* __restrict__ nodebox, uint2* pairs,
-                                                            uint32_t cap, CScalars* cs) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= G.ms) return;
+// queryBvhAll (contact-assembly.ts:124-140): cloth triangles whose swept box
+// overlaps a static triangle box padded by `pad` (staticTriAabb :597-605).
+// Queries are the cloth leaves (wave packets as above) against a fixed
+// hierarchy over the static triangles holding their unpadded boxes; the
+// padded leaf box is formed exactly as the reference forms it, and inner
+// boxes are widened by pad, which keeps them containing the padded leaves.
+__global__ void __launch_bounds__(kBlock) k_traverse_static(const Leaf* __restrict__ leaves, uint32_t m,
+                                                            const Leaf* __restrict__ sleaves,
+                                                            const double* __restrict__ snodes, Levels SL, double pad,
+                                                            uint2* pairs, uint32_t cap, CScalars* cs) {
+  __shared__ uint32_t stk[kBlock / kWave][kTravStack];
+  __shared__ uint2 stage[kBlock / kWave][kPairStage];
+  const int wid = threadIdx.x / kWave;
+  const uint32_t i = (blockIdx.x * (kBlock / kWave) + wid) * kWave + threadIdx.x % kWave;
+  const bool act = i < m;
+  PairStage ps{stage[wid], 0};
   double Q[6];
-  {
-    double P[3][3];
-    for (int c = 0; c < 3; c++) vpos(G, nullptr, -(int)G.sidx[(size_t)s * 3 + c] - 2, P[c]);
-    for (int k = 0; k < 3; k++) {
-      Q[k] = jsm::jmin(jsm::jmin(P[0][k], P[1][k]), P[2][k]) - pad;
-      Q[3 + k] = jsm::jmax(jsm::jmax(P[0][k], P[1][k]), P[2][k]) + pad;
-    }
+  uint32_t ti = 0;
+  if (act) {
+    const Leaf& R = leaves[i];
+    for (int k = 0; k < 3; k++) { Q[k] = R.lo[k]; Q[3 + k] = R.hi[k]; }
+    ti = R.tri;
+  } else {
+    for (int k = 0; k < 6; k++) Q[k] = __builtin_nan("");
   }
-  if (G.m == 1) {
-    if (box_overlap(boxes, Q)) {
-      const uint32_t k = atomicAdd(&cs->spair_count, 1u);
-      if (k < cap) pairs[k] = make_uint2(0, s);
-    }
+  uint32_t* S = stk[wid];
+  int sp = 0;
+  // the root is entered like any node: stack entries name the node whose
+  // children are tested; a single static triangle is tested directly.
+  const uint32_t ms = SL.cnt[0];
+  bool overflow = false;
+  auto test_leaf = [&](uint32_t j) {
+    const Leaf& R = sleaves[j];
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) { lo[k] = R.lo[k] - pad; hi[k] = R.hi[k] + pad; }
+    ps.push(act && box_overlap6(Q, lo, hi), make_uint2(ti, R.tri), &cs->spair_count, pairs, cap);
+  };
+  if (SL.L == 0) {
+    if (ms) test_leaf(0);
+    ps.flush(&cs->spair_count, pairs, cap);
     return;
   }
-  uint32_t stack[kStack];
-  int sp = 0;
-  stack[sp++] = 0;
+  S[sp++] = stk_enc(SL.L, 0);
   while (sp > 0) {
-    const uint32_t node = stack[--sp];
-    const uint32_t ch[2] = {left[node], right[node]};
-    for (int c = 0; c < 2; c++) {
-      const uint32_t cc = ch[c];
-      const double* bx = child_box(cc, sorted_tri, boxes, nodebox);
-      if (!box_overlap(bx, Q)) continue;
-      if (cc & LEAF) {
-        const uint32_t k = atomicAdd(&cs->spair_count, 1u);
-        if (k < cap) pairs[k] = make_uint2(sorted_tri[cc & ~LEAF], s);
-      } else {
-        if (sp < kStack) stack[sp++] = cc;
-        else atomicOr(&cs->nonfinite, 2u);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(S[--sp]);
+    const int cl = (int)(e >> 28) - 1;
+    const uint32_t k = e & 0x0fffffffu;
+    const uint32_t c0 = k * kFan, c1 = min(c0 + kFan, SL.cnt[cl]);
+    if (cl == 0) {
+      for (uint32_t j = c0; j < c1; j++) test_leaf(j);
+    } else {
+      for (uint32_t c = c1; c-- > c0;) {
+        const double* b = snodes + ((size_t)SL.off[cl] + c) * 6;
+        const double lo[3] = {b[0] - pad, b[1] - pad, b[2] - pad};
+        const double hi[3] = {b[3] + pad, b[4] + pad, b[5] + pad};
+        const bool hit = act && box_overlap6(Q, lo, hi);
+        if (__any(hit)) {
+          if (sp < kTravStack) S[sp++] = stk_enc(cl, c);
+          else overflow = true;
+        }
       }
     }
   }
+  ps.flush(&cs->spair_count, pairs, cap);
+  if (overflow && (threadIdx.x % kWave) == 0) atomicOr(&cs->nonfinite, 2u);
 }
 
 // ------------------------------------------------------------- prims
@@ -295,19 +482,23 @@ __global__ void __launch_bounds__(kBlock) k_traverse_static(DGeom G, double pad,
 // pairs; :105-114 for cloth/static pairs).
 struct Prim {
   int kind, p, a, b, c, d;
-  uint64_t key, nkey;
+  uint32_t r1, r2;  // VT: triangle id; EE: the two directed edge slots (tri * 3 + e)
 };
 
 __device__ __forceinline__ uint64_t rkey(const DGeom& G, uint64_t kind, uint64_t f1, uint64_t f2) {
   return (kind << (2 * G.B)) | (f1 << G.B) | f2;
 }
 
-__device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint32_t np, const uint2* spairs,
+__device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint32_t np, const uint2* spairs, uint32_t nsp,
                                         uint64_t tid, Prim& q) {
   const int TE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+  // prim ranges: cloth VT (6 per pair), cloth EE (9 per pair), static VT (3),
+  // static EE (9), so that a wave mostly runs one closest-point routine
   if (tid < (uint64_t)np * 15) {
-    const uint2 pr = pairs[tid / 15];
-    const int k = (int)(tid % 15);
+    const bool vt = tid < (uint64_t)np * 6;
+    const uint64_t t1 = vt ? tid : tid - (uint64_t)np * 6;
+    const uint2 pr = pairs[vt ? t1 / 6 : t1 / 9];
+    const int k = vt ? (int)(t1 % 6) : 6 + (int)(t1 % 9);
     const uint32_t tA = pr.x, tB = pr.y;
     const int A[3] = {(int)G.idx[(size_t)tA * 3], (int)G.idx[(size_t)tA * 3 + 1], (int)G.idx[(size_t)tA * 3 + 2]};
     const int B[3] = {(int)G.idx[(size_t)tB * 3], (int)G.idx[(size_t)tB * 3 + 1], (int)G.idx[(size_t)tB * 3 + 2]};
@@ -317,21 +508,21 @@ __device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint
       const int* T = fromA ? B : A;
       const uint32_t tri = fromA ? tB : tA;
       q.kind = 0; q.p = P[k % 3]; q.a = T[0]; q.b = T[1]; q.c = T[2]; q.d = -1;
-      q.key = rkey(G, 0, (uint64_t)q.p, G.tri_rank[tri]);
-      q.nkey = rkey(G, 0, (uint64_t)q.p, tri);
+      q.r1 = tri;
     } else {
       const int e = k - 6, ea = e / 3, eb = e % 3;
       q.kind = 1; q.p = -1;
       q.a = A[TE[ea][0]]; q.b = A[TE[ea][1]]; q.c = B[TE[eb][0]]; q.d = B[TE[eb][1]];
       const uint32_t de1 = tA * 3 + ea, de2 = tB * 3 + eb;
-      q.key = rkey(G, 1, G.dedge_rank[de1], G.dedge_rank[de2]);
-      const uint32_t u1 = G.uedge[de1], u2 = G.uedge[de2];
-      q.nkey = rkey(G, 1, u1 < u2 ? u1 : u2, u1 < u2 ? u2 : u1);
+      q.r1 = de1;
+      q.r2 = de2;
     }
   } else {
     const uint64_t t2 = tid - (uint64_t)np * 15;
-    const uint2 pr = spairs[t2 / 12];
-    const int k = (int)(t2 % 12);
+    const bool vt = t2 < (uint64_t)nsp * 3;
+    const uint64_t t3 = vt ? t2 : t2 - (uint64_t)nsp * 3;
+    const uint2 pr = spairs[vt ? t3 / 3 : t3 / 9];
+    const int k = vt ? (int)(t3 % 3) : 3 + (int)(t3 % 9);
     const uint32_t ct = pr.x, st = pr.y;
     const int A[3] = {(int)G.idx[(size_t)ct * 3], (int)G.idx[(size_t)ct * 3 + 1], (int)G.idx[(size_t)ct * 3 + 2]};
     const int B[3] = {-(int)G.sidx[(size_t)st * 3] - 2, -(int)G.sidx[(size_t)st * 3 + 1] - 2,
@@ -339,18 +530,27 @@ __device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint
     const uint32_t stri = G.m + st;
     if (k < 3) {
       q.kind = 0; q.p = A[k]; q.a = B[0]; q.b = B[1]; q.c = B[2]; q.d = -2;
-      q.key = rkey(G, 0, (uint64_t)q.p, G.tri_rank[stri]);
-      q.nkey = rkey(G, 0, (uint64_t)q.p, stri);
+      q.r1 = stri;
     } else {
       const int e = k - 3, ea = e / 3, eb = e % 3;
       q.kind = 1; q.p = -1;
       q.a = A[TE[ea][0]]; q.b = A[TE[ea][1]]; q.c = B[TE[eb][0]]; q.d = B[TE[eb][1]];
       const uint32_t de1 = ct * 3 + ea, de2 = stri * 3 + eb;
-      q.key = rkey(G, 1, G.dedge_rank[de1], G.dedge_rank[de2]);
-      const uint32_t u1 = G.uedge[de1], u2 = G.uedge[de2];
-      q.nkey = rkey(G, 1, u1 < u2 ? u1 : u2, u1 < u2 ? u2 : u1);
+      q.r1 = de1;
+      q.r2 = de2;
     }
   }
+}
+
+// Sort key (candidate order) and lagged-friction key of a prim.
+__device__ __forceinline__ uint64_t prim_key(const DGeom& G, const Prim& q) {
+  if (q.kind == 0) return rkey(G, 0, (uint64_t)q.p, G.tri_rank[q.r1]);
+  return rkey(G, 1, G.dedge_rank[q.r1], G.dedge_rank[q.r2]);
+}
+__device__ __forceinline__ uint64_t prim_nkey(const DGeom& G, const Prim& q) {
+  if (q.kind == 0) return rkey(G, 0, (uint64_t)q.p, q.r1);
+  const uint32_t u1 = G.uedge[q.r1], u2 = G.uedge[q.r2];
+  return rkey(G, 1, u1 < u2 ? u1 : u2, u1 < u2 ? u2 : u1);
 }
 
 __device__ __forceinline__ Closest prim_closest(const DGeom& G, const double* x, const Prim& q) {
@@ -372,31 +572,37 @@ __global__ void __launch_bounds__(kBlock) k_prims_active(DGeom G, const double* 
                                                          double* partials, CScalars* cs) {
   const uint64_t total = (uint64_t)np * 15 + (uint64_t)nsp * 12;
   double minD = __builtin_inf();
-  for (uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; tid < total;
-       tid += (uint64_t)gridDim.x * blockDim.x) {
+  // uniform trip count per block so that the wave-aggregated emission sees
+  // every lane
+  for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < total; base += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t tid = base + threadIdx.x;
+    bool emit = false;
     Prim q;
-    prim_of(G, pairs, np, spairs, tid, q);
-    const Closest cp = prim_closest(G, x, q);
-    if (cp.dist < minD) minD = cp.dist;
-    if (cp.dist >= G.d_hat || cp.dist < 1e-12) continue;
     Frozen f;
-    // frozen VT records carry d = -1 even for static candidates (contact-assembly.ts:181)
-    f.kind = q.kind; f.p = q.p; f.a = q.a; f.b = q.b; f.c = q.c; f.d = q.kind == 0 ? -1 : q.d;
-    f.nx = cp.rx / cp.dist; f.ny = cp.ry / cp.dist; f.nz = cp.rz / cp.dist;
-    if (q.kind == 0) {
-      f.w1 = cp.s; f.w2 = cp.t; f.w0 = 1 - cp.s - cp.t; f.w3 = 0;
-    } else {
-      f.w0 = 1 - cp.s; f.w1 = cp.s; f.w2 = -(1 - cp.t); f.w3 = -cp.t;
+    if (tid < total) {
+      prim_of(G, pairs, np, spairs, nsp, tid, q);
+      const Closest cp = prim_closest(G, x, q);
+      if (cp.dist < minD) minD = cp.dist;
+      if (!(cp.dist >= G.d_hat || cp.dist < 1e-12)) {
+        emit = true;
+        // frozen VT records carry d = -1 even for static candidates (contact-assembly.ts:181)
+        f.kind = q.kind; f.p = q.p; f.a = q.a; f.b = q.b; f.c = q.c; f.d = q.kind == 0 ? -1 : q.d;
+        f.nx = cp.rx / cp.dist; f.ny = cp.ry / cp.dist; f.nz = cp.rz / cp.dist;
+        if (q.kind == 0) {
+          f.w1 = cp.s; f.w2 = cp.t; f.w0 = 1 - cp.s - cp.t; f.w3 = 0;
+        } else {
+          f.w0 = 1 - cp.s; f.w1 = cp.s; f.w2 = -(1 - cp.t); f.w3 = -cp.t;
+        }
+      }
     }
-    const uint32_t k = atomicAdd(&cs->active_count, 1u);
-    if (k < cap) {
+    const uint32_t k = wave_alloc(emit, &cs->active_count);
+    if (emit && k < cap) {
       rec[k] = f;
-      keys[k] = q.key;
-      nkeys[k] = q.nkey;
+      keys[k] = prim_key(G, q);
+      nkeys[k] = prim_nkey(G, q);
     }
   }
-  double r;
-  if (grid_reduce_ext<kBlock, LtMin>(minD, partials, &cs->tickets[0], r)) cs->red[0] = r;
+  block_min_to(minD, &cs->mink[0]);
 }
 
 // Floor actives (contact-assembly.ts:202-213), appended after the candidates.
@@ -404,28 +610,36 @@ __global__ void __launch_bounds__(kBlock) k_floor_active(DGeom G, const double* 
                                                          uint64_t* keys, uint64_t* nkeys, uint32_t cap,
                                                          double* partials, CScalars* cs) {
   double minD = __builtin_inf();
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < G.n; v += gridDim.x * blockDim.x) {
-    const double d = x[(size_t)v * 3 + 1] - G.floor_y;
-    if (d < minD) minD = d;
-    if (d >= G.d_hat) continue;
-    Frozen f;
-    f.kind = 2; f.p = (int)v; f.a = f.b = f.c = f.d = -1;
-    f.w0 = f.w1 = f.w2 = f.w3 = 0;
-    f.nx = 0; f.ny = 1; f.nz = 0;
-    const uint32_t k = atomicAdd(&cs->active_count, 1u);
-    if (k < cap) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < G.n; base += gridDim.x * blockDim.x) {
+    const uint32_t v = base + threadIdx.x;
+    bool emit = false;
+    if (v < G.n) {
+      const double d = x[(size_t)v * 3 + 1] - G.floor_y;
+      if (d < minD) minD = d;
+      emit = !(d >= G.d_hat);
+    }
+    const uint32_t k = wave_alloc(emit, &cs->active_count);
+    if (emit && k < cap) {
+      Frozen f;
+      f.kind = 2; f.p = (int)v; f.a = f.b = f.c = f.d = -1;
+      f.w0 = f.w1 = f.w2 = f.w3 = 0;
+      f.nx = 0; f.ny = 1; f.nz = 0;
       rec[k] = f;
       keys[k] = rkey(G, 2, 0, v);
       nkeys[k] = rkey(G, 2, 0, v);
     }
   }
-  double r;
-  if (grid_reduce_ext<kBlock, LtMin>(minD, partials, &cs->tickets[1], r)) cs->red[1] = r;
+  block_min_to(minD, &cs->mink[1]);
 }
 
-__global__ void k_count_kinds(const Frozen* __restrict__ act, uint32_t C, CScalars* cs) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < C) atomicAdd(&cs->kind_count[act[i].kind & 3], 1u);
+__global__ void __launch_bounds__(kBlock) k_count_kinds(const Frozen* __restrict__ act, uint32_t C, CScalars* cs) {
+  __shared__ uint32_t s_cnt[4];
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < C; i += gridDim.x * blockDim.x)
+    atomicAdd(&s_cnt[act[i].kind & 3], 1u);
+  __syncthreads();
+  if (threadIdx.x < 4 && s_cnt[threadIdx.x]) atomicAdd(&cs->kind_count[threadIdx.x], s_cnt[threadIdx.x]);
 }
 
 __global__ void k_iota(uint32_t* v, uint32_t n) {
@@ -443,10 +657,8 @@ __global__ void k_gather_active(const uint32_t* __restrict__ perm, const Frozen*
 }
 
 // Vertex -> (contact << 3 | slot) incidence, one key per cloth stencil vertex.
-__global__ void k_emit_entries(const Frozen* __restrict__ act, uint32_t C, int cshift, uint64_t* keys,
-                               uint32_t* vals, CScalars* cs) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C) return;
+__device__ __forceinline__ void emit_entry(const Frozen* __restrict__ act, uint32_t c, int cshift, uint64_t* keys,
+                                           uint32_t* vals, uint32_t* s_valid) {
   const Frozen f = act[c];
   int vs[4];
   if (f.kind == 2) {
@@ -467,20 +679,38 @@ __global__ void k_emit_entries(const Frozen* __restrict__ act, uint32_t C, int c
     }
     vals[o] = 0;
   }
-  atomicAdd(&cs->entry_count, valid);
+  atomicAdd(s_valid, valid);
 }
 
-__global__ void k_entry_bounds(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t n,
-                               uint32_t* v_off, uint32_t* v_ent) {
+__global__ void __launch_bounds__(kBlock) k_emit_entries(const Frozen* __restrict__ act, uint32_t C, int cshift,
+                                                         uint64_t* keys, uint32_t* vals, CScalars* cs) {
+  __shared__ uint32_t s_valid;
+  if (threadIdx.x == 0) s_valid = 0;
+  __syncthreads();
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) emit_entry(act, c, cshift, keys, vals, &s_valid);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_valid) atomicAdd(&cs->entry_count, s_valid);
+}
+
+// v_ent[i] = low bits of the sorted incidence keys; v_off[u] = first entry
+// of vertex u (lower bound of u in the sorted vertex field), u in [0, n].
+__global__ void k_entry_ent(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t* v_ent) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cnt) return;
-  const uint64_t k = keys[i];
-  v_ent[i] = (uint32_t)(k & ((1ull << cshift) - 1));
-  const int64_t v = (int64_t)(k >> cshift);
-  const int64_t prev = i ? (int64_t)(keys[i - 1] >> cshift) : -1;
-  for (int64_t u = prev + 1; u <= v; u++) v_off[u] = i;
-  if (i == cnt - 1)
-    for (int64_t u = v + 1; u <= (int64_t)n; u++) v_off[u] = cnt;
+  if (i < cnt) v_ent[i] = (uint32_t)(keys[i] & ((1ull << cshift) - 1));
+}
+
+__global__ void k_entry_offsets(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t n,
+                                uint32_t* v_off) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > n) return;
+  uint32_t lo = 0, hi = cnt;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((keys[mid] >> cshift) < (uint64_t)u) lo = mid + 1;
+    else hi = mid;
+  }
+  v_off[u] = lo;
 }
 
 // ---------------------------------------------------------- per contact
@@ -595,121 +825,136 @@ __global__ void __launch_bounds__(kBlock) k_contact_eval(DGeom G, const double* 
     const double b2 = -(2 * Lg + (4 * uu) / d - (uu * uu) / (d * d));
     ev.coeff[i] = G.kappa * jsm::jmax(b2, 0);
   }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(energy, partials, &cs->tickets[2], t)) cs->red[2] = jsm::dd_value(t);
-  if (grid_reduce_dd<kBlock>(fwork, partials + 1024, &cs->tickets[3], t)) cs->red[3] = jsm::dd_value(t);
+  block_partial_dd<kBlock>(energy, partials);
+  block_partial_dd<kBlock>(fwork, partials + kMaxRedBlocks);
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Contact gradient per vertex: addDistGrad then addFrictionForce for each
 // active contact in list order (contact-assembly.ts:272-279, 288-341).
+// One wave per vertex: lanes evaluate 64 entries at a time and every lane
+// adds them in list order (the two additions of an entry in their order;
+// components an entry does not touch get +0, which leaves a sum that
+// started at +0 unchanged).
 __global__ void __launch_bounds__(kBlock) k_contact_grad(uint32_t n, const Frozen* __restrict__ act,
                                                          const uint32_t* __restrict__ v_off,
                                                          const uint32_t* __restrict__ v_ent, CEval ev,
                                                          double* __restrict__ ceg) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
   if (v >= n) return;
   double acc[3] = {0, 0, 0};
-  for (uint32_t k = v_off[v]; k < v_off[v + 1]; k++) {
-    const uint32_t ent = v_ent[k];
-    const uint32_t ci = ent >> 3;
-    const int slot = ent & 7;
-    if (!ev.inr[ci]) continue;
-    const Frozen c = act[ci];
-    const double g = ev.g[ci];
-    const double n3[3] = {c.nx, c.ny, c.nz};
-    const double fp[3] = {-ev.fr[(size_t)ci * 3], -ev.fr[(size_t)ci * 3 + 1], -ev.fr[(size_t)ci * 3 + 2]};
-    if (c.kind == 2) {
-      acc[1] += g;
-      acc[0] += fp[0]; acc[1] += fp[1]; acc[2] += fp[2];
-    } else if (c.kind == 0) {
-      if (slot == 0) {
-        for (int q = 0; q < 3; q++) acc[q] += g * n3[q];
-        for (int q = 0; q < 3; q++) acc[q] += fp[q];
-      } else {
-        const double w = slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2;
-        for (int q = 0; q < 3; q++) acc[q] += -g * w * n3[q];
-        for (int q = 0; q < 3; q++) acc[q] += -fp[q] * w;
+  const uint32_t b = v_off[v], e = v_off[v + 1];
+  for (uint32_t r0 = b; r0 < e; r0 += kWave) {
+    double ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
+    const uint32_t k = r0 + lane;
+    if (k < e) {
+      const uint32_t ent = v_ent[k];
+      const uint32_t ci = ent >> 3;
+      const int slot = ent & 7;
+      if (ev.inr[ci]) {
+        const Frozen c = act[ci];
+        const double g = ev.g[ci];
+        const double n3[3] = {c.nx, c.ny, c.nz};
+        const double fp[3] = {-ev.fr[(size_t)ci * 3], -ev.fr[(size_t)ci * 3 + 1], -ev.fr[(size_t)ci * 3 + 2]};
+        if (c.kind == 2) {
+          ta[1] = g;
+          for (int q = 0; q < 3; q++) tb[q] = fp[q];
+        } else if (c.kind == 0) {
+          if (slot == 0) {
+            for (int q = 0; q < 3; q++) ta[q] = g * n3[q];
+            for (int q = 0; q < 3; q++) tb[q] = fp[q];
+          } else {
+            const double w = slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2;
+            for (int q = 0; q < 3; q++) ta[q] = -g * w * n3[q];
+            for (int q = 0; q < 3; q++) tb[q] = -fp[q] * w;
+          }
+        } else {
+          const double w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
+          const double ww = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? -c.w2 : -c.w3;
+          for (int q = 0; q < 3; q++) ta[q] = g * w * n3[q];
+          for (int q = 0; q < 3; q++) tb[q] = fp[q] * ww;
+        }
       }
-    } else {
-      const double w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
-      const double ww = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? -c.w2 : -c.w3;
-      for (int q = 0; q < 3; q++) acc[q] += g * w * n3[q];
-      for (int q = 0; q < 3; q++) acc[q] += fp[q] * ww;
     }
+    const int cnt = (int)min((uint32_t)kWave, e - r0);
+    for (int j = 0; j < cnt; j++)
+      for (int q = 0; q < 3; q++) {
+        acc[q] += readlane_d(ta[q], j);
+        acc[q] += readlane_d(tb[q], j);
+      }
   }
-  ceg[(size_t)v * 3] = acc[0];
-  ceg[(size_t)v * 3 + 1] = acc[1];
-  ceg[(size_t)v * 3 + 2] = acc[2];
+  if (lane == 0) {
+    ceg[(size_t)v * 3] = acc[0];
+    ceg[(size_t)v * 3 + 1] = acc[1];
+    ceg[(size_t)v * 3 + 2] = acc[2];
+  }
 }
 
-// addDiagEstimate (contact-assembly.ts:436-471): accumulate into diag.
+// addDiagEstimate (contact-assembly.ts:436-471): accumulate into diag, one
+// wave per vertex as above.
 __global__ void __launch_bounds__(kBlock) k_contact_diag(uint32_t n, const Frozen* __restrict__ act,
                                                          const uint32_t* __restrict__ v_off,
                                                          const uint32_t* __restrict__ v_ent, CEval ev,
                                                          double* __restrict__ diag) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
   if (v >= n) return;
+  const uint32_t b = v_off[v], e = v_off[v + 1];
+  if (b == e) return;
   double dg[3] = {diag[(size_t)v * 3], diag[(size_t)v * 3 + 1], diag[(size_t)v * 3 + 2]};
-  for (uint32_t k = v_off[v]; k < v_off[v + 1]; k++) {
-    const uint32_t ent = v_ent[k];
-    const uint32_t ci = ent >> 3;
-    const int slot = ent & 7;
-    if (!ev.inr[ci]) continue;
-    const Frozen c = act[ci];
-    const double kk = ev.coeff[ci];
-    if (c.kind == 2) {
-      dg[1] += kk;
-      continue;
+  for (uint32_t r0 = b; r0 < e; r0 += kWave) {
+    double t[3] = {0, 0, 0};
+    int has = 0;
+    const uint32_t k = r0 + lane;
+    if (k < e) {
+      const uint32_t ent = v_ent[k];
+      const uint32_t ci = ent >> 3;
+      const int slot = ent & 7;
+      if (ev.inr[ci]) {
+        has = 1;
+        const Frozen c = act[ci];
+        const double kk = ev.coeff[ci];
+        if (c.kind == 2) {
+          has = 2;
+          t[1] = kk;
+        } else {
+          const double n2[3] = {c.nx * c.nx, c.ny * c.ny, c.nz * c.nz};
+          if (c.kind == 0 && slot == 0) {
+            for (int q = 0; q < 3; q++) t[q] = kk * n2[q];
+          } else {
+            const double w = c.kind == 0 ? (slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2)
+                                         : (slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3);
+            for (int q = 0; q < 3; q++) t[q] = kk * w * w * n2[q];
+          }
+        }
+      }
     }
-    const double n2[3] = {c.nx * c.nx, c.ny * c.ny, c.nz * c.nz};
-    if (c.kind == 0 && slot == 0) {
-      for (int q = 0; q < 3; q++) dg[q] += kk * n2[q];
-    } else {
-      const double w = c.kind == 0 ? (slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2)
-                                   : (slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3);
-      for (int q = 0; q < 3; q++) dg[q] += kk * w * w * n2[q];
+    const int cnt = (int)min((uint32_t)kWave, e - r0);
+    for (int j = 0; j < cnt; j++) {
+      const int h = __builtin_amdgcn_readlane(has, j);
+      if (h == 0) continue;
+      if (h == 2) {
+        dg[1] += readlane_d(t[1], j);
+      } else {
+        for (int q = 0; q < 3; q++) dg[q] += readlane_d(t[q], j);
+      }
     }
   }
-  diag[(size_t)v * 3] = dg[0];
-  diag[(size_t)v * 3 + 1] = dg[1];
-  diag[(size_t)v * 3 + 2] = dg[2];
+  if (lane == 0) {
+    diag[(size_t)v * 3] = dg[0];
+    diag[(size_t)v * 3 + 1] = dg[1];
+    diag[(size_t)v * 3 + 2] = dg[2];
+  }
 }
 
 // distJvp (contact-assembly.ts:394-414) times the frozen curvature.
-__global__ void __launch_bounds__(kBlock) k_contact_jv(const Frozen* __restrict__ act, uint32_t C, CEval ev,
-                                                       const double* __restrict__ p, double* __restrict__ dd,
-                                                       const int32_t* done) {
-  if (done && *done) return;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C) return;
-  if (!ev.inr[i]) { dd[i] = 0; return; }
-  const Frozen c = act[i];
-  double j;
-  if (c.kind == 2) {
-    j = p[(size_t)c.p * 3 + 1];
-  } else if (c.kind == 0) {
-    const size_t q = c.p;
-    j = c.nx * p[q * 3] + c.ny * p[q * 3 + 1] + c.nz * p[q * 3 + 2];
-    const double ws[3] = {c.w0, c.w1, c.w2};
-    const int vs[3] = {c.a, c.b, c.c};
-    for (int k = 0; k < 3; k++) {
-      if (vs[k] < 0) continue;
-      const size_t r = vs[k];
-      j -= ws[k] * (c.nx * p[r * 3] + c.ny * p[r * 3 + 1] + c.nz * p[r * 3 + 2]);
-    }
-  } else {
-    const double ws[4] = {c.w0, c.w1, c.w2, c.w3};
-    const int vs[4] = {c.a, c.b, c.c, c.d};
-    j = 0;
-    for (int k = 0; k < 4; k++) {
-      if (vs[k] < 0) continue;
-      const size_t r = vs[k];
-      j += ws[k] * (c.nx * p[r * 3] + c.ny * p[r * 3 + 1] + c.nz * p[r * 3 + 2]);
-    }
-  }
-  dd[i] = ev.coeff[i] * j;
-}
-
 // meritEnergy (contact-assembly.ts:419-428)
 __global__ void __launch_bounds__(kBlock) k_merit(DGeom G, const double* __restrict__ x,
                                                   const Frozen* __restrict__ act, uint32_t C, jsm::dd* partials,
@@ -720,8 +965,7 @@ __global__ void __launch_bounds__(kBlock) k_merit(DGeom G, const double* __restr
     if (d >= G.d_hat) continue;
     jsm::dd_add(e, G.kappa * barrier_value(d, G.d_hat));
   }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(e, partials, &cs->tickets[4], t)) cs->red[4] = jsm::dd_value(t);
+  block_partial_dd<kBlock>(e, partials);
 }
 
 // commit (contact-assembly.ts:475-485): one table entry per in-range contact
@@ -872,28 +1116,52 @@ __global__ void __launch_bounds__(kBlock) k_prims_trial(DGeom G, const double* _
   for (uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; tid < total;
        tid += (uint64_t)gridDim.x * blockDim.x) {
     Prim q;
-    prim_of(G, pairs, np, spairs, tid, q);
-    const double d = prim_closest(G, xt, q).dist;
-    if (d < minD) minD = d;
-    if (!jsm::isfinite_(d)) atomicOr(&cs->nonfinite, 1u);
-    Seg4 S;
+    prim_of(G, pairs, np, spairs, nsp, tid, q);
     const int ids[4] = {q.kind == 0 ? q.p : q.a, q.kind == 0 ? q.a : q.b, q.kind == 0 ? q.b : q.c,
                         q.kind == 0 ? q.c : q.d};
+    Seg4 S;
     for (int s = 0; s < 4; s++) {
       vpos(G, xs, ids[s], S.x0[s]);
       vpos(G, xt, ids[s], S.x1[s]);
     }
+    const double d = q.kind == 0 ? closest_vt(S.x1[0], S.x1[1], S.x1[2], S.x1[3]).dist
+                                 : closest_ee(S.x1[0], S.x1[1], S.x1[2], S.x1[3]).dist;
+    if (d < minD) minD = d;
+    if (!jsm::isfinite_(d)) atomicOr(&cs->nonfinite, 1u);
+    // Swept boxes of the prim's two parts (VT: the point / the triangle; EE:
+    // edge / edge) further apart than the thickness, with a 1e-6 relative
+    // margin, mean no configuration on the segment x0 -> x1 comes within the
+    // thickness: the reference's CCD then returns Infinity on every branch
+    // (not resting, no validated root, no sample within the thickness).
+    const int split = q.kind == 0 ? 1 : 2;
+    double gap2 = 0;
+    for (int k = 0; k < 3; k++) {
+      double lo1 = __builtin_inf(), hi1 = -__builtin_inf(), lo2 = __builtin_inf(), hi2 = -__builtin_inf();
+      for (int s = 0; s < 4; s++) {
+        const double a0 = S.x0[s][k], a1 = S.x1[s][k];
+        const double l = fmin(a0, a1), h = fmax(a0, a1);
+        if (s < split) { lo1 = fmin(lo1, l); hi1 = fmax(hi1, h); }
+        else { lo2 = fmin(lo2, l); hi2 = fmax(hi2, h); }
+      }
+      const double g = fmax(fmax(lo2 - hi1, lo1 - hi2), 0.0);
+      gap2 += g * g;
+    }
+    const double thr = G.d_min * (1 + 1e-6);
+    bool all_finite = true;
+    for (int s = 0; s < 4; s++)
+      for (int k = 0; k < 3; k++) all_finite = all_finite && jsm::isfinite_(S.x0[s][k]) && jsm::isfinite_(S.x1[s][k]);
+    if (all_finite && gap2 > thr * thr) continue;  // toi = Infinity
     const double toi = q.kind == 0 ? vt_ccd(S, G.d_min) : ee_ccd(S, G.d_min);
     if (jsm::isnan_(toi)) {
-      atomicMin((unsigned long long*)&cs->nan_key, (unsigned long long)q.key);
+      atomicMin((unsigned long long*)&cs->nan_key, (unsigned long long)prim_key(G, q));
     } else {
       if (toi < minT) minT = toi;
-      if (toi > 0 && toi < 1 - 1e-9) atomicMin((unsigned long long*)&cs->cross_key, (unsigned long long)q.key);
+      if (toi > 0 && toi < 1 - 1e-9)
+        atomicMin((unsigned long long*)&cs->cross_key, (unsigned long long)prim_key(G, q));
     }
   }
-  double r;
-  if (grid_reduce_ext<kBlock, LtMin>(minD, partials, &cs->tickets[5], r)) cs->red[5] = r;
-  if (grid_reduce_ext<kBlock, LtMin>(minT, partials + 1024, &cs->tickets[6], r)) cs->red[6] = r;
+  block_min_to(minD, &cs->mink[5]);
+  block_min_to(minT, &cs->mink[6]);
 }
 
 // checkTrial floor terms (contact-assembly.ts:503-509, 537-549).
@@ -913,9 +1181,8 @@ __global__ void __launch_bounds__(kBlock) k_floor_trial(DGeom G, const double* _
       if (toi < 1 - 1e-9) atomicOr(&cs->floor_fail, 1u);
     }
   }
-  double r;
-  if (grid_reduce_ext<kBlock, LtMin>(minD, partials, &cs->tickets[7], r)) cs->red[7] = r;
-  if (grid_reduce_ext<kBlock, LtMin>(minT, partials + 1024, &cs->tickets[8], r)) cs->red[8] = r;
+  block_min_to(minD, &cs->mink[7]);
+  block_min_to(minT, &cs->mink[8]);
 }
 
 // ====================================================================== host
@@ -935,12 +1202,15 @@ struct ContactImpl {
   int B;
   DevBuf<uint32_t> idx, sidx, tri_rank, dedge_rank, uedge;
   DevBuf<double> spos;
-  // LBVH
-  DevBuf<double> boxes, nodebox, bounds, dpart;
-  DevBuf<uint64_t> mkeys, mkeys_t;
-  DevBuf<uint32_t> mvals, mvals_t, left, right, first, last, parent_int, parent_leaf;
-  DevBuf<unsigned> flags;
-  const uint32_t* sorted_tri = nullptr;
+  // LBVH over the cloth triangles (Morton order kept until a re-sort) and a
+  // fixed one over the static triangles
+  Levels LV, SL;
+  DevBuf<Leaf> leaves, sleaves;
+  DevBuf<double> nodes, snodes, dpart;
+  DevBuf<uint64_t> mkeys, mkeys_t, bkeys;
+  DevBuf<uint32_t> mvals, mvals_t;
+  const uint32_t* order = nullptr;
+  bool need_sort = true;
   // candidates
   DevBuf<uint2> pairs, spairs;
   uint32_t pair_cap = 0, spair_cap = 0, np = 0, nsp = 0;
@@ -954,7 +1224,7 @@ struct ContactImpl {
   DevBuf<uint32_t> evals, evals_t, v_off, v_ent;
   // per-contact evaluation
   DevBuf<uint8_t> inr, kstate;
-  DevBuf<double> cg, fr, coeff, dd;
+  DevBuf<double> cg, fr, coeff, dd, hterm;
   DevBuf<uint32_t> heads;
   // lagged friction table
   DevBuf<uint64_t> lag_keys, lag_keys_t;
@@ -992,6 +1262,15 @@ struct ContactImpl {
   void sync_cs() {
     CS_HIP(hipMemcpyAsync(hcs, cs.ptr, sizeof(CScalars), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
+  }
+
+  double mink_value(int k) const {
+    const uint64_t key = hcs->mink[k];
+    if (key == ~0ull) return INFINITY;
+    const uint64_t u = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
   }
 
   void ensure_sort(size_t n_items) {
@@ -1065,15 +1344,16 @@ struct ContactImpl {
     B = bits_for(std::max<uint64_t>({(uint64_t)n, (uint64_t)T * 3}));
     if (2 * B + 2 > 64) throw std::invalid_argument("mesh too large for 64-bit contact keys");
     // LBVH buffers
-    boxes.alloc((size_t)m * 6);
-    nodebox.alloc((size_t)std::max<uint32_t>(m, 1) * 6);
-    bounds.alloc(6);
+    size_t nn = 0;
+    LV = make_levels(m, &nn);
+    leaves.alloc(std::max<uint32_t>(m, 1));
+    nodes.alloc(std::max<size_t>(nn, 1) * 6);
+    bkeys.alloc(6);
     dpart.alloc(9 * 1024);
     mkeys.alloc(m); mkeys_t.alloc(m); mvals.alloc(m); mvals_t.alloc(m);
-    for (auto* b : {&left, &right, &first, &last, &parent_int, &parent_leaf}) b->alloc(m);
-    flags.alloc(m);
+    build_static_tree(d);
     cs.alloc(1);
-    ddpart.alloc(2 * 1024);
+    ddpart.alloc(2 * kMaxRedBlocks);
     CS_HIP(hipHostMalloc(&hcs, sizeof(CScalars)));
     std::memset(hcs, 0, sizeof(CScalars));
     CS_HIP(hipMemsetAsync(cs.ptr, 0, sizeof(CScalars), st));
@@ -1104,12 +1384,94 @@ struct ContactImpl {
     evals.alloc((size_t)cap * 4); evals_t.alloc((size_t)cap * 4); v_ent.alloc((size_t)cap * 4);
     inr.alloc(cap); kstate.alloc(cap);
     cg.alloc(cap); fr.alloc((size_t)cap * 3); coeff.alloc(cap); dd.alloc(cap);
+    hterm.alloc((size_t)cap * 12);
     act_cap = cap;
     ensure_sort((size_t)cap * 4);
   }
 
   void clear_counters() {
     CS_HIP(hipMemsetAsync(cs.ptr, 0, sizeof(CScalars), st));
+    CS_HIP(hipMemsetAsync(cs.ptr->mink, 0xff, sizeof(hcs->mink), st));
+  }
+
+  // Fixed hierarchy over the static triangles: unpadded exact boxes
+  // (Math.min / Math.max of the three corners, staticTriAabb :597-605) in
+  // Morton order of their centroids, inner boxes as NaN-ignoring unions.
+  void build_static_tree(const cs_scene_desc& d) {
+    size_t nn = 0;
+    SL = make_levels(ms, &nn);
+    std::vector<Leaf> sl(ms);
+    std::vector<double> cen((size_t)ms * 3);
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t t = 0; t < ms; t++) {
+      Leaf& R = sl[t];
+      for (int k = 0; k < 3; k++) {
+        const double a = d.static_pos[(size_t)d.static_idx[(size_t)t * 3] * 3 + k];
+        const double b = d.static_pos[(size_t)d.static_idx[(size_t)t * 3 + 1] * 3 + k];
+        const double c = d.static_pos[(size_t)d.static_idx[(size_t)t * 3 + 2] * 3 + k];
+        R.lo[k] = jsm::jmin(jsm::jmin(a, b), c);
+        R.hi[k] = jsm::jmax(jsm::jmax(a, b), c);
+        cen[(size_t)t * 3 + k] = 0.5 * (R.lo[k] + R.hi[k]);
+        blo[k] = std::fmin(blo[k], cen[(size_t)t * 3 + k]);
+        bhi[k] = std::fmax(bhi[k], cen[(size_t)t * 3 + k]);
+      }
+      for (int c = 0; c < 3; c++) R.v[c] = -(int32_t)d.static_idx[(size_t)t * 3 + c] - 2;
+      R.tri = t;
+    }
+    auto expand = [](uint32_t v) {
+      uint32_t r = 0;
+      for (int b = 0; b < 10; b++) r |= ((v >> b) & 1u) << (3 * b);
+      return r;
+    };
+    std::vector<uint64_t> code(ms);
+    double ext = 0;
+    for (int k = 0; k < 3; k++) ext = std::fmax(ext, bhi[k] - blo[k]);
+    for (uint32_t t = 0; t < ms; t++) {
+      uint32_t q[3];
+      for (int k = 0; k < 3; k++) {
+        double u = ext > 0 ? (cen[(size_t)t * 3 + k] - blo[k]) / ext : 0.0;
+        if (!(u >= 0)) u = 0;
+        if (u > 1) u = 1;
+        q[k] = (uint32_t)std::fmin(1023.0, u * 1024.0);
+      }
+      code[t] = ((uint64_t)(expand(q[0]) << 2 | expand(q[1]) << 1 | expand(q[2])) << 32) | t;
+    }
+    std::sort(code.begin(), code.end());
+    std::vector<Leaf> sorted(ms);
+    for (uint32_t j = 0; j < ms; j++) sorted[j] = sl[(uint32_t)code[j]];
+    std::vector<double> nb(std::max<size_t>(nn, 1) * 6);
+    for (int l = 1; l <= SL.L; l++)
+      for (uint32_t k = 0; k < SL.cnt[l]; k++) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const uint32_t c0 = k * kFan, c1 = std::min(c0 + kFan, SL.cnt[l - 1]);
+        for (uint32_t c = c0; c < c1; c++) {
+          const double* b = l == 1 ? nullptr : &nb[((size_t)SL.off[l - 1] + c) * 6];
+          for (int q = 0; q < 3; q++) {
+            lo[q] = std::fmin(lo[q], l == 1 ? sorted[c].lo[q] : b[q]);
+            hi[q] = std::fmax(hi[q], l == 1 ? sorted[c].hi[q] : b[3 + q]);
+          }
+        }
+        for (int q = 0; q < 3; q++) {
+          nb[((size_t)SL.off[l] + k) * 6 + q] = lo[q];
+          nb[((size_t)SL.off[l] + k) * 6 + 3 + q] = hi[q];
+        }
+      }
+    if (ms) sleaves.upload(sorted.data(), ms, st);
+    else sleaves.alloc(1);
+    snodes.upload(nb.data(), nb.size(), st);
+  }
+
+  // Morton order of the cloth triangles at x (radix sort of 30-bit codes).
+  void sort_leaves(const double* x) {
+    const DGeom G = geom();
+    CS_HIP(hipMemsetAsync(bkeys.ptr, 0xff, 3 * sizeof(uint64_t), st));
+    CS_HIP(hipMemsetAsync(bkeys.ptr + 3, 0, 3 * sizeof(uint64_t), st));
+    hipLaunchKernelGGL(k_vert_bounds, dim3(red_grid(n)), dim3(kBlock), 0, st, n, x, bkeys.ptr);
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(m)), dim3(kBlock), 0, st, G, x, bkeys.ptr, mkeys.ptr, mvals.ptr);
+    CS_LAUNCH_CHECK();
+    const bool in_tmp = radix_sort_pairs(st, mkeys.ptr, mvals.ptr, mkeys_t.ptr, mvals_t.ptr, m, 0, 30, sort);
+    order = in_tmp ? mvals_t.ptr : mvals.ptr;
+    need_sort = false;
   }
 
   // buildCandidates (contact-assembly.ts:87-121) minus the expansion into
@@ -1118,31 +1480,26 @@ struct ContactImpl {
     const DGeom G = geom();
     clear_counters();
     CS_HIP(hipMemsetAsync(&cs.ptr->nan_key, 0xff, 2 * sizeof(uint64_t), st));
-    hipLaunchKernelGGL(k_tri_boxes, dim3(red_grid(m)), dim3(kBlock), 0, st, G, xstep, x, pad, boxes.ptr, dpart.ptr,
-                       cs.ptr->tickets, bounds.ptr);
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(m)), dim3(kBlock), 0, st, m, boxes.ptr, bounds.ptr, mkeys.ptr,
-                       mvals.ptr);
-    CS_LAUNCH_CHECK();
-    const bool in_tmp = radix_sort_pairs(st, mkeys.ptr, mvals.ptr, mkeys_t.ptr, mvals_t.ptr, m, 0, 30, sort);
-    const uint64_t* codes = in_tmp ? mkeys_t.ptr : mkeys.ptr;
-    sorted_tri = in_tmp ? mvals_t.ptr : mvals.ptr;
-    if (m > 1) {
-      CS_HIP(hipMemsetAsync(parent_int.ptr, 0xff, (size_t)m * sizeof(uint32_t), st));
-      CS_HIP(hipMemsetAsync(flags.ptr, 0, (size_t)m * sizeof(unsigned), st));
-      hipLaunchKernelGGL(k_karras, dim3(grid_for(m - 1)), dim3(kBlock), 0, st, (int)m, codes, left.ptr, right.ptr,
-                         first.ptr, last.ptr, parent_int.ptr, parent_leaf.ptr);
-      hipLaunchKernelGGL(k_refit, dim3(grid_for(m)), dim3(kBlock), 0, st, (int)m, sorted_tri, boxes.ptr, left.ptr,
-                         right.ptr, parent_int.ptr, parent_leaf.ptr, nodebox.ptr, flags.ptr);
-      CS_LAUNCH_CHECK();
+    if (m == 0) {
+      np = 0;
+      nsp = 0;
+      diag.candidate_pairs = 0;
+      return;
     }
+    if (need_sort) sort_leaves(x);
+    const unsigned waves8 = (unsigned)((m + kWave * kFan - 1) / (kWave * kFan));
+    hipLaunchKernelGGL(k_refit8, dim3((waves8 + 3) / 4), dim3(kBlock), 0, st, G, xstep, x, pad, order, leaves.ptr,
+                       nodes.ptr, LV);
+    if (LV.L > 3) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, nodes.ptr, LV);
+    CS_LAUNCH_CHECK();
     for (int attempt = 0; attempt < 3; attempt++) {
       CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 2 * sizeof(uint32_t), st));
       if (m > 1)
-        hipLaunchKernelGGL(k_traverse_self, dim3(grid_for(m)), dim3(kBlock), 0, st, G, sorted_tri, boxes.ptr,
-                           left.ptr, right.ptr, last.ptr, nodebox.ptr, pairs.ptr, pair_cap, cs.ptr);
+        hipLaunchKernelGGL(k_traverse_self, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, nodes.ptr, LV,
+                           pairs.ptr, pair_cap, cs.ptr);
       if (ms)
-        hipLaunchKernelGGL(k_traverse_static, dim3(grid_for(ms)), dim3(kBlock), 0, st, G, pad, sorted_tri,
-                           boxes.ptr, left.ptr, right.ptr, nodebox.ptr, spairs.ptr, spair_cap, cs.ptr);
+        hipLaunchKernelGGL(k_traverse_static, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, m, sleaves.ptr,
+                           snodes.ptr, SL, pad, spairs.ptr, spair_cap, cs.ptr);
       CS_LAUNCH_CHECK();
       sync_cs();
       if (hcs->nonfinite & 2u) throw OverflowError("LBVH traversal stack overflow");
@@ -1173,8 +1530,8 @@ struct ContactImpl {
       if (attempt == 2) throw OverflowError("active contact buffer could not be grown");
     }
     C = hcs->active_count;
-    double minD = total ? hcs->red[0] : INFINITY;
-    if (has_floor && hcs->red[1] < minD) minD = hcs->red[1];
+    double minD = total ? mink_value(0) : INFINITY;
+    if (has_floor && mink_value(1) < minD) minD = mink_value(1);
     // sort the frozen multiset into the reference's candidate order
     if (C) {
       hipLaunchKernelGGL(k_iota, dim3(grid_for(C)), dim3(kBlock), 0, st, avals.ptr, C);
@@ -1186,7 +1543,7 @@ struct ContactImpl {
       CS_LAUNCH_CHECK();
     }
     CS_HIP(hipMemsetAsync(cs.ptr->kind_count, 0, sizeof(hcs->kind_count), st));
-    if (C) hipLaunchKernelGGL(k_count_kinds, dim3(grid_for(C)), dim3(kBlock), 0, st, act.ptr, C, cs.ptr);
+    if (C) hipLaunchKernelGGL(k_count_kinds, dim3(red_grid(C)), dim3(kBlock), 0, st, act.ptr, C, cs.ptr);
     build_incidence();  // syncs the scalars
     // diagnostics (contact-assembly.ts:215-219)
     diag.active_pairs = C;
@@ -1197,8 +1554,10 @@ struct ContactImpl {
   }
 
   int cshift = 3;
+  uint32_t E = 0;  // vertex -> contact incidence entries
   void build_incidence() {
     CS_HIP(hipMemsetAsync(v_off.ptr, 0, (size_t)(n + 1) * sizeof(uint32_t), st));
+    E = 0;
     if (!C) {
       sync_cs();
       return;
@@ -1210,14 +1569,17 @@ struct ContactImpl {
     hipLaunchKernelGGL(k_emit_entries, dim3(grid_for(C)), dim3(kBlock), 0, st, act.ptr, C, cshift, ekeys.ptr,
                        evals.ptr, cs.ptr);
     CS_LAUNCH_CHECK();
-    const size_t E = (size_t)C * 4;
-    const bool in_tmp = radix_sort_pairs(st, ekeys.ptr, evals.ptr, ekeys_t.ptr, evals_t.ptr, E, 0, cshift + vb, sort);
+    const size_t nent = (size_t)C * 4;
+    const bool in_tmp = radix_sort_pairs(st, ekeys.ptr, evals.ptr, ekeys_t.ptr, evals_t.ptr, nent, 0, cshift + vb, sort);
     const uint64_t* sk = in_tmp ? ekeys_t.ptr : ekeys.ptr;
     sync_cs();
     const uint32_t cnt = hcs->entry_count;
-    if (cnt)
-      hipLaunchKernelGGL(k_entry_bounds, dim3(grid_for(cnt)), dim3(kBlock), 0, st, sk, cnt, cshift, n, v_off.ptr,
-                         v_ent.ptr);
+    E = cnt;
+    if (cnt) {
+      hipLaunchKernelGGL(k_entry_ent, dim3(grid_for(cnt)), dim3(kBlock), 0, st, sk, cnt, cshift, v_ent.ptr);
+      hipLaunchKernelGGL(k_entry_offsets, dim3(grid_for((size_t)n + 1)), dim3(kBlock), 0, st, sk, cnt, cshift, n,
+                         v_off.ptr);
+    }
     CS_LAUNCH_CHECK();
   }
 
@@ -1229,9 +1591,13 @@ struct ContactImpl {
       hipLaunchKernelGGL(k_contact_eval, dim3(red_grid(C)), dim3(kBlock), 0, st, G, x, xstep, act.ptr, anke.ptr, C,
                          lag_keys.ptr, lag_idx.ptr, lag_L, lag_vals.ptr, cev(), ddpart.ptr, cs.ptr);
       CS_LAUNCH_CHECK();
+      FinishJobs f{};
+      f.j[0] = FinishJob{ddpart.ptr, red_grid(C), FIN_DD, 0, &cs.ptr->red[2]};
+      f.j[1] = FinishJob{ddpart.ptr + kMaxRedBlocks, red_grid(C), FIN_DD, 0, &cs.ptr->red[3]};
+      launch_finish(st, f, 2);
     }
-    hipLaunchKernelGGL(k_contact_grad, dim3(grid_for(n)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr, v_ent.ptr, cev(),
-                       ceg);
+    hipLaunchKernelGGL(k_contact_grad, dim3(grid_for((size_t)n * kWave)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr,
+                       v_ent.ptr, cev(), ceg);
     CS_LAUNCH_CHECK();
     if (C) {
       sync_cs();
@@ -1244,8 +1610,8 @@ struct ContactImpl {
 
   void add_diag_estimate(double* dg) {
     if (!C) return;
-    hipLaunchKernelGGL(k_contact_diag, dim3(grid_for(n)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr, v_ent.ptr, cev(),
-                       dg);
+    hipLaunchKernelGGL(k_contact_diag, dim3(grid_for((size_t)n * kWave)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr,
+                       v_ent.ptr, cev(), dg);
     CS_LAUNCH_CHECK();
   }
 
@@ -1253,6 +1619,9 @@ struct ContactImpl {
     if (!C) return 0;
     hipLaunchKernelGGL(k_merit, dim3(red_grid(C)), dim3(kBlock), 0, st, geom(), x, act.ptr, C, ddpart.ptr, cs.ptr);
     CS_LAUNCH_CHECK();
+    FinishJobs f{};
+    f.j[0] = FinishJob{ddpart.ptr, red_grid(C), FIN_DD, 0, &cs.ptr->red[4]};
+    launch_finish(st, f, 1);
     sync_cs();
     return hcs->red[4];
   }
@@ -1324,13 +1693,13 @@ struct ContactImpl {
       hipLaunchKernelGGL(k_floor_trial, dim3(red_grid(n)), dim3(kBlock), 0, st, G, xstep, xt, dpart.ptr + 2048, cs.ptr);
     CS_LAUNCH_CHECK();
     sync_cs();
-    double minDist = total ? hcs->red[5] : INFINITY;
-    double minTOI = total ? hcs->red[6] : INFINITY;
+    double minDist = total ? mink_value(5) : INFINITY;
+    double minTOI = total ? mink_value(6) : INFINITY;
     if (hcs->nonfinite & 1u) {
       diag.ccd_failures++;
       return false;
     }
-    if (has_floor && hcs->red[7] < minDist) minDist = hcs->red[7];
+    if (has_floor && mink_value(7) < minDist) minDist = mink_value(7);
     if (minDist <= prm.d_min) return false;
     if (hcs->nan_key != SENT && hcs->nan_key < hcs->cross_key) {
       diag.ccd_failures++;
@@ -1338,7 +1707,7 @@ struct ContactImpl {
     }
     if (hcs->cross_key != SENT) return false;
     if (has_floor) {
-      if (hcs->red[8] < minTOI) minTOI = hcs->red[8];
+      if (mink_value(8) < minTOI) minTOI = mink_value(8);
       if (hcs->floor_fail) return false;
     }
     if (minDist < diag.min_distance) diag.min_distance = minDist;
@@ -1353,6 +1722,7 @@ ContactSystem::ContactSystem(const cs_scene_desc& d, hipStream_t st) : im_(new C
 ContactSystem::~ContactSystem() { delete im_; }
 void ContactSystem::begin_step(const double* x0) {
   im_->xstep = x0;
+  im_->need_sort = true;
   im_->lag_L = 0;
   im_->lag_vbase = 0;
   im_->C = 0;
@@ -1370,6 +1740,7 @@ cs_contact_diag ContactSystem::diagnostics() const { return im_->diag; }
 uint64_t ContactSystem::active_count() const { return im_->C; }
 
 uint64_t ContactSystem::candidate_count() const { return im_->diag.candidate_pairs; }
+uint64_t ContactSystem::incidence_count() const { return im_->E; }
 
 uint64_t ContactSystem::copy_active(int32_t* ids6, double* vals7, uint64_t cap) {
   const uint64_t C = im_->C;
@@ -1455,19 +1826,14 @@ void probe_closest(hipStream_t st, const double* x0, uint32_t n, const int32_t* 
 DContactHvp ContactSystem::hvp_hook() {
   DContactHvp h;
   h.C = im_->C;
+  h.E = im_->E;
+  h.term = im_->hterm.ptr;
   h.act = im_->act.ptr;
+  h.inr = im_->inr.ptr;
   h.coeff = im_->coeff.ptr;
-  h.dd = im_->dd.ptr;
   h.v_off = im_->v_off.ptr;
   h.v_ent = im_->v_ent.ptr;
   return h;
-}
-
-void ContactSystem::launch_hvp_coeffs(const double* p, const int32_t* done) {
-  if (!im_->C) return;
-  hipLaunchKernelGGL(k_contact_jv, dim3(grid_for(im_->C)), dim3(kBlock), 0, im_->st, im_->act.ptr, im_->C, im_->cev(),
-                     p, im_->dd.ptr, done);
-  CS_LAUNCH_CHECK();
 }
 
 }  // namespace cloth

@@ -55,8 +55,7 @@ struct Scalars;  // device scalar block (solver.hip)
 // g = M (x - yHat) / h^2 + grad_membrane + grad_bending (+ ceg), pins zeroed;
 // result[slot] = correctly rounded sum of g^2.
 void launch_gradient(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
-                     double inv_h2, const double* ceg, double* g, jsm::dd* partials, unsigned* ticket,
-                     double* out_sumsq);
+                     double inv_h2, const double* ceg, double* g, jsm::dd* partials, double* out_sumsq);
 
 // out = M p / h^2 + H_membrane p (+ contact), pins zeroed, for PCG iteration;
 // the FD operator reads h from *h_fd; partial sums of p.out are reduced into
@@ -66,10 +65,9 @@ enum HvpMode { HVP_FD = 0, HVP_ANALYTIC = 1 };
 // Energies at x: inertia sum and internal (membrane + bending) sum, each
 // correctly rounded; written to out2[0], out2[1].
 void launch_energy(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
-                   double h, jsm::dd* partials, unsigned* tickets, double* out2);
+                   double h, jsm::dd* partials, double* out2);
 
 // maxStrain over triangles (JS Math.max semantics).
-void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, unsigned* ticket,
-                       double* out);
+void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, double* out);
 
 }  // namespace cloth

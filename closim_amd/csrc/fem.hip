@@ -10,7 +10,7 @@ namespace cloth {
 __global__ void __launch_bounds__(kBlock) k_gradient(DMesh me, DMat mat, const double* __restrict__ x,
                                                      const double* __restrict__ yhat, double inv_h2,
                                                      const double* __restrict__ ceg, double* __restrict__ g,
-                                                     jsm::dd* partials, unsigned* ticket, double* out) {
+                                                     jsm::dd* partials) {
   jsm::dd acc2 = jsm::dd_zero();
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < me.n; v += gridDim.x * blockDim.x) {
     double acc[3];
@@ -27,24 +27,25 @@ __global__ void __launch_bounds__(kBlock) k_gradient(DMesh me, DMat mat, const d
       jsm::dd_add_prod(acc2, gi, gi);
     }
   }
-  jsm::dd tot;
-  if (grid_reduce_dd<kBlock>(acc2, partials, ticket, tot)) *out = jsm::dd_value(tot);
+  block_partial_dd<kBlock>(acc2, partials);
 }
 
 void launch_gradient(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
-                     double inv_h2, const double* ceg, double* g, jsm::dd* partials, unsigned* ticket,
-                     double* out_sumsq) {
-  hipLaunchKernelGGL(k_gradient, dim3(red_grid(me.n)), dim3(kBlock), 0, st, me, mat, x, yhat, inv_h2, ceg, g,
-                     partials, ticket, out_sumsq);
+                     double inv_h2, const double* ceg, double* g, jsm::dd* partials, double* out_sumsq) {
+  const unsigned grid = red_grid(me.n);
+  hipLaunchKernelGGL(k_gradient, dim3(grid), dim3(kBlock), 0, st, me, mat, x, yhat, inv_h2, ceg, g, partials);
   CS_LAUNCH_CHECK();
+  FinishJobs f{};
+  f.j[0] = FinishJob{partials, grid, FIN_DD, 0, out_sumsq};
+  launch_finish(st, f, 1);
 }
 
 // totalEnergy (newton.ts:76-88) minus the contact merit term: inertia sum
 // over DOFs and internalEnergyOnly (fem.ts:56-75) over triangles then hinges,
 // each returned correctly rounded.
 __global__ void __launch_bounds__(kBlock) k_energy(DMesh me, DMat mat, const double* __restrict__ x,
-                                                   const double* __restrict__ yhat, double h, jsm::dd* partials,
-                                                   unsigned* tickets, double* out2) {
+                                                   const double* __restrict__ yhat, double h,
+                                                   jsm::dd* partials) {
   jsm::dd ein = jsm::dd_zero(), eint = jsm::dd_zero();
   const size_t n3 = (size_t)me.n * 3;
   const size_t nh = mat.kb > 0 ? me.nh : 0;
@@ -69,22 +70,24 @@ __global__ void __launch_bounds__(kBlock) k_energy(DMesh me, DMat mat, const dou
       jsm::dd_add(eint, hinge_energy4(P, me.hrest[hh], me.hlen[hh], me.hasum[hh], mat.kb));
     }
   }
-  jsm::dd t0, t1;
-  if (grid_reduce_dd<kBlock>(ein, partials, &tickets[0], t0)) out2[0] = jsm::dd_value(t0);
-  if (grid_reduce_dd<kBlock>(eint, partials + gridDim.x, &tickets[1], t1)) out2[1] = jsm::dd_value(t1);
+  block_partial_dd<kBlock>(ein, partials);
+  block_partial_dd<kBlock>(eint, partials + kMaxRedBlocks);
 }
 
 void launch_energy(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
-                   double h, jsm::dd* partials, unsigned* tickets, double* out2) {
+                   double h, jsm::dd* partials, double* out2) {
   const size_t total = (size_t)me.n * 3 + me.m + me.nh;
-  hipLaunchKernelGGL(k_energy, dim3(red_grid(total)), dim3(kBlock), 0, st, me, mat, x, yhat, h, partials,
-                     tickets, out2);
+  const unsigned grid = red_grid(total);
+  hipLaunchKernelGGL(k_energy, dim3(grid), dim3(kBlock), 0, st, me, mat, x, yhat, h, partials);
   CS_LAUNCH_CHECK();
+  FinishJobs f{};
+  f.j[0] = FinishJob{partials, grid, FIN_DD, 0, out2};
+  f.j[1] = FinishJob{partials + kMaxRedBlocks, grid, FIN_DD, 0, out2 + 1};
+  launch_finish(st, f, 2);
 }
 
 // maxStrain (fem.ts:37): Math.max over |E00|, |E11|, |E01| of every triangle.
-__global__ void __launch_bounds__(kBlock) k_max_strain(DMesh me, const double* __restrict__ x, double* partials,
-                                                       unsigned* ticket, double* out) {
+__global__ void __launch_bounds__(kBlock) k_max_strain(DMesh me, const double* __restrict__ x, double* partials) {
   double mx = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < me.m; t += gridDim.x * blockDim.x) {
     double x0[3], x1[3], x2[3];
@@ -94,14 +97,16 @@ __global__ void __launch_bounds__(kBlock) k_max_strain(DMesh me, const double* _
     const TriState s = tri_state(x0, x1, x2, me.inv_dm + (size_t)t * 4);
     mx = jsm::jmax(jsm::jmax(jsm::jmax(mx, jsm::fabs_(s.E00)), jsm::fabs_(s.E11)), jsm::fabs_(s.E01));
   }
-  double r;
-  if (grid_reduce_ext<kBlock, JsMax>(mx, partials, ticket, r)) *out = jsm::jmax(0.0, r);
+  block_partial_ext<kBlock, JsMax>(mx, partials);
 }
 
-void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, unsigned* ticket,
-                       double* out) {
-  hipLaunchKernelGGL(k_max_strain, dim3(red_grid(me.m)), dim3(kBlock), 0, st, me, x, partials, ticket, out);
+void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, double* out) {
+  const unsigned grid = red_grid(me.m);
+  hipLaunchKernelGGL(k_max_strain, dim3(grid), dim3(kBlock), 0, st, me, x, partials);
   CS_LAUNCH_CHECK();
+  FinishJobs f{};
+  f.j[0] = FinishJob{partials, grid, FIN_JSMAX, 1, out};
+  launch_finish(st, f, 1);
 }
 
 }  // namespace cloth

@@ -33,6 +33,9 @@ class MixedSolve {
   template <typename F>
   int solve(const MixedInputs& in, F&& hvp_timer);
   int solve_plain(const MixedInputs& in, void (*timer)(void*, int, bool), void* ctx);
+  // Mean duration (ms) of `reps` back-to-back HVP launches on the operands of
+  // the last solve, bracketed by HIP events on the solver stream.
+  double time_hvp(int reps);
   void set_material(const cs_material&) {}
 
  private:

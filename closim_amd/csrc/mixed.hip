@@ -14,48 +14,16 @@ namespace cloth {
 
 struct Frozen32 {
   int32_t kind, p, a, b, c, d;
-  float w0, w1, w2, w3, nx, ny, nz, coeff;
+  float w0, w1, w2, w3, nx, ny, nz, coeff;  // coeff = 0 outside dHat
 };
 
 struct MxScalars {
-  float rz, pAp, alpha, rznew, residual, abs_tol;
+  float rz[2];  // r.z of iteration k in rz[k & 1]
+  float pAp, residual, abs_tol;
   int32_t done, iters;
-  unsigned tickets[8];
 };
 
-template <int BLOCK>
-__device__ __forceinline__ bool grid_reduce_f(float v, float* partials, unsigned* ticket, float& out) {
-  __shared__ float s[BLOCK / kWave];
-  __shared__ bool s_last;
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  if (lane == 0) s[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float b = 0;
-    for (int w = 0; w < BLOCK / kWave; w++) b += s[w];
-    partials[blockIdx.x] = b;
-    __threadfence();
-    s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  float acc = 0;
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK) acc += partials[i];
-  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
-  __syncthreads();
-  if (lane == 0) s[wid] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float t = 0;
-    for (int w = 0; w < BLOCK / kWave; w++) t += s[w];
-    *ticket = 0;
-    out = t;
-    return true;
-  }
-  return false;
-}
+enum { MX_RZ = 0, MX_BB = 1, MX_RR = 2, MX_PAP = 3, MX_NPART = 4 };
 
 struct MxArgs {
   DMesh32 me;
@@ -70,14 +38,48 @@ struct MxArgs {
   float* Ap;
   float tol;
   MxScalars* s;
-  float* partials;  // 4 * 1024
+  float* partials;  // MX_NPART * kMaxRedBlocks
   // contacts
-  uint32_t C;
+  uint32_t C, E;
   const Frozen32* act;
-  float* dd;
   const uint32_t* v_off;
   const uint32_t* v_ent;
+  float* term;  // [3E] per-entry terms of the current p
 };
+
+__device__ __forceinline__ float* mpart(const MxArgs& a, int which) {
+  return a.partials + (size_t)which * kMaxRedBlocks;
+}
+
+// FP32 block sum (fixed tree), every thread gets the total.
+__device__ __forceinline__ float block_sum_f(float v) {
+  __shared__ float s[kBlock / kWave];
+  __shared__ float s_res;
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0) s[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0;
+    for (int w = 0; w < kBlock / kWave; w++) t += s[w];
+    s_res = t;
+  }
+  __syncthreads();
+  const float r = s_res;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void block_partial_f(float v, float* partials) {
+  const float b = block_sum_f(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+__device__ __forceinline__ float merge_partials_f(const float* partials, unsigned count) {
+  float acc = 0;
+  for (unsigned i = threadIdx.x; i < count; i += kBlock) acc += partials[i];
+  return block_sum_f(acc);
+}
 
 __device__ __forceinline__ void ld3f(const float* x, uint32_t v, float* o) {
   o[0] = x[(size_t)v * 3];
@@ -123,6 +125,48 @@ __device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint32_t t, int co
   }
 }
 
+// Frozen barrier HVP term of one incidence entry: coeff (J.p) times the
+// entry's row of J^T (applyHvp, contact-assembly.ts:371-392).
+__device__ __forceinline__ void contact_term32(const MxArgs& a, uint32_t ent, const float* p, float* o) {
+  const Frozen32 c = a.act[ent >> 3];
+  const int slot = ent & 7;
+  o[0] = o[1] = o[2] = 0.f;
+  if (c.coeff == 0.f) return;
+  float j;
+  float q[3];
+  if (c.kind == 2) {
+    j = p[(size_t)c.p * 3 + 1];
+  } else if (c.kind == 0) {
+    ld3f(p, c.p, q);
+    j = c.nx * q[0] + c.ny * q[1] + c.nz * q[2];
+    const float ws[3] = {c.w0, c.w1, c.w2};
+    const int vs[3] = {c.a, c.b, c.c};
+    for (int k = 0; k < 3; k++) {
+      if (vs[k] < 0) continue;
+      ld3f(p, vs[k], q);
+      j -= ws[k] * (c.nx * q[0] + c.ny * q[1] + c.nz * q[2]);
+    }
+  } else {
+    const float ws[4] = {c.w0, c.w1, c.w2, c.w3};
+    const int vs[4] = {c.a, c.b, c.c, c.d};
+    j = 0;
+    for (int k = 0; k < 4; k++) {
+      if (vs[k] < 0) continue;
+      ld3f(p, vs[k], q);
+      j += ws[k] * (c.nx * q[0] + c.ny * q[1] + c.nz * q[2]);
+    }
+  }
+  const float g = c.coeff * j;
+  if (c.kind == 2) {
+    o[1] = g;
+  } else {
+    float w;
+    if (c.kind == 0) w = slot == 0 ? 1.f : -(slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2);
+    else w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
+    o[0] = g * w * c.nx; o[1] = g * w * c.ny; o[2] = g * w * c.nz;
+  }
+}
+
 __device__ __forceinline__ void precond32(const MxArgs& a, uint32_t v, const float* r3, float* z3) {
   if (a.binv) {
     const float* f = a.binv + (size_t)v * 9;
@@ -156,8 +200,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_convert(uint32_t n, const double*
     for (int q = 0; q < 9; q++) binv32[(size_t)v * 9 + q] = (float)binv[(size_t)v * 9 + q];
 }
 
-__global__ void k_mx_contacts(const Frozen* __restrict__ act, const double* __restrict__ coeff, uint32_t C,
-                              Frozen32* out) {
+__global__ void k_mx_contacts(const Frozen* __restrict__ act, const uint8_t* __restrict__ inr,
+                              const double* __restrict__ coeff, uint32_t C, Frozen32* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= C) return;
   const Frozen f = act[i];
@@ -165,7 +209,7 @@ __global__ void k_mx_contacts(const Frozen* __restrict__ act, const double* __re
   o.kind = f.kind; o.p = f.p; o.a = f.a; o.b = f.b; o.c = f.c; o.d = f.d;
   o.w0 = (float)f.w0; o.w1 = (float)f.w1; o.w2 = (float)f.w2; o.w3 = (float)f.w3;
   o.nx = (float)f.nx; o.ny = (float)f.ny; o.nz = (float)f.nz;
-  o.coeff = (float)coeff[i];
+  o.coeff = inr[i] ? (float)coeff[i] : 0.f;
   out[i] = o;
 }
 
@@ -186,112 +230,99 @@ __global__ void __launch_bounds__(kBlock) k_mx_init(MxArgs a) {
       bb += r3[i] * r3[i];
     }
   }
-  MxScalars* s = a.s;
-  float t;
-  if (grid_reduce_f<kBlock>(rz, a.partials, &s->tickets[0], t)) s->rz = t;
-  if (grid_reduce_f<kBlock>(bb, a.partials + 1024, &s->tickets[1], t)) {
-    s->residual = sqrtf(t);
+  block_partial_f(rz, mpart(a, MX_RZ));
+  block_partial_f(bb, mpart(a, MX_BB));
+}
+
+__global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) {
+  const float rz = merge_partials_f(mpart(a, MX_RZ), count);
+  const float bb = merge_partials_f(mpart(a, MX_BB), count);
+  if (threadIdx.x == 0) {
+    MxScalars* s = a.s;
+    s->rz[0] = rz;
+    s->residual = sqrtf(bb);
     s->abs_tol = fmaxf(a.tol * fmaxf(s->residual, 1e-12f), 1e-12f);
+    s->iters = 0;
+    s->done = s->residual < s->abs_tol ? 1 : 0;
   }
 }
 
-__global__ void k_mx_decide(MxScalars* s) {
-  s->iters = 0;
-  s->done = s->residual < s->abs_tol ? 1 : 0;
+// One thread per incidence entry: its contact term for the current p.
+__global__ void __launch_bounds__(kBlock) k_mx_terms(MxArgs a) {
+  if (block_latched(&a.s->done)) return;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  float t[3];
+  contact_term32(a, a.v_ent[e], a.p, t);
+  a.term[(size_t)e * 3] = t[0];
+  a.term[(size_t)e * 3 + 1] = t[1];
+  a.term[(size_t)e * 3 + 2] = t[2];
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_jv(MxArgs a) {
-  if (a.s->done) return;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.C) return;
-  const Frozen32 c = a.act[i];
-  float j;
-  if (c.kind == 2) {
-    j = a.p[(size_t)c.p * 3 + 1];
-  } else if (c.kind == 0) {
-    float q[3];
-    ld3f(a.p, c.p, q);
-    j = c.nx * q[0] + c.ny * q[1] + c.nz * q[2];
-    const float ws[3] = {c.w0, c.w1, c.w2};
-    const int vs[3] = {c.a, c.b, c.c};
-    for (int k = 0; k < 3; k++) {
-      if (vs[k] < 0) continue;
-      ld3f(a.p, vs[k], q);
-      j -= ws[k] * (c.nx * q[0] + c.ny * q[1] + c.nz * q[2]);
-    }
-  } else {
-    const float ws[4] = {c.w0, c.w1, c.w2, c.w3};
-    const int vs[4] = {c.a, c.b, c.c, c.d};
-    j = 0;
-    for (int k = 0; k < 4; k++) {
-      if (vs[k] < 0) continue;
-      float q[3];
-      ld3f(a.p, vs[k], q);
-      j += ws[k] * (c.nx * q[0] + c.ny * q[1] + c.nz * q[2]);
-    }
-  }
-  a.dd[i] = c.coeff * j;
-}
+constexpr int kMxLanes = 8;  // lanes per vertex in the HVP gather
+constexpr int kMxVertsPerBlock = kBlock / kMxLanes;
 
+// Ap = M p / h^2 + H_membrane p + contact, pins zeroed.  The 8 lanes of a
+// vertex group evaluate the incident corners (and contact entries) in
+// parallel; an xor-butterfly sums them (fixed order, deterministic).
 __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
-  MxScalars* s = a.s;
-  if (s->done) return;
+  if (block_latched(&a.s->done)) return;
   float pap = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
-    float acc[3] = {0, 0, 0};
-    const uint32_t e = a.me.vt_off[v + 1];
-    for (uint32_t q = a.me.vt_off[v]; q < e; q++) {
-      const uint32_t rec = a.me.vt_rec[q];
-      float gc[3];
-      hvp_corner32(a, rec >> 2, rec & 3, a.p, gc);
-      acc[0] += gc[0]; acc[1] += gc[1]; acc[2] += gc[2];
-    }
-    float p3[3];
-    ld3f(a.p, v, p3);
-    const float mh = a.me.mass[v] * a.inv_h2;
-    float o[3] = {mh * p3[0] + acc[0], mh * p3[1] + acc[1], mh * p3[2] + acc[2]};
-    if (a.act) {
-      for (uint32_t q = a.v_off[v]; q < a.v_off[v + 1]; q++) {
-        const uint32_t ent = a.v_ent[q];
-        const uint32_t ci = ent >> 3;
-        const int slot = ent & 7;
-        const float g = a.dd[ci];
-        if (g == 0.f) continue;
-        const Frozen32 c = a.act[ci];
-        if (c.kind == 2) {
-          o[1] += g;
-        } else {
-          float w;
-          if (c.kind == 0) w = slot == 0 ? 1.f : -(slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2);
-          else w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
-          o[0] += g * w * c.nx; o[1] += g * w * c.ny; o[2] += g * w * c.nz;
+  const int sub = threadIdx.x % kMxLanes;
+  const uint32_t stride = gridDim.x * kMxVertsPerBlock;
+  for (uint32_t v0 = blockIdx.x * kMxVertsPerBlock; v0 < a.me.n; v0 += stride) {
+    const uint32_t v = v0 + threadIdx.x / kMxLanes;
+    const bool valid = v < a.me.n;
+    float acc[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+      const uint32_t e = a.me.vt_off[v + 1];
+      for (uint32_t q = a.me.vt_off[v] + sub; q < e; q += kMxLanes) {
+        const uint32_t rec = a.me.vt_rec[q];
+        float gc[3];
+        hvp_corner32(a, rec >> 2, rec & 3, a.p, gc);
+        acc[0] += gc[0]; acc[1] += gc[1]; acc[2] += gc[2];
+      }
+      if (a.act) {
+        const uint32_t ce = a.v_off[v + 1];
+        for (uint32_t q = a.v_off[v] + sub; q < ce; q += kMxLanes) {
+          acc[0] += a.term[(size_t)q * 3];
+          acc[1] += a.term[(size_t)q * 3 + 1];
+          acc[2] += a.term[(size_t)q * 3 + 2];
         }
       }
     }
-    const bool pinned = a.me.pinmask[v] != 0;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const float ap = pinned ? 0.f : o[i];
-      a.Ap[(size_t)v * 3 + i] = ap;
-      pap += p3[i] * ap;
+    for (int o = 1; o < kMxLanes; o <<= 1)
+      for (int i = 0; i < 3; i++) acc[i] += __shfl_xor(acc[i], o, kWave);
+    if (valid && sub == 0) {
+      float p3[3];
+      ld3f(a.p, v, p3);
+      const float mh = a.me.mass[v] * a.inv_h2;
+      const bool pinned = a.me.pinmask[v] != 0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const float ap = pinned ? 0.f : mh * p3[i] + acc[i];
+        a.Ap[(size_t)v * 3 + i] = ap;
+        pap += p3[i] * ap;
+      }
     }
   }
-  float t;
-  if (grid_reduce_f<kBlock>(pap, a.partials, &s->tickets[2], t)) {
-    s->pAp = t;
-    if (!(t > 1e-30f)) {
-      s->done = 1;
-      s->iters = k;
-    } else {
-      s->alpha = s->rz / t;
-    }
-  }
+  block_partial_f(pap, mpart(a, MX_PAP));
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k) {
+__global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned pap_count) {
   MxScalars* s = a.s;
-  if (s->done) return;
-  const float alpha = s->alpha;
+  if (block_latched(&s->done)) return;
+  const float pAp = merge_partials_f(mpart(a, MX_PAP), pap_count);
+  if (!(pAp > 1e-30f)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      s->pAp = pAp;
+      s->done = 1;
+      s->iters = k;
+    }
+    return;
+  }
+  const float alpha = s->rz[k & 1] / pAp;
   float rr = 0, rz = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
@@ -315,22 +346,24 @@ __global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k) {
       rz += r3[i] * z3[i];
     }
   }
-  float t;
-  if (grid_reduce_f<kBlock>(rr, a.partials, &s->tickets[3], t)) {
-    s->residual = sqrtf(t);
-    if (s->residual < s->abs_tol) {
+  block_partial_f(rr, mpart(a, MX_RR));
+  block_partial_f(rz, mpart(a, MX_RZ));
+}
+
+__global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a, int k) {
+  MxScalars* s = a.s;
+  if (block_latched(&s->done)) return;
+  const float residual = sqrtf(merge_partials_f(mpart(a, MX_RR), gridDim.x));
+  if (residual < s->abs_tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      s->residual = residual;
       s->done = 2;
       s->iters = k + 1;
     }
+    return;
   }
-  if (grid_reduce_f<kBlock>(rz, a.partials + 1024, &s->tickets[4], t)) s->rznew = t;
-}
-
-__global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a) {
-  MxScalars* s = a.s;
-  if (s->done) return;
-  const float beta = s->rznew / s->rz;
-  float dummy = 0;
+  const float rznew = merge_partials_f(mpart(a, MX_RZ), gridDim.x);
+  const float beta = rznew / s->rz[k & 1];
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
 #pragma unroll
@@ -339,8 +372,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a) {
       a.p[j] = pinned ? 0.f : a.z[j] + beta * a.p[j];
     }
   }
-  float t;
-  if (grid_reduce_f<kBlock>(dummy, a.partials + 2048, &s->tickets[5], t)) s->rz = s->rznew;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->rz[(k + 1) & 1] = rznew;
+    s->residual = residual;
+  }
 }
 
 __global__ void k_mx_out(const float* sol, double* dx, size_t n3) {
@@ -351,17 +386,24 @@ __global__ void k_mx_out(const float* sol, double* dx, size_t n3) {
 struct MixedImpl {
   hipStream_t st;
   uint32_t n;
-  DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, dd, partials;
+  DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, term;
   DevBuf<Frozen32> act32;
   DevBuf<MxScalars> s;
+  MxArgs last{};
+  bool has_last = false;
 };
+
+static unsigned mx_grid(uint32_t n) { return red_grid(n); }
+static unsigned mx_hvp_grid(uint32_t n) {
+  return (unsigned)std::min<size_t>(kMaxRedBlocks, ((size_t)n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
+}
 
 MixedSolve::MixedSolve(const cs_scene_desc& d, const cs_options&, hipStream_t st) : im_(new MixedImpl) {
   im_->st = st;
   im_->n = d.n_verts;
   const size_t n3 = (size_t)d.n_verts * 3;
   for (auto* b : {&im_->x32, &im_->d32, &im_->sol, &im_->r, &im_->z, &im_->p, &im_->Ap}) b->alloc(n3);
-  im_->partials.alloc(4 * 1024);
+  im_->partials.alloc((size_t)MX_NPART * kMaxRedBlocks);
   im_->s.alloc(1);
   CS_HIP(hipMemsetAsync(im_->s.ptr, 0, sizeof(MxScalars), st));
 }
@@ -374,12 +416,12 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   const uint32_t n = in.me.n;
   if (in.binv) M.binv32.alloc((size_t)n * 9);
   const uint32_t C = in.contact.act ? in.contact.C : 0;
-  if (in.contact.act) {
-    M.act32.alloc(C ? C : 1);
-    M.dd.alloc(C ? C : 1);
-    if (C)
-      hipLaunchKernelGGL(k_mx_contacts, dim3(grid_for(C)), dim3(kBlock), 0, st, in.contact.act, in.contact.coeff, C,
-                         M.act32.ptr);
+  const uint32_t E = C ? in.contact.E : 0;
+  if (C) {
+    M.act32.alloc(C);
+    M.term.alloc((size_t)std::max<uint32_t>(E, 1) * 3);
+    hipLaunchKernelGGL(k_mx_contacts, dim3(grid_for(C)), dim3(kBlock), 0, st, in.contact.act, in.contact.inr,
+                       in.contact.coeff, C, M.act32.ptr);
   }
   hipLaunchKernelGGL(k_mx_convert, dim3(grid_for(n)), dim3(kBlock), 0, st, n, in.x, in.b, in.diag, in.binv,
                      in.me.pinmask, M.x32.ptr, M.r.ptr, M.d32.ptr, in.binv ? M.binv32.ptr : nullptr, M.sol.ptr);
@@ -396,22 +438,25 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.s = M.s.ptr;
   a.partials = M.partials.ptr;
   a.C = C;
-  a.act = in.contact.act ? M.act32.ptr : nullptr;
-  a.dd = in.contact.act ? M.dd.ptr : nullptr;
+  a.E = E;
+  a.term = C ? M.term.ptr : nullptr;
+  a.act = C ? M.act32.ptr : nullptr;
   a.v_off = in.contact.v_off;
   a.v_ent = in.contact.v_ent;
-  const dim3 g(red_grid(n)), b(kBlock);
+  M.last = a;
+  M.has_last = true;
+  const dim3 g(mx_grid(n)), gh(mx_hvp_grid(n)), b(kBlock);
   hipLaunchKernelGGL(k_mx_init, g, b, 0, st, a);
-  hipLaunchKernelGGL(k_mx_decide, dim3(1), dim3(1), 0, st, M.s.ptr);
+  hipLaunchKernelGGL(k_mx_decide, dim3(1), b, 0, st, a, mx_grid(n));
   CS_LAUNCH_CHECK();
   const int chunk = 8;
   for (int k = 0; k < in.max_iters; k++) {
-    if (C) hipLaunchKernelGGL(k_mx_jv, dim3(grid_for(C)), dim3(kBlock), 0, st, a);
+    if (E) hipLaunchKernelGGL(k_mx_terms, dim3(grid_for(E)), b, 0, st, a);
     if (timer) timer(ctx, k, true);
-    hipLaunchKernelGGL(k_mx_hvp, g, b, 0, st, a, k);
+    hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, k);
     if (timer) timer(ctx, k, false);
-    hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k);
-    hipLaunchKernelGGL(k_mx_dir, g, b, 0, st, a);
+    hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k, mx_hvp_grid(n));
+    hipLaunchKernelGGL(k_mx_dir, g, b, 0, st, a, k);
     CS_LAUNCH_CHECK();
     if ((k + 1) % chunk == 0 || k + 1 == in.max_iters) {
       MxScalars hs;
@@ -426,6 +471,33 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   CS_HIP(hipMemcpyAsync(&hs, M.s.ptr, sizeof(hs), hipMemcpyDeviceToHost, st));
   CS_HIP(hipStreamSynchronize(st));
   return hs.done ? hs.iters : in.max_iters;
+}
+
+// Back-to-back HVP launches on the operands of the last solve (inputs stay
+// resident; Ap and the p.Ap partials are overwritten).
+double MixedSolve::time_hvp(int reps) {
+  MixedImpl& M = *im_;
+  if (!M.has_last || reps < 1) return 0.0;
+  hipStream_t st = M.st;
+  CS_HIP(hipMemsetAsync(&M.s.ptr->done, 0, sizeof(int32_t), st));
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  const dim3 gh(mx_hvp_grid(M.n)), b(kBlock);
+  hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, M.last, 0);
+  CS_HIP(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, M.last, 0);
+  CS_HIP(hipEventRecord(e1, st));
+  CS_LAUNCH_CHECK();
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const int32_t one = 1;
+  CS_HIP(hipMemcpyAsync(&M.s.ptr->done, &one, sizeof(one), hipMemcpyHostToDevice, st));
+  CS_HIP(hipStreamSynchronize(st));
+  return (double)ms / reps;
 }
 
 }  // namespace cloth

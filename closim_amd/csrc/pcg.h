@@ -12,25 +12,29 @@ struct Frozen {
   double w0, w1, w2, w3, nx, ny, nz;
 };
 
-// Contact Hessian-vector hook: dd[c] = kappa max(b'', 0) (J_c . v) per active
-// contact, gathered per vertex through a CSR in active-list order.
+// Contact Hessian-vector hook: per active contact c the HVP adds
+// kappa max(b'', 0) (J_c . p) J_c^T, gathered per vertex through a CSR in
+// active-list order.
 struct DContactHvp {
   uint32_t C = 0;
+  uint32_t E = 0;                   // incidence entries
   const Frozen* act = nullptr;
+  const uint8_t* inr = nullptr;     // 1 when the contact is inside dHat
   const double* coeff = nullptr;    // kappa * max(b''(d), 0), 0 when d >= dHat
-  double* dd = nullptr;             // per contact, per PCG iteration
   const uint32_t* v_off = nullptr;  // n+1
   const uint32_t* v_ent = nullptr;  // contact << 3 | slot
+  double* term = nullptr;           // [3E] per-entry HVP terms of the current p
 };
 
-// Device scalars of one PCG solve.
+// Device scalars of one PCG solve (written by block 0 of the kernel that
+// takes the decision).
 struct PcgScalars {
-  double rz, pAp, alpha, beta, residual, abs_tol, h_fd, bnorm;
+  double rz[2];  // r.z of iteration k in rz[k & 1]
+  double pAp, residual, abs_tol, bnorm;
   int32_t done, iters;
-  unsigned tickets[8];
 };
 
-enum PrecondKind { PRE_JACOBI = 0, PRE_BLOCK = 1, PRE_EXTERNAL = 2 };
+enum PrecondKind { PRE_JACOBI = 0, PRE_BLOCK = 1 };
 
 struct PcgArgs {
   DMesh me;
@@ -49,20 +53,18 @@ struct PcgArgs {
   int precond;         // PrecondKind
   double tol;
   PcgScalars* s;
-  jsm::dd* partials;   // >= 4 * 1024
+  jsm::dd* partials;   // >= 5 * kMaxRedBlocks
   DContactHvp contact;
 };
 
 void pcg_init(hipStream_t st, const PcgArgs& a);
-// One PCG iteration k (0-based): HVP + alpha, update + preconditioner +
-// residual test + beta, direction update (+ |p| for the FD operator).
+// One PCG iteration k (0-based): HVP, then update + preconditioner +
+// residual test + direction update.
 void pcg_iteration(hipStream_t st, const PcgArgs& a, int k);
 void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k);   // k_pcg_hvp only
 void pcg_iteration_rest(hipStream_t st, const PcgArgs& a, int k);  // update + direction
-// External preconditioner hook (Schwarz): z = M^-1 r must be produced between
-// pcg_update_x and pcg_finish_z.
-void pcg_iteration_split_a(hipStream_t st, const PcgArgs& a, int k);
-void pcg_iteration_split_b(hipStream_t st, const PcgArgs& a, int k);
+// Mean duration (ms) of `reps` back-to-back k_pcg_hvp launches (HIP events).
+double pcg_time_hvp(hipStream_t st, const PcgArgs& a, int reps);
 
 // Block-Jacobi factors (block-jacobi.ts:30-87) from the membrane vertex
 // blocks at x and the Jacobi diagonal.
@@ -70,9 +72,8 @@ void launch_block_factors(hipStream_t st, const DMesh& me, const DMat& mat, cons
                           double* binv, uint8_t* flag);
 
 // Vector helpers used by the Newton driver.
-void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, unsigned* ticket,
-                double* out);
-void launch_max_abs(hipStream_t st, const double* a, size_t n, double* partials, unsigned* ticket, double* out);
+void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, double* out);
+void launch_max_abs(hipStream_t st, const double* a, size_t n, double* partials, double* out);
 void launch_fallback_dir(hipStream_t st, const double* g, const double* diag, double* dx, size_t n);
 void launch_scale(hipStream_t st, double* v, double s, size_t n);
 void launch_trial(hipStream_t st, const double* x, const double* dx, double alpha, double* xt, size_t n);

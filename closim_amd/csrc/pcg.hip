@@ -1,52 +1,142 @@
 // pcg.hip — device-resident PCG of the Newton step (pcg.ts:11-95) in FP64.
 //
-// Per iteration three kernels, each ending in a grid reduction whose last
-// block takes the scalar decision on the device (no host round trip):
-//   k_hvp     Ap = hvpFull(p) (newton.ts:135-146), pins zeroed; pAp; alpha
-//   k_update  x += alpha p, r -= alpha Ap; residual test; z = M^-1 r; beta
-//   k_dir     p = z + beta p; |p| for the FD operator's step h
-// A `done` latch turns the remaining launches of a converged solve into no-ops,
-// which is exactly the reference's early return.
+// Per iteration three kernels; each writes block partials of its dot
+// products and the next kernel merges them (fixed order, every block the
+// same bits), so no scalar ever makes a host round trip or needs a fence:
+//   k_hvp     [merge |p|^2 -> FD step h] Ap = hvpFull(p) (newton.ts:135-146),
+//             pins zeroed; partials of p.Ap
+//   k_update  [merge p.Ap -> breakdown test, alpha] x += alpha p,
+//             r -= alpha Ap, z = M^-1 r; partials of r.r and r.z
+//   k_dir     [merge r.r -> residual test; r.z -> beta] p = z + beta p;
+//             partials of p.p
+// A `done` latch turns the remaining launches of a converged solve into
+// no-ops, which is exactly the reference's early return.  Block 0 records
+// decisions; every block takes them from the same merged values.
 #include "elements.h"
 #include "pcg.h"
 
 namespace cloth {
 
-// hvpFull at vertex v: M p / h^2 + H_membrane p (+ contact), unfiltered.
-__device__ __forceinline__ void hvp_vertex(const PcgArgs& a, uint32_t v, const double* __restrict__ p, double h,
-                                           double* out) {
+enum { PART_BB = 0, PART_RR = 1, PART_RZ = 2, PART_PP = 3, PART_PAP = 4 };
+
+__device__ __forceinline__ jsm::dd* part(const PcgArgs& a, int which) {
+  return a.partials + (size_t)which * kMaxRedBlocks;
+}
+
+// dd_c = kappa max(b'', 0) (J_c . p) of active contact c (distJvp,
+// contact-assembly.ts:395-414, scaled as in applyHvp :376-386).
+__device__ __forceinline__ double contact_dd(const DContactHvp& H, uint32_t ci, const double* __restrict__ p) {
+  if (!H.inr[ci]) return 0.0;
+  const Frozen& c = H.act[ci];
+  double j;
+  if (c.kind == 2) {
+    j = p[(size_t)c.p * 3 + 1];
+  } else if (c.kind == 0) {
+    const size_t q = c.p;
+    j = c.nx * p[q * 3] + c.ny * p[q * 3 + 1] + c.nz * p[q * 3 + 2];
+    const double ws[3] = {c.w0, c.w1, c.w2};
+    const int vs[3] = {c.a, c.b, c.c};
+    for (int k = 0; k < 3; k++) {
+      if (vs[k] < 0) continue;
+      const size_t r = vs[k];
+      j -= ws[k] * (c.nx * p[r * 3] + c.ny * p[r * 3 + 1] + c.nz * p[r * 3 + 2]);
+    }
+  } else {
+    const double ws[4] = {c.w0, c.w1, c.w2, c.w3};
+    const int vs[4] = {c.a, c.b, c.c, c.d};
+    j = 0;
+    for (int k = 0; k < 4; k++) {
+      if (vs[k] < 0) continue;
+      const size_t r = vs[k];
+      j += ws[k] * (c.nx * p[r * 3] + c.ny * p[r * 3 + 1] + c.nz * p[r * 3 + 2]);
+    }
+  }
+  return H.coeff[ci] * j;
+}
+
+// Contribution of one incident triangle corner to the membrane HVP at the
+// vertex: analytic, or the two gradient corners of the FD operator.
+__device__ __forceinline__ void hvp_corner(const PcgArgs& a, uint32_t rec, const double* __restrict__ p, double h,
+                                           double* g0, double* g1) {
   const DMesh& me = a.me;
+  const uint32_t t = rec >> 2;
+  const int corner = rec & 3;
+  const uint32_t i0 = me.idx[(size_t)t * 3], i1 = me.idx[(size_t)t * 3 + 1], i2 = me.idx[(size_t)t * 3 + 2];
+  double x0[3], x1[3], x2[3], p0[3], p1[3], p2[3];
+  ld3(a.x, i0, x0); ld3(a.x, i1, x1); ld3(a.x, i2, x2);
+  ld3(p, i0, p0); ld3(p, i1, p1); ld3(p, i2, p2);
+  const float* inv = me.inv_dm + (size_t)t * 4;
+  const double area = me.area[t];
+  if (a.hvp_mode == HVP_FD) {
+    // membraneHvp (fem.ts:108-130): gradients at x + h p and x - h p
+    double xp0[3], xp1[3], xp2[3], xm0[3], xm1[3], xm2[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      xp0[i] = x0[i] + h * p0[i]; xm0[i] = x0[i] - h * p0[i];
+      xp1[i] = x1[i] + h * p1[i]; xm1[i] = x1[i] - h * p1[i];
+      xp2[i] = x2[i] + h * p2[i]; xm2[i] = x2[i] - h * p2[i];
+    }
+    const TriState sp = tri_state(xp0, xp1, xp2, inv);
+    tri_grad_corner(sp, area, a.mat, corner, g0);
+    const TriState sm = tri_state(xm0, xm1, xm2, inv);
+    tri_grad_corner(sm, area, a.mat, corner, g1);
+  } else {
+    const TriState s = tri_state(x0, x1, x2, inv);
+    tri_hvp_corner(s, p0, p1, p2, area, a.mat, corner, g0);
+    g1[0] = g1[1] = g1[2] = 0;
+  }
+}
+
+// One contact incidence entry's term at the vertex (applyHvp's addDistGrad).
+__device__ __forceinline__ bool contact_term(const DContactHvp& H, uint32_t ent, const double* __restrict__ p,
+                                             double* c) {
+  const uint32_t ci = ent >> 3;
+  const int slot = ent & 7;
+  const double g = contact_dd(H, ci, p);
+  if (g == 0) return false;
+  const Frozen& f = H.act[ci];
+  if (f.kind == 2) {
+    c[0] = 0; c[1] = g; c[2] = 0;
+  } else if (f.kind == 0) {
+    if (slot == 0) {
+      c[0] = g * f.nx; c[1] = g * f.ny; c[2] = g * f.nz;
+    } else {
+      const double w = slot == 1 ? f.w0 : slot == 2 ? f.w1 : f.w2;
+      c[0] = -g * w * f.nx; c[1] = -g * w * f.ny; c[2] = -g * w * f.nz;
+    }
+  } else {
+    const double w = slot == 0 ? f.w0 : slot == 1 ? f.w1 : slot == 2 ? f.w2 : f.w3;
+    c[0] = g * w * f.nx; c[1] = g * w * f.ny; c[2] = g * w * f.nz;
+  }
+  return true;
+}
+
+constexpr int kLanesPerVertex = 8;
+constexpr int kVertsPerBlock = kBlock / kLanesPerVertex;
+
+// hvpFull at vertex v: M p / h^2 + H_membrane p (+ contact), unfiltered.
+// The 8 lanes of a vertex group evaluate up to 8 incident corners (and
+// contact entries) at once; lane 0 of the group adds them in list order, so
+// the sums are the reference's element-order sums bit for bit.
+__device__ __forceinline__ void hvp_vertex8(const PcgArgs& a, uint32_t v, bool valid, const double* __restrict__ p,
+                                            double h, double* out) {
+  const DMesh& me = a.me;
+  const int sub = threadIdx.x % kLanesPerVertex;
+  const int base = (threadIdx.x % kWave) - sub;
   double acc0[3] = {0, 0, 0}, acc1[3] = {0, 0, 0};
-  const uint32_t b = me.vt_off[v], e = me.vt_off[v + 1];
-  for (uint32_t k = b; k < e; k++) {
-    const uint32_t rec = me.vt_rec[k];
-    const uint32_t t = rec >> 2;
-    const int corner = rec & 3;
-    const uint32_t i0 = me.idx[(size_t)t * 3], i1 = me.idx[(size_t)t * 3 + 1], i2 = me.idx[(size_t)t * 3 + 2];
-    double x0[3], x1[3], x2[3], p0[3], p1[3], p2[3], gc[3];
-    ld3(a.x, i0, x0); ld3(a.x, i1, x1); ld3(a.x, i2, x2);
-    ld3(p, i0, p0); ld3(p, i1, p1); ld3(p, i2, p2);
-    const float* inv = me.inv_dm + (size_t)t * 4;
-    const double area = me.area[t];
-    if (a.hvp_mode == HVP_FD) {
-      // membraneHvp (fem.ts:108-130): gradients at x + h p and x - h p
-      double xp0[3], xp1[3], xp2[3], xm0[3], xm1[3], xm2[3];
+  const uint32_t b = valid ? me.vt_off[v] : 0, e = valid ? me.vt_off[v + 1] : 0;
+  // rounds of 8 corners; the round count is uniform within the group
+  for (uint32_t r0 = b; r0 < e; r0 += kLanesPerVertex) {
+    double g0[3] = {0, 0, 0}, g1[3] = {0, 0, 0};
+    const uint32_t q = r0 + sub;
+    if (q < e) hvp_corner(a, me.vt_rec[q], p, h, g0, g1);
+    const uint32_t cnt = min((uint32_t)kLanesPerVertex, e - r0);
+    for (uint32_t j = 0; j < cnt; j++) {
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        xp0[i] = x0[i] + h * p0[i]; xm0[i] = x0[i] - h * p0[i];
-        xp1[i] = x1[i] + h * p1[i]; xm1[i] = x1[i] - h * p1[i];
-        xp2[i] = x2[i] + h * p2[i]; xm2[i] = x2[i] - h * p2[i];
+        acc0[i] += __shfl(g0[i], base + (int)j, kWave);
+        acc1[i] += __shfl(g1[i], base + (int)j, kWave);
       }
-      const TriState sp = tri_state(xp0, xp1, xp2, inv);
-      tri_grad_corner(sp, area, a.mat, corner, gc);
-      acc0[0] += gc[0]; acc0[1] += gc[1]; acc0[2] += gc[2];
-      const TriState sm = tri_state(xm0, xm1, xm2, inv);
-      tri_grad_corner(sm, area, a.mat, corner, gc);
-      acc1[0] += gc[0]; acc1[1] += gc[1]; acc1[2] += gc[2];
-    } else {
-      const TriState s = tri_state(x0, x1, x2, inv);
-      tri_hvp_corner(s, p0, p1, p2, area, a.mat, corner, gc);
-      acc0[0] += gc[0]; acc0[1] += gc[1]; acc0[2] += gc[2];
     }
   }
   double Hp[3];
@@ -56,37 +146,46 @@ __device__ __forceinline__ void hvp_vertex(const PcgArgs& a, uint32_t v, const d
   } else {
     Hp[0] = acc0[0]; Hp[1] = acc0[1]; Hp[2] = acc0[2];
   }
-  const double m = me.mass[v];
+  const double m = valid ? (double)me.mass[v] : 0.0;
 #pragma unroll
-  for (int i = 0; i < 3; i++) out[i] = m * p[(size_t)v * 3 + i] * a.inv_h2 + Hp[i];
+  for (int i = 0; i < 3; i++) out[i] = m * (valid ? p[(size_t)v * 3 + i] : 0.0) * a.inv_h2 + Hp[i];
   if (a.contact.act) {
     // applyHvp (contact-assembly.ts:371-392): out.fill(0) then addDistGrad per
-    // active contact in list order, skipping dd == 0; then out += cHp.
+    // active contact in list order (terms of entries with dd == 0 are +0 and
+    // leave the sum unchanged); then out += cHp.
     double c[3] = {0, 0, 0};
-    const uint32_t cb = a.contact.v_off[v], ce = a.contact.v_off[v + 1];
-    for (uint32_t k = cb; k < ce; k++) {
-      const uint32_t ent = a.contact.v_ent[k];
-      const uint32_t ci = ent >> 3;
-      const int slot = ent & 7;
-      const double g = a.contact.dd[ci];
-      if (g == 0) continue;
-      const Frozen& f = a.contact.act[ci];
-      if (f.kind == 2) {
-        c[1] += g;
-      } else if (f.kind == 0) {
-        if (slot == 0) {
-          c[0] += g * f.nx; c[1] += g * f.ny; c[2] += g * f.nz;
-        } else {
-          const double w = slot == 1 ? f.w0 : slot == 2 ? f.w1 : f.w2;
-          c[0] += -g * w * f.nx; c[1] += -g * w * f.ny; c[2] += -g * w * f.nz;
-        }
-      } else {
-        const double w = slot == 0 ? f.w0 : slot == 1 ? f.w1 : slot == 2 ? f.w2 : f.w3;
-        c[0] += g * w * f.nx; c[1] += g * w * f.ny; c[2] += g * w * f.nz;
+    const uint32_t cb = valid ? a.contact.v_off[v] : 0, ce = valid ? a.contact.v_off[v + 1] : 0;
+    for (uint32_t r0 = cb; r0 < ce; r0 += kLanesPerVertex) {
+      double t[3] = {0, 0, 0};
+      const uint32_t q = r0 + sub;
+      if (q < ce) {
+        t[0] = a.contact.term[(size_t)q * 3];
+        t[1] = a.contact.term[(size_t)q * 3 + 1];
+        t[2] = a.contact.term[(size_t)q * 3 + 2];
+      }
+      const uint32_t cnt = min((uint32_t)kLanesPerVertex, ce - r0);
+      for (uint32_t j = 0; j < cnt; j++) {
+        c[0] += __shfl(t[0], base + (int)j, kWave);
+        c[1] += __shfl(t[1], base + (int)j, kWave);
+        c[2] += __shfl(t[2], base + (int)j, kWave);
       }
     }
     out[0] += c[0]; out[1] += c[1]; out[2] += c[2];
   }
+}
+
+// Per incidence entry: the frozen barrier HVP term of its contact at its
+// stencil slot for the current p (one thread per entry).
+__global__ void __launch_bounds__(kBlock) k_contact_terms(DContactHvp H, const double* __restrict__ p,
+                                                          const int32_t* done) {
+  if (block_latched(done)) return;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= H.E) return;
+  double t[3] = {0, 0, 0};
+  if (!contact_term(H, H.v_ent[e], p, t)) t[0] = t[1] = t[2] = 0;
+  H.term[(size_t)e * 3] = t[0];
+  H.term[(size_t)e * 3 + 1] = t[1];
+  H.term[(size_t)e * 3 + 2] = t[2];
 }
 
 // z = M^-1 r at vertex v (Jacobi with the diag > 1e-12 guard, or 3x3 block).
@@ -119,76 +218,84 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(PcgArgs a) {
       a.sol[j] = 0.0;
       a.r[j] = r3[i];
     }
-    if (a.precond != PRE_EXTERNAL) {
-      precond_vertex(a, v, r3, z3);
+    precond_vertex(a, v, r3, z3);
 #pragma unroll
-      for (int i = 0; i < 3; i++) {
-        const size_t j = (size_t)v * 3 + i;
-        if (pinned) z3[i] = 0.0;
-        a.z[j] = z3[i];
-        a.p[j] = z3[i];
-        jsm::dd_add_prod(rz, r3[i], z3[i]);
-        jsm::dd_add_prod(pp, z3[i], z3[i]);
-      }
+    for (int i = 0; i < 3; i++) {
+      const size_t j = (size_t)v * 3 + i;
+      if (pinned) z3[i] = 0.0;
+      a.z[j] = z3[i];
+      a.p[j] = z3[i];
+      jsm::dd_add_prod(rz, r3[i], z3[i]);
+      jsm::dd_add_prod(pp, z3[i], z3[i]);
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) jsm::dd_add_prod(rr, r3[i], r3[i]);
   }
-  PcgScalars* s = a.s;
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(bb, a.partials, &s->tickets[0], t)) {
-    s->bnorm = jsm::sqrt_(jsm::dd_value(t));
+  block_partial_dd<kBlock>(bb, part(a, PART_BB));
+  block_partial_dd<kBlock>(rr, part(a, PART_RR));
+  block_partial_dd<kBlock>(rz, part(a, PART_RZ));
+  block_partial_dd<kBlock>(pp, part(a, PART_PP));
+}
+
+// After init, one block: tolerance, initial residual and rz, early return.
+__global__ void __launch_bounds__(kBlock) k_pcg_init_decide(PcgArgs a, unsigned count) {
+  const double bb = jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_BB), count));
+  const double rr = jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_RR), count));
+  const double rz = jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_RZ), count));
+  if (threadIdx.x == 0) {
+    PcgScalars* s = a.s;
+    s->bnorm = jsm::sqrt_(bb);
     s->abs_tol = jsm::jmax(a.tol * jsm::jmax(s->bnorm, 1e-12), 1e-12);
-  }
-  if (grid_reduce_dd<kBlock>(rr, a.partials + 1024, &s->tickets[1], t)) s->residual = jsm::sqrt_(jsm::dd_value(t));
-  if (a.precond != PRE_EXTERNAL) {
-    if (grid_reduce_dd<kBlock>(rz, a.partials + 2048, &s->tickets[2], t)) s->rz = jsm::dd_value(t);
-    if (grid_reduce_dd<kBlock>(pp, a.partials + 3072, &s->tickets[3], t))
-      s->h_fd = 1e-8 * (1 + jsm::sqrt_(jsm::dd_value(t)));
+    s->residual = jsm::sqrt_(rr);
+    s->rz[0] = rz;
+    s->iters = 0;
+    s->done = (s->residual < s->abs_tol) ? 1 : 0;
   }
 }
 
-// Decide after init (separate tiny kernel: the init reductions finish in
-// different last blocks).
-__global__ void k_pcg_init_decide(PcgScalars* s) {
-  s->iters = 0;
-  s->done = (s->residual < s->abs_tol) ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kBlock) k_pcg_hvp(PcgArgs a, int k) {
+__global__ void __launch_bounds__(kBlock) k_pcg_hvp(PcgArgs a, int k, unsigned pp_count) {
   PcgScalars* s = a.s;
-  if (s->done) return;
-  const double h = s->h_fd;
+  if (block_latched(&s->done)) return;
+  double h = 0;
+  if (a.hvp_mode == HVP_FD)
+    h = 1e-8 * (1 + jsm::sqrt_(jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_PP), pp_count))));
   jsm::dd pap = jsm::dd_zero();
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+  const int sub = threadIdx.x % kLanesPerVertex;
+  const uint32_t stride = gridDim.x * kVertsPerBlock;
+  // the loop count is uniform per vertex group (all 8 lanes share v)
+  for (uint32_t v0 = blockIdx.x * kVertsPerBlock; v0 < a.me.n; v0 += stride) {
+    const uint32_t v = v0 + threadIdx.x / kLanesPerVertex;
+    const bool valid = v < a.me.n;
     double o[3];
-    hvp_vertex(a, v, a.p, h, o);
-    const bool pinned = a.me.pinmask[v] != 0;
+    hvp_vertex8(a, v, valid, a.p, h, o);
+    if (valid && sub == 0) {
+      const bool pinned = a.me.pinmask[v] != 0;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const size_t j = (size_t)v * 3 + i;
-      const double ap = pinned ? 0.0 : o[i];
-      a.Ap[j] = ap;
-      jsm::dd_add_prod(pap, a.p[j], ap);
+      for (int i = 0; i < 3; i++) {
+        const size_t j = (size_t)v * 3 + i;
+        const double ap = pinned ? 0.0 : o[i];
+        a.Ap[j] = ap;
+        jsm::dd_add_prod(pap, a.p[j], ap);
+      }
     }
   }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(pap, a.partials, &s->tickets[4], t)) {
-    const double pAp = jsm::dd_value(t);
-    s->pAp = pAp;
-    if (!(pAp > 1e-30)) {
+  block_partial_dd<kBlock>(pap, part(a, PART_PAP));
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k, unsigned count) {
+  PcgScalars* s = a.s;
+  if (block_latched(&s->done)) return;
+  const double pAp = jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_PAP), count));
+  if (!(pAp > 1e-30)) {
+    // non-positive curvature: stop with the current x (pcg.ts:65-68)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      s->pAp = pAp;
       s->done = 1;
       s->iters = k;
-    } else {
-      s->alpha = s->rz / pAp;
     }
+    return;
   }
-}
-
-__global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k) {
-  PcgScalars* s = a.s;
-  if (s->done) return;
-  const double alpha = s->alpha;
+  const double alpha = s->rz[k & 1] / pAp;
   jsm::dd rr = jsm::dd_zero(), rz = jsm::dd_zero();
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
@@ -204,54 +311,32 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k) {
       r3[i] = ri;
       jsm::dd_add_prod(rr, ri, ri);
     }
-    if (a.precond != PRE_EXTERNAL) {
-      precond_vertex(a, v, r3, z3);
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        if (pinned) z3[i] = 0.0;
-        a.z[(size_t)v * 3 + i] = z3[i];
-        jsm::dd_add_prod(rz, r3[i], z3[i]);
-      }
-    }
-  }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(rr, a.partials, &s->tickets[5], t)) {
-    s->residual = jsm::sqrt_(jsm::dd_value(t));
-    if (s->residual < s->abs_tol) {
-      s->done = 2;  // converged in this iteration (any non-zero value stops the solve)
-      s->iters = k + 1;
-    }
-  }
-  if (a.precond != PRE_EXTERNAL) {
-    if (grid_reduce_dd<kBlock>(rz, a.partials + 1024, &s->tickets[6], t)) s->beta = jsm::dd_value(t);  // rzNew
-  }
-}
-
-// External preconditioner path: z was written by another kernel; reduce rz.
-__global__ void __launch_bounds__(kBlock) k_pcg_rz(PcgArgs a) {
-  PcgScalars* s = a.s;
-  if (s->done) return;
-  jsm::dd rz = jsm::dd_zero();
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
-    const bool pinned = a.me.pinmask[v] != 0;
+    precond_vertex(a, v, r3, z3);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      const size_t j = (size_t)v * 3 + i;
-      const double zi = pinned ? 0.0 : a.z[j];
-      a.z[j] = zi;
-      jsm::dd_add_prod(rz, a.r[j], zi);
+      if (pinned) z3[i] = 0.0;
+      a.z[(size_t)v * 3 + i] = z3[i];
+      jsm::dd_add_prod(rz, r3[i], z3[i]);
     }
   }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(rz, a.partials + 1024, &s->tickets[6], t)) s->beta = jsm::dd_value(t);
+  block_partial_dd<kBlock>(rr, part(a, PART_RR));
+  block_partial_dd<kBlock>(rz, part(a, PART_RZ));
 }
 
-__global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a) {
+__global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a, int k) {
   PcgScalars* s = a.s;
-  if (s->done) return;
-  // beta holds rzNew here: beta = rzNew / rz; rz = rzNew (pcg.ts:88-92)
-  const double rzNew = s->beta;
-  const double beta = rzNew / s->rz;
+  if (block_latched(&s->done)) return;
+  const double residual = jsm::sqrt_(jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_RR), gridDim.x)));
+  if (residual < s->abs_tol) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      s->residual = residual;
+      s->done = 2;  // converged in this iteration
+      s->iters = k + 1;
+    }
+    return;
+  }
+  const double rzNew = jsm::dd_value(merge_partials_dd<kBlock>(part(a, PART_RZ), gridDim.x));
+  const double beta = rzNew / s->rz[k & 1];
   jsm::dd pp = jsm::dd_zero();
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
@@ -263,18 +348,47 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a) {
       jsm::dd_add_prod(pp, pi, pi);
     }
   }
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(pp, a.partials + 2048, &s->tickets[7], t)) {
-    s->h_fd = 1e-8 * (1 + jsm::sqrt_(jsm::dd_value(t)));
-    s->rz = rzNew;
+  block_partial_dd<kBlock>(pp, part(a, PART_PP));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->rz[(k + 1) & 1] = rzNew;
+    s->residual = residual;
   }
 }
 
+unsigned pcg_grid(const PcgArgs& a) { return red_grid(a.me.n); }
+unsigned pcg_hvp_grid(const PcgArgs& a) {
+  return (unsigned)std::min<size_t>(kMaxRedBlocks, ((size_t)a.me.n + kVertsPerBlock - 1) / kVertsPerBlock);
+}
+
 void pcg_init(hipStream_t st, const PcgArgs& a) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(red_grid(a.me.n)), dim3(kBlock), 0, st, a);
+  hipLaunchKernelGGL(k_pcg_init, dim3(pcg_grid(a)), dim3(kBlock), 0, st, a);
   CS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_pcg_init_decide, dim3(1), dim3(1), 0, st, a.s);
+  hipLaunchKernelGGL(k_pcg_init_decide, dim3(1), dim3(kBlock), 0, st, a, pcg_grid(a));
   CS_LAUNCH_CHECK();
+}
+
+// Back-to-back HVP launches on the operands of the last solve (Ap and the
+// p.Ap partials are overwritten; the latch is cleared for the run and set after).
+double pcg_time_hvp(hipStream_t st, const PcgArgs& a, int reps) {
+  if (reps < 1) return 0.0;
+  CS_HIP(hipMemsetAsync(&a.s->done, 0, sizeof(int32_t), st));
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  const dim3 g(pcg_hvp_grid(a)), b(kBlock);
+  hipLaunchKernelGGL(k_pcg_hvp, g, b, 0, st, a, 0, pcg_grid(a));
+  CS_HIP(hipEventRecord(e0, st));
+  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_pcg_hvp, g, b, 0, st, a, 0, pcg_grid(a));
+  CS_HIP(hipEventRecord(e1, st));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const int32_t one = 1;
+  CS_HIP(hipMemcpyAsync(&a.s->done, &one, sizeof(one), hipMemcpyHostToDevice, st));
+  CS_HIP(hipStreamSynchronize(st));
+  return (double)ms / reps;
 }
 
 void pcg_iteration(hipStream_t st, const PcgArgs& a, int k) {
@@ -282,29 +396,19 @@ void pcg_iteration(hipStream_t st, const PcgArgs& a, int k) {
   pcg_iteration_rest(st, a, k);
 }
 
+// The HVP grid differs from the vector grid; its partial count is passed on.
 void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k) {
-  hipLaunchKernelGGL(k_pcg_hvp, dim3(red_grid(a.me.n)), dim3(kBlock), 0, st, a, k);
+  if (a.contact.act && a.contact.E)
+    hipLaunchKernelGGL(k_contact_terms, dim3(grid_for(a.contact.E)), dim3(kBlock), 0, st, a.contact, a.p,
+                       &a.s->done);
+  hipLaunchKernelGGL(k_pcg_hvp, dim3(pcg_hvp_grid(a)), dim3(kBlock), 0, st, a, k, pcg_grid(a));
   CS_LAUNCH_CHECK();
 }
 
 void pcg_iteration_rest(hipStream_t st, const PcgArgs& a, int k) {
-  const dim3 g(red_grid(a.me.n)), b(kBlock);
-  hipLaunchKernelGGL(k_pcg_update, g, b, 0, st, a, k);
-  hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a);
-  CS_LAUNCH_CHECK();
-}
-
-void pcg_iteration_split_a(hipStream_t st, const PcgArgs& a, int k) {
-  const dim3 g(red_grid(a.me.n)), b(kBlock);
-  hipLaunchKernelGGL(k_pcg_hvp, g, b, 0, st, a, k);
-  hipLaunchKernelGGL(k_pcg_update, g, b, 0, st, a, k);
-  CS_LAUNCH_CHECK();
-}
-
-void pcg_iteration_split_b(hipStream_t st, const PcgArgs& a, int k) {
-  const dim3 g(red_grid(a.me.n)), b(kBlock);
-  hipLaunchKernelGGL(k_pcg_rz, g, b, 0, st, a);
-  hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a);
+  const dim3 g(pcg_grid(a)), b(kBlock);
+  hipLaunchKernelGGL(k_pcg_update, g, b, 0, st, a, k, pcg_hvp_grid(a));
+  hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a, k);
   CS_LAUNCH_CHECK();
 }
 
@@ -438,34 +542,38 @@ void launch_block_factors(hipStream_t st, const DMesh& me, const DMat& mat, cons
 // ---- vector helpers ---------------------------------------------------------
 
 __global__ void __launch_bounds__(kBlock) k_dot(const double* __restrict__ a, const double* __restrict__ b,
-                                                size_t n, jsm::dd* partials, unsigned* ticket, double* out) {
+                                                size_t n, jsm::dd* partials) {
   jsm::dd acc = jsm::dd_zero();
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     jsm::dd_add_prod(acc, a[i], b[i]);
-  jsm::dd t;
-  if (grid_reduce_dd<kBlock>(acc, partials, ticket, t)) *out = jsm::dd_value(t);
+  block_partial_dd<kBlock>(acc, partials);
 }
 
-void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, unsigned* ticket,
-                double* out) {
-  hipLaunchKernelGGL(k_dot, dim3(red_grid(n)), dim3(kBlock), 0, st, a, b, n, partials, ticket, out);
+void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, double* out) {
+  const unsigned grid = red_grid(n);
+  hipLaunchKernelGGL(k_dot, dim3(grid), dim3(kBlock), 0, st, a, b, n, partials);
   CS_LAUNCH_CHECK();
+  FinishJobs f{};
+  f.j[0] = FinishJob{partials, grid, FIN_DD, 0, out};
+  launch_finish(st, f, 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_max_abs(const double* __restrict__ a, size_t n, double* partials,
-                                                    unsigned* ticket, double* out) {
+__global__ void __launch_bounds__(kBlock) k_max_abs(const double* __restrict__ a, size_t n, double* partials) {
   double m = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const double v = jsm::fabs_(a[i]);
     if (v > m) m = v;
   }
-  double r;
-  if (grid_reduce_ext<kBlock, GtMax>(m, partials, ticket, r)) *out = r > 0 ? r : 0.0;
+  block_partial_ext<kBlock, GtMax>(m, partials);
 }
 
-void launch_max_abs(hipStream_t st, const double* a, size_t n, double* partials, unsigned* ticket, double* out) {
-  hipLaunchKernelGGL(k_max_abs, dim3(red_grid(n)), dim3(kBlock), 0, st, a, n, partials, ticket, out);
+void launch_max_abs(hipStream_t st, const double* a, size_t n, double* partials, double* out) {
+  const unsigned grid = red_grid(n);
+  hipLaunchKernelGGL(k_max_abs, dim3(grid), dim3(kBlock), 0, st, a, n, partials);
   CS_LAUNCH_CHECK();
+  FinishJobs f{};
+  f.j[0] = FinishJob{partials, grid, FIN_GTMAX, 1, out};
+  launch_finish(st, f, 1);
 }
 
 __global__ void k_fallback_dir(const double* __restrict__ g, const double* __restrict__ diag, double* dx, size_t n) {

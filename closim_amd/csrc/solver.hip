@@ -29,7 +29,6 @@ struct NewtonScalars {
   double gtdx;
   double maxdx;
   double max_strain;
-  unsigned tickets[16];
 };
 
 struct NonFinite {};
@@ -229,8 +228,8 @@ struct Solver {
     pos.upload(d->positions, n3, st);
     vel.upload(d->velocities, n3, st);
     for (auto* bf : {&x0, &yhat, &x, &xt, &g, &b, &dx, &diag, &ceg, &r, &z, &p, &Ap}) bf->alloc(n3);
-    partials.alloc(4 * 1024);
-    mpartials.alloc(4 * 1024);
+    partials.alloc(8 * kMaxRedBlocks);
+    mpartials.alloc(4 * kMaxRedBlocks);
     scal.alloc(1);
     scal.zero(st);
     CS_HIP(hipHostMalloc(&hscal, sizeof(NewtonScalars)));
@@ -269,7 +268,7 @@ struct Solver {
 
   // totalEnergy (newton.ts:76-88).
   double total_energy(const double* xx, double h) {
-    launch_energy(st, dmesh(), dmat(), xx, yhat.ptr, h, partials.ptr, &ds()->tickets[0], ds()->e2);
+    launch_energy(st, dmesh(), dmat(), xx, yhat.ptr, h, partials.ptr, ds()->e2);
     double merit = 0;
     if (contact) merit = contact->merit_energy(xx);
     sync_scalars();
@@ -328,10 +327,9 @@ struct Solver {
         cg = ceg.ptr;
       }
       const int e0 = ev_mark();
-      launch_gradient(st, dmesh(), dmat(), x.ptr, yhat.ptr, invH2, cg, g.ptr, partials.ptr, &ds()->tickets[0],
-                      &ds()->grad_sumsq);
+      launch_gradient(st, dmesh(), dmat(), x.ptr, yhat.ptr, invH2, cg, g.ptr, partials.ptr, &ds()->grad_sumsq);
       ev_span(CS_KERNEL_GRADIENT, e0, ev_mark());
-      launch_max_strain(st, dmesh(), x.ptr, mpartials.ptr, &ds()->tickets[1], &ds()->max_strain);
+      launch_max_strain(st, dmesh(), x.ptr, mpartials.ptr, &ds()->max_strain);
       sync_scalars();
       maxStrain = hscal->max_strain;
       gradNorm = std::sqrt(hscal->grad_sumsq);
@@ -349,15 +347,15 @@ struct Solver {
       }
       pcgItersTotal += iters;
       // descent safeguard (newton.ts:154-160)
-      launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->tickets[2], &ds()->gtdx);
+      launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
       double gtdx = read(&ds()->gtdx);
       if (!std::isfinite(gtdx) || gtdx >= 0) {
         launch_fallback_dir(st, g.ptr, diag.ptr, dx.ptr, n3);
-        launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->tickets[2], &ds()->gtdx);
+        launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
         gtdx = read(&ds()->gtdx);
       }
       // trust region (newton.ts:167-179)
-      launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->tickets[3], &ds()->maxdx);
+      launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->maxdx);
       const double maxDx = read(&ds()->maxdx);
       const double TRUST = 0.002;
       if (maxDx > TRUST) {
@@ -472,8 +470,35 @@ struct Solver {
     });
   }
 
+  PcgArgs last_pcg{};
+  bool has_last_pcg = false;
+
+  // Algorithmic HBM bytes of one HVP launch: every input element read once
+  // and Ap written once (per vertex: x, p, Ap, mass, pin flag, CSR offset;
+  // per triangle: corners, Dm^-1, area and three CSR records; with contacts,
+  // the incidence offsets and the per-entry contact terms).
+  double hvp_bytes() const {
+    const bool mixed_p = opt.precision == CS_PREC_MIXED;
+    const double fs = mixed_p ? 4.0 : 8.0;
+    double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (12 + 16 + 4 + 12);
+    if (contact && contact->active_count()) {
+      // CSR offsets plus one precomputed 3-vector term per incidence entry
+      const double E = (double)contact->incidence_count();
+      b += 4.0 * n + 3 * fs * E;
+    }
+    return b;
+  }
+
+  double time_hvp(int reps) {
+    if (opt.precision == CS_PREC_MIXED) return mixed->time_hvp(reps);
+    if (!has_last_pcg) return 0.0;
+    return pcg_time_hvp(st, last_pcg, reps);
+  }
+
   int pcg_solve(double invH2, int maxIters) {
     PcgArgs a = pcg_args(invH2);
+    last_pcg = a;
+    has_last_pcg = true;
     if (a.precond == PRE_BLOCK) {
       const int e0 = ev_mark();
       launch_block_factors(st, a.me, a.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
@@ -482,7 +507,6 @@ struct Solver {
     pcg_init(st, a);
     const int chunk = 8;
     for (int k = 0; k < maxIters; k++) {
-      if (contact) contact->launch_hvp_coeffs(p.ptr, &ds()->pcg.done);
       const int e0 = ev_mark();
       pcg_iteration_hvp(st, a, k);
       ev_span(CS_KERNEL_HVP, e0, ev_mark());
@@ -720,15 +744,22 @@ int cs_kernel_timing(const cs_solver* s, int kernel, double* total_ms, uint64_t*
   if (kernel < 0 || kernel >= CS_KERNEL_COUNT) return fail(CS_E_INVALID, "unknown kernel family");
   if (total_ms) *total_ms = s->s.kt_ms[kernel];
   if (count) *count = s->s.kt_count[kernel];
-  if (bytes) {
-    const double n = s->s.n, m = s->s.m;
-    double b = 0;
-    const bool mixed = s->s.opt.precision == CS_PREC_MIXED;
-    const double C = s->s.contact ? (double)s->s.contact->active_count() : 0.0;
-    if (kernel == CS_KERNEL_HVP) b = mixed ? 40 * n + 32 * m + 64 * C : 80 * n + 52 * m + 96 * C;
-    *bytes = b;
-  }
+  if (bytes) *bytes = kernel == CS_KERNEL_HVP ? s->s.hvp_bytes() : 0.0;
   return CS_OK;
+}
+
+int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (reps < 1 || reps > 100000) return fail(CS_E_INVALID, "reps must be in [1, 100000]");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const double ms = S.time_hvp(reps);
+    if (ms <= 0) return fail(CS_E_INVALID, "no HVP operands yet: run cs_step first");
+    if (mean_ms) *mean_ms = ms;
+    if (bytes) *bytes = S.hvp_bytes();
+    return CS_OK;
+  });
 }
 
 int cs_reset_timing(cs_solver* s) {

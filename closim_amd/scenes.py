@@ -174,7 +174,7 @@ def head_on_patches():
     return scene
 
 
-def static_plate_patch():
+def static_plate_patch(mu=0.3):
     """static-collider.test.ts:35-69: patch above a 2-triangle static plate."""
     from .scene import DEFAULT_MATERIAL, DEFAULT_CONTACT_PARAMS
     from dataclasses import replace
@@ -183,7 +183,7 @@ def static_plate_patch():
     scene = create_scene(mesh, DEFAULT_MATERIAL, (0.0, -9.81, 0.0))
     sp = np.array([-0.05, 0.03, -0.05, 0.15, 0.03, -0.05, 0.15, 0.03, 0.15, -0.05, 0.03, 0.15], np.float32)
     si = np.array([0, 1, 2, 0, 2, 3], np.uint32)
-    c = ContactSystem(replace(DEFAULT_CONTACT_PARAMS, frictionMu=0.3))
+    c = ContactSystem(replace(DEFAULT_CONTACT_PARAMS, frictionMu=mu))
     c.setStaticMesh(sp, si)
     scene.contact = c
     return scene

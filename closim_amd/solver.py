@@ -55,6 +55,7 @@ def load_library(path=LIB_PATH):
         "cs_abi_version": (C.c_int, []),
         "cs_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_uint64), _dp]),
         "cs_reset_timing": (C.c_int, [C.c_void_p]),
+        "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cs_probe_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, _dp]),
@@ -203,6 +204,15 @@ class HipSolver:
     def reset_timing(self):
         self._need()
         _check(self._lib.cs_reset_timing(self._h))
+
+    def time_hvp(self, reps=200):
+        """Mean HVP launch time (ms, back-to-back launches bracketed by HIP
+        events on the solver stream) and algorithmic bytes per launch."""
+        self._need()
+        ms = C.c_double()
+        b = C.c_double()
+        _check(self._lib.cs_time_hvp(self._h, int(reps), C.byref(ms), C.byref(b)))
+        return ms.value, b.value
 
     # ---- component probes (parity tests) ------------------------------------
     def probe_active_set(self, x0, x, cap=1 << 20):

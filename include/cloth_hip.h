@@ -216,6 +216,13 @@ enum {
 int cs_kernel_timing(const cs_solver* s, int kernel, double* total_ms, uint64_t* count, double* bytes);
 int cs_reset_timing(cs_solver* s);
 
+/* Measurement hook, no counterpart in the reference: mean duration (ms) of
+ * `reps` back-to-back launches of the PCG Hessian-vector kernel (k_mx_hvp in
+ * mixed precision, k_pcg_hvp in FP64) on the operands the last cs_step left
+ * resident, bracketed by HIP events on the solver stream; `bytes` receives
+ * the algorithmic bytes of one launch.  Requires a previous cs_step. */
+int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
+
 /* ---- component probes for parity tests (no reference counterpart) ------- */
 /* Begin a step at x0 and build the frozen active set at x (updateActiveSet,
  * contact-assembly.ts:164-220).  Writes min(count, cap) records in active-list

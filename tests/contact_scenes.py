@@ -51,4 +51,17 @@ def crumpled_sheet(nx=10, seed=3):
     return scene
 
 
-ALL = {"stacked": stacked_patches, "plate": plate_close, "floor": floor_close, "crumpled": crumpled_sheet}
+def stacked_large():
+    """Two stacked 33x33-cell patches (4,356 triangles): a hierarchy five
+    levels deep, so every refit level and multi-wave traversal is exercised."""
+    return stacked_patches(gap=1.2e-3, nx=33)
+
+
+def sphere_touch():
+    """48x48-vertex drape sheet 0.5 mm above the top of the 64x32 sphere
+    (4,096 static triangles): cloth/static candidates through both hierarchies."""
+    return S.sphere_drape(47, sphere_y=0.2495)
+
+
+ALL = {"stacked": stacked_patches, "plate": plate_close, "floor": floor_close, "crumpled": crumpled_sheet,
+       "stacked_large": stacked_large, "sphere_touch": sphere_touch}
