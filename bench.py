@@ -72,25 +72,59 @@ def timed_steps(g, h, steps):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(name, precond, budget_s):
-    """Oracle (single thread) on a bounded sample: whole steps from the
-    initial state until `budget_s` seconds have elapsed (at least one)."""
+def cpu_baseline(name, precond, budget_s, newton_per_step):
+    """CPU oracle (FP64, one host thread) on a bounded sample of the same
+    workload.  Small scenes run whole steps until `budget_s` has elapsed.
+    When one step exceeds the budget (the oracle needs ~100 s per step of the
+    256x256 drape), the sample is step 0 with the Newton cap lowered until it
+    fits, and the rate is scaled to whole steps by the Newton iterations per
+    step the GPU run used."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi as O
     from closim_amd import _abi
     scene = make_scene(name)
-    o = O.OracleSolver(scene, _abi.options(precond=precond))
-    n = 0
+    if scene.mesh.count <= 20000:
+        o = O.OracleSolver(scene, _abi.options(precond=precond))
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            o.step(1 / 60)
+            n += 1
+            if time.perf_counter() - t0 >= budget_s:
+                break
+        dt = time.perf_counter() - t0
+        return dict(value=n / dt, unit="steps/s", cores=1, kind="port",
+                    sample=f"first {n} step(s) of the same workload from its initial state "
+                           f"(CPU oracle, FP64, one host thread), {dt:.1f} s")
+    cap = 1
+    o = O.OracleSolver(scene, _abi.options(precond=precond, newton_iters=cap))
     t0 = time.perf_counter()
-    while True:
-        o.step(1 / 60)
-        n += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
+    o.step(1 / 60)
     dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="steps/s", cores=1, kind="port",
-                sample=f"first {n} step(s) of the same workload from its initial state on one host thread "
-                       f"(CPU oracle, FP64, same preconditioner), {dt:.1f} s")
+    used = max(o.lastStats["newton_used"], 1)
+    per_newton = dt / used
+    steps_per_s = 1.0 / (per_newton * newton_per_step)
+    return dict(value=steps_per_s, unit="steps/s", cores=1, kind="port",
+                sample=f"step 0 of the same workload with the Newton cap set to {cap} (predictor and CCD "
+                       f"bisection, {used} Newton iteration(s) with PCG and line search, velocity update and "
+                       f"filter) on one host thread, {dt:.1f} s; scaled to a whole step by the "
+                       f"{newton_per_step} Newton iterations per step of the GPU FP64 run")
+
+
+def copy_calibration(nbytes=256 << 20, reps=8):
+    """Device-to-device copies of a known size (hipMemcpyAsync), so that the
+    FETCH_SIZE / WRITE_SIZE counters of the same run can be scaled to bytes."""
+    lib = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert lib.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert lib.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert lib.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert lib.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert lib.hipDeviceSynchronize() == 0
+    lib.hipFree(a)
+    lib.hipFree(b)
+    print(f"[bench] copy calibration: {reps} device copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
 def main():
@@ -105,6 +139,8 @@ def main():
     ap.add_argument("--hvp-reps", type=int, default=200)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per HVP launch from the FETCH_SIZE/WRITE_SIZE counter runs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run device copies of known size (for counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the cloth step is single-device); got --gpus {args.gpus}",
@@ -151,6 +187,19 @@ def main():
                            newton_used_last_step=st_d["newton_used"], pcg_iters_last_step=st_d["pcg_iters"])
         res["mixed_over_fp64"] = t_d / t_m
         res["pos_rel_err_mixed_vs_fp64"] = err
+        # sensitivity of the same capped solve to the Hessian operator alone:
+        # FP64 with the analytic HVP instead of the FD one
+        sc_a = make_scene(args.workload)
+        from closim_amd import _abi as A_
+        from closim_amd.solver import HipSolver
+        ga = HipSolver("fp64", precond=precond, hvp_mode=A_.CS_HVP_ANALYTIC)
+        ga.initialize(sc_a)
+        for _ in range(args.warmup + args.steps):
+            raw_step(ga, h)
+        ga._copy_out()
+        pa = ga.getPositions()
+        res["pos_rel_err_fp64_analytic_vs_fp64"] = float(np.max(np.abs(pa - pd)) / np.max(np.abs(pd)))
+        ga.close()
     mean_s = hvp_ms / 1000.0
     achieved = hvp_bytes / mean_s / 1e9 if mean_s > 0 else 0.0
     # PCIe-inclusive rate (positions + velocities copied out each step)
@@ -184,12 +233,16 @@ def main():
         "pcg_iters_last_step": st_m["pcg_iters"],
         "active_contacts_last_step": st_m["contact"]["active_pairs"] if st_m["contact"] else 0,
         "candidate_prims_last_step": st_m["contact"]["candidate_pairs"] if st_m["contact"] else 0,
+        "velocity_filter_rounds_last_step": st_m["contact"]["filter_rounds"] if st_m["contact"] else 0,
         "pcie_inclusive_steps_per_s": pcie_rate,
     }
     out.update(res)
+    if args.copy_calibration:
+        copy_calibration()
     if not args.no_cpu:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.workload, precond, args.cpu_budget)
+        nps = res.get("fp64", {}).get("newton_used_last_step", st_m["newton_used"])
+        out["cpu_baseline"] = cpu_baseline(args.workload, precond, args.cpu_budget, nps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -80,6 +80,7 @@ class CsContactDiag(C.Structure):
         ("min_distance", C.c_double), ("min_toi", C.c_double),
         ("barrier_energy", C.c_double), ("friction_work", C.c_double),
         ("rejected_line_search_steps", C.c_uint64), ("ccd_failures", C.c_uint64),
+        ("filter_rounds", C.c_uint64),
     ]
 
 

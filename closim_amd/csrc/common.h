@@ -64,6 +64,7 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
 // Bounded grid for reduction kernels: a fixed count per size keeps the
 // partial layout (and thus the reduction tree) deterministic.
 constexpr unsigned kMaxRedBlocks = 1024;  // partial slots per reduction
+constexpr unsigned kMaxHvpBlocks = 4096;  // HVP kernels: more blocks to hide gather latency
 inline unsigned red_grid(size_t n, int block = kBlock) {
   size_t g = (n + block - 1) / block;
   if (g > kMaxRedBlocks) g = kMaxRedBlocks;
@@ -168,6 +169,33 @@ __device__ __forceinline__ double merge_partials_ext(const double* partials, uns
   double acc = Op::ident();
   for (unsigned i = threadIdx.x; i < count; i += BLOCK) acc = Op::op(acc, partials[i]);
   return block_reduce_ext<BLOCK, Op>(acc);
+}
+
+// acc + v[b*S] + v[(b+1)*S] + ... + v[(e-1)*S], added in that order (two
+// values per entry, v[k*S] then v[k*S + D], when D > 0).  Loads are issued
+// 32 entries ahead of the dependent additions.
+template <int S, int D>
+__device__ __forceinline__ double ordered_sum(const double* __restrict__ v, uint32_t b, uint32_t e, double acc) {
+  constexpr int U = 32;
+  uint32_t k = b;
+  for (; k + U <= e; k += U) {
+    double t0[U], t1[U];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      t0[j] = v[(size_t)(k + j) * S];
+      if (D > 0) t1[j] = v[(size_t)(k + j) * S + D];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      acc += t0[j];
+      if (D > 0) acc += t1[j];
+    }
+  }
+  for (; k < e; k++) {
+    acc += v[(size_t)k * S];
+    if (D > 0) acc += v[(size_t)k * S + D];
+  }
+  return acc;
 }
 
 // A latch read once per block: thread 0 loads it and the block shares the

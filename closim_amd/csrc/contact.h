@@ -35,6 +35,7 @@ class ContactSystem {
   uint64_t copy_active(int32_t* ids6, double* vals7, uint64_t cap);
   uint64_t candidate_count() const;
   uint64_t incidence_count() const;
+  uint64_t incidence_vertices() const;
   ContactImpl* impl() { return im_; }
 
  private:

@@ -46,7 +46,7 @@ struct CScalars {
   double red[16];
   unsigned tickets[16];
   uint64_t mink[16];   // okey images of LtMin reductions (all ones = no value)
-  uint32_t pair_count, spair_count, active_count, entry_count, nonfinite, floor_fail;
+  uint32_t pair_count, spair_count, active_count, entry_count, nonfinite, floor_fail, nverts, kill_rounds;
   uint32_t kind_count[4];
   uint64_t nan_key, cross_key;
 };
@@ -62,6 +62,13 @@ __device__ __forceinline__ void vpos(const DGeom& G, const double* x, int v, dou
 
 __device__ __forceinline__ bool box_overlap(const double* a, const double* b) {
   return a[0] <= b[3] && a[3] >= b[0] && a[1] <= b[4] && a[4] >= b[1] && a[2] <= b[5] && a[5] >= b[2];
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // ---------------------------------------------------------------- LBVH
@@ -360,6 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_traverse_self(const Leaf* __restrict
   __shared__ uint32_t stk[kBlock / kWave][kTravStack];
   __shared__ uint2 stage[kBlock / kWave][kPairStage];
   const int wid = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
   const uint32_t m = LV.cnt[0];
   PairStage ps{stage[wid], 0};
   const uint32_t i = (blockIdx.x * (kBlock / kWave) + wid) * kWave + threadIdx.x % kWave;
@@ -384,20 +392,36 @@ __global__ void __launch_bounds__(kBlock) k_traverse_self(const Leaf* __restrict
     const int cl = (int)(e >> 28) - 1;
     const uint32_t k = e & 0x0fffffffu;
     const uint32_t c0 = k * kFan, c1 = min(c0 + kFan, LV.cnt[cl]);
+    // lanes 0..7 fetch the (up to 8) children with one vector load; each is
+    // then broadcast to the packet with readlane
+    const int nch = (int)(c1 - c0);
     if (cl == 0) {
-      for (uint32_t j = c0; j < c1; j++) {
-        const Leaf& R = leaves[j];
-        const bool hit = act && j > i && box_overlap6(Q, R.lo, R.hi) &&
-                         !(A[0] == R.v[0] || A[0] == R.v[1] || A[0] == R.v[2] || A[1] == R.v[0] ||
-                           A[1] == R.v[1] || A[1] == R.v[2] || A[2] == R.v[0] || A[2] == R.v[1] || A[2] == R.v[2]);
-        const uint32_t tj = R.tri;
+      Leaf mine;
+      if (lane < nch) mine = leaves[c0 + lane];
+      for (int q = 0; q < nch; q++) {
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; k++) { lo[k] = readlane_d(mine.lo[k], q); hi[k] = readlane_d(mine.hi[k], q); }
+        const int32_t v0 = __builtin_amdgcn_readlane(mine.v[0], q), v1 = __builtin_amdgcn_readlane(mine.v[1], q),
+                      v2 = __builtin_amdgcn_readlane(mine.v[2], q);
+        const uint32_t tj = (uint32_t)__builtin_amdgcn_readlane((int)mine.tri, q);
+        const uint32_t j = c0 + q;
+        const bool hit = act && j > i && box_overlap6(Q, lo, hi) &&
+                         !(A[0] == v0 || A[0] == v1 || A[0] == v2 || A[1] == v0 || A[1] == v1 || A[1] == v2 ||
+                           A[2] == v0 || A[2] == v1 || A[2] == v2);
         ps.push(hit, make_uint2(ti < tj ? ti : tj, ti < tj ? tj : ti), &cs->pair_count, pairs, cap);
       }
     } else {
-      for (uint32_t c = c1; c-- > c0;) {
-        const double* b = nodes + ((size_t)LV.off[cl] + c) * 6;
+      double mb[6];
+      if (lane < nch) {
+        const double* b = nodes + ((size_t)LV.off[cl] + c0 + lane) * 6;
+        for (int k = 0; k < 6; k++) mb[k] = b[k];
+      }
+      for (int q = nch - 1; q >= 0; q--) {
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; k++) { lo[k] = readlane_d(mb[k], q); hi[k] = readlane_d(mb[3 + k], q); }
+        const uint32_t c = c0 + q;
         const uint64_t last = min((uint64_t)(c + 1) * LV.span[cl], (uint64_t)m) - 1;
-        const bool hit = act && last > i && box_overlap6(Q, b, b + 3);
+        const bool hit = act && last > i && box_overlap6(Q, lo, hi);
         if (__any(hit)) {
           if (sp < kTravStack) S[sp++] = stk_enc(cl, c);
           else overflow = true;
@@ -440,14 +464,24 @@ __global__ void __launch_bounds__(kBlock) k_traverse_static(const Leaf* __restri
   // children are tested; a single static triangle is tested directly.
   const uint32_t ms = SL.cnt[0];
   bool overflow = false;
-  auto test_leaf = [&](uint32_t j) {
-    const Leaf& R = sleaves[j];
-    double lo[3], hi[3];
-    for (int k = 0; k < 3; k++) { lo[k] = R.lo[k] - pad; hi[k] = R.hi[k] + pad; }
-    ps.push(act && box_overlap6(Q, lo, hi), make_uint2(ti, R.tri), &cs->spair_count, pairs, cap);
+  const int lane = threadIdx.x % kWave;
+  // leaves [c0, c1) (at most 8): one vector load, readlane broadcast
+  auto test_leaves = [&](uint32_t c0, uint32_t c1) {
+    const int nch = (int)(c1 - c0);
+    Leaf mine;
+    if (lane < nch) mine = sleaves[c0 + lane];
+    for (int q = 0; q < nch; q++) {
+      double lo[3], hi[3];
+      for (int k = 0; k < 3; k++) {
+        lo[k] = readlane_d(mine.lo[k], q) - pad;
+        hi[k] = readlane_d(mine.hi[k], q) + pad;
+      }
+      const uint32_t st_id = (uint32_t)__builtin_amdgcn_readlane((int)mine.tri, q);
+      ps.push(act && box_overlap6(Q, lo, hi), make_uint2(ti, st_id), &cs->spair_count, pairs, cap);
+    }
   };
   if (SL.L == 0) {
-    if (ms) test_leaf(0);
+    if (ms) test_leaves(0, 1);
     ps.flush(&cs->spair_count, pairs, cap);
     return;
   }
@@ -458,15 +492,23 @@ __global__ void __launch_bounds__(kBlock) k_traverse_static(const Leaf* __restri
     const uint32_t k = e & 0x0fffffffu;
     const uint32_t c0 = k * kFan, c1 = min(c0 + kFan, SL.cnt[cl]);
     if (cl == 0) {
-      for (uint32_t j = c0; j < c1; j++) test_leaf(j);
+      test_leaves(c0, c1);
     } else {
-      for (uint32_t c = c1; c-- > c0;) {
-        const double* b = snodes + ((size_t)SL.off[cl] + c) * 6;
-        const double lo[3] = {b[0] - pad, b[1] - pad, b[2] - pad};
-        const double hi[3] = {b[3] + pad, b[4] + pad, b[5] + pad};
+      const int nch = (int)(c1 - c0);
+      double mb[6];
+      if (lane < nch) {
+        const double* b = snodes + ((size_t)SL.off[cl] + c0 + lane) * 6;
+        for (int k = 0; k < 6; k++) mb[k] = b[k];
+      }
+      for (int q = nch - 1; q >= 0; q--) {
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; k++) {
+          lo[k] = readlane_d(mb[k], q) - pad;
+          hi[k] = readlane_d(mb[3 + k], q) + pad;
+        }
         const bool hit = act && box_overlap6(Q, lo, hi);
         if (__any(hit)) {
-          if (sp < kTravStack) S[sp++] = stk_enc(cl, c);
+          if (sp < kTravStack) S[sp++] = stk_enc(cl, c0 + q);
           else overflow = true;
         }
       }
@@ -695,9 +737,26 @@ __global__ void __launch_bounds__(kBlock) k_emit_entries(const Frozen* __restric
 
 // v_ent[i] = low bits of the sorted incidence keys; v_off[u] = first entry
 // of vertex u (lower bound of u in the sorted vertex field), u in [0, n].
-__global__ void k_entry_ent(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t* v_ent) {
+__global__ void k_entry_ent(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t* v_ent,
+                            uint32_t* ent_vtx) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cnt) v_ent[i] = (uint32_t)(keys[i] & ((1ull << cshift) - 1));
+  if (i >= cnt) return;
+  v_ent[i] = (uint32_t)(keys[i] & ((1ull << cshift) - 1));
+  ent_vtx[i] = (uint32_t)(keys[i] >> cshift);
+}
+
+// Vertices that own incidence entries (order irrelevant).
+__global__ void __launch_bounds__(kBlock) k_contact_verts(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift,
+                                                          uint32_t* cverts, CScalars* cs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool head = false;
+  uint32_t u = 0;
+  if (i < cnt) {
+    u = (uint32_t)(keys[i] >> cshift);
+    head = i == 0 || (uint32_t)(keys[i - 1] >> cshift) != u;
+  }
+  const uint32_t k = wave_alloc(head, &cs->nverts);
+  if (head) cverts[k] = u;
 }
 
 __global__ void k_entry_offsets(const uint64_t* __restrict__ keys, uint32_t cnt, int cshift, uint32_t n,
@@ -829,132 +888,103 @@ __global__ void __launch_bounds__(kBlock) k_contact_eval(DGeom G, const double* 
   block_partial_dd<kBlock>(fwork, partials + kMaxRedBlocks);
 }
 
-__device__ __forceinline__ double readlane_d(double x, int l) {
-  const long long u = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
-  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // Contact gradient per vertex: addDistGrad then addFrictionForce for each
-// active contact in list order (contact-assembly.ts:272-279, 288-341).
-// One wave per vertex: lanes evaluate 64 entries at a time and every lane
-// adds them in list order (the two additions of an entry in their order;
-// components an entry does not touch get +0, which leaves a sum that
-// started at +0 unchanged).
-__global__ void __launch_bounds__(kBlock) k_contact_grad(uint32_t n, const Frozen* __restrict__ act,
-                                                         const uint32_t* __restrict__ v_off,
-                                                         const uint32_t* __restrict__ v_ent, CEval ev,
-                                                         double* __restrict__ ceg) {
-  const uint32_t v = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
-  if (v >= n) return;
-  double acc[3] = {0, 0, 0};
-  const uint32_t b = v_off[v], e = v_off[v + 1];
-  for (uint32_t r0 = b; r0 < e; r0 += kWave) {
-    double ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
-    const uint32_t k = r0 + lane;
-    if (k < e) {
-      const uint32_t ent = v_ent[k];
-      const uint32_t ci = ent >> 3;
-      const int slot = ent & 7;
-      if (ev.inr[ci]) {
-        const Frozen c = act[ci];
-        const double g = ev.g[ci];
-        const double n3[3] = {c.nx, c.ny, c.nz};
-        const double fp[3] = {-ev.fr[(size_t)ci * 3], -ev.fr[(size_t)ci * 3 + 1], -ev.fr[(size_t)ci * 3 + 2]};
-        if (c.kind == 2) {
-          ta[1] = g;
-          for (int q = 0; q < 3; q++) tb[q] = fp[q];
-        } else if (c.kind == 0) {
-          if (slot == 0) {
-            for (int q = 0; q < 3; q++) ta[q] = g * n3[q];
-            for (int q = 0; q < 3; q++) tb[q] = fp[q];
-          } else {
-            const double w = slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2;
-            for (int q = 0; q < 3; q++) ta[q] = -g * w * n3[q];
-            for (int q = 0; q < 3; q++) tb[q] = -fp[q] * w;
-          }
-        } else {
-          const double w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
-          const double ww = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? -c.w2 : -c.w3;
-          for (int q = 0; q < 3; q++) ta[q] = g * w * n3[q];
-          for (int q = 0; q < 3; q++) tb[q] = fp[q] * ww;
-        }
-      }
-    }
-    const int cnt = (int)min((uint32_t)kWave, e - r0);
-    for (int j = 0; j < cnt; j++)
-      for (int q = 0; q < 3; q++) {
-        acc[q] += readlane_d(ta[q], j);
-        acc[q] += readlane_d(tb[q], j);
-      }
-  }
-  if (lane == 0) {
-    ceg[(size_t)v * 3] = acc[0];
-    ceg[(size_t)v * 3 + 1] = acc[1];
-    ceg[(size_t)v * 3 + 2] = acc[2];
-  }
-}
-
-// addDiagEstimate (contact-assembly.ts:436-471): accumulate into diag, one
-// wave per vertex as above.
-__global__ void __launch_bounds__(kBlock) k_contact_diag(uint32_t n, const Frozen* __restrict__ act,
-                                                         const uint32_t* __restrict__ v_off,
-                                                         const uint32_t* __restrict__ v_ent, CEval ev,
-                                                         double* __restrict__ diag) {
-  const uint32_t v = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x % kWave;
-  if (v >= n) return;
-  const uint32_t b = v_off[v], e = v_off[v + 1];
-  if (b == e) return;
-  double dg[3] = {diag[(size_t)v * 3], diag[(size_t)v * 3 + 1], diag[(size_t)v * 3 + 2]};
-  for (uint32_t r0 = b; r0 < e; r0 += kWave) {
-    double t[3] = {0, 0, 0};
-    int has = 0;
-    const uint32_t k = r0 + lane;
-    if (k < e) {
-      const uint32_t ent = v_ent[k];
-      const uint32_t ci = ent >> 3;
-      const int slot = ent & 7;
-      if (ev.inr[ci]) {
-        has = 1;
-        const Frozen c = act[ci];
-        const double kk = ev.coeff[ci];
-        if (c.kind == 2) {
-          has = 2;
-          t[1] = kk;
-        } else {
-          const double n2[3] = {c.nx * c.nx, c.ny * c.ny, c.nz * c.nz};
-          if (c.kind == 0 && slot == 0) {
-            for (int q = 0; q < 3; q++) t[q] = kk * n2[q];
-          } else {
-            const double w = c.kind == 0 ? (slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2)
-                                         : (slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3);
-            for (int q = 0; q < 3; q++) t[q] = kk * w * w * n2[q];
-          }
-        }
-      }
-    }
-    const int cnt = (int)min((uint32_t)kWave, e - r0);
-    for (int j = 0; j < cnt; j++) {
-      const int h = __builtin_amdgcn_readlane(has, j);
-      if (h == 0) continue;
-      if (h == 2) {
-        dg[1] += readlane_d(t[1], j);
+// active contact in list order (contact-assembly.ts:272-279, 288-341), in
+// two passes: every incidence entry's two additions (distance-gradient term,
+// then friction term) in parallel, then one thread per contact vertex and
+// component adds them in list order.  Components an entry does not touch
+// carry +0, which leaves a sum that started at +0 unchanged.
+__global__ void __launch_bounds__(kBlock) k_grad_terms(uint32_t E, const Frozen* __restrict__ act,
+                                                       const uint32_t* __restrict__ v_ent, CEval ev,
+                                                       double* __restrict__ gterm) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  double ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0};
+  const uint32_t ent = v_ent[k];
+  const uint32_t ci = ent >> 3;
+  const int slot = ent & 7;
+  if (ev.inr[ci]) {
+    const Frozen c = act[ci];
+    const double g = ev.g[ci];
+    const double n3[3] = {c.nx, c.ny, c.nz};
+    const double fp[3] = {-ev.fr[(size_t)ci * 3], -ev.fr[(size_t)ci * 3 + 1], -ev.fr[(size_t)ci * 3 + 2]};
+    if (c.kind == 2) {
+      ta[1] = g;
+      for (int q = 0; q < 3; q++) tb[q] = fp[q];
+    } else if (c.kind == 0) {
+      if (slot == 0) {
+        for (int q = 0; q < 3; q++) ta[q] = g * n3[q];
+        for (int q = 0; q < 3; q++) tb[q] = fp[q];
       } else {
-        for (int q = 0; q < 3; q++) dg[q] += readlane_d(t[q], j);
+        const double w = slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2;
+        for (int q = 0; q < 3; q++) ta[q] = -g * w * n3[q];
+        for (int q = 0; q < 3; q++) tb[q] = -fp[q] * w;
       }
+    } else {
+      const double w = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
+      const double ww = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? -c.w2 : -c.w3;
+      for (int q = 0; q < 3; q++) ta[q] = g * w * n3[q];
+      for (int q = 0; q < 3; q++) tb[q] = fp[q] * ww;
     }
   }
-  if (lane == 0) {
-    diag[(size_t)v * 3] = dg[0];
-    diag[(size_t)v * 3 + 1] = dg[1];
-    diag[(size_t)v * 3 + 2] = dg[2];
+  for (int q = 0; q < 3; q++) {
+    gterm[(size_t)k * 6 + q] = ta[q];
+    gterm[(size_t)k * 6 + 3 + q] = tb[q];
   }
 }
 
-// distJvp (contact-assembly.ts:394-414) times the frozen curvature.
+__global__ void __launch_bounds__(kBlock) k_grad_vsum(uint32_t nv, const uint32_t* __restrict__ cverts,
+                                                      const uint32_t* __restrict__ v_off,
+                                                      const double* __restrict__ gterm, double* __restrict__ ceg) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nv * 3) return;
+  const uint32_t v = cverts[t / 3];
+  const int q = t % 3;
+  ceg[(size_t)v * 3 + q] = ordered_sum<6, 3>(gterm + q, v_off[v], v_off[v + 1], 0.0);
+}
+
+// addDiagEstimate (contact-assembly.ts:436-471): accumulate into diag, the
+// same two passes (a floor contact touches only the y component; +0 added to
+// a positive diagonal leaves it unchanged).
+__global__ void __launch_bounds__(kBlock) k_diag_terms(uint32_t E, const Frozen* __restrict__ act,
+                                                       const uint32_t* __restrict__ v_ent, CEval ev,
+                                                       double* __restrict__ dterm) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  double t[3] = {0, 0, 0};
+  const uint32_t ent = v_ent[k];
+  const uint32_t ci = ent >> 3;
+  const int slot = ent & 7;
+  if (ev.inr[ci]) {
+    const Frozen c = act[ci];
+    const double kk = ev.coeff[ci];
+    if (c.kind == 2) {
+      t[1] = kk;
+    } else {
+      const double n2[3] = {c.nx * c.nx, c.ny * c.ny, c.nz * c.nz};
+      if (c.kind == 0 && slot == 0) {
+        for (int q = 0; q < 3; q++) t[q] = kk * n2[q];
+      } else {
+        const double w = c.kind == 0 ? (slot == 1 ? c.w0 : slot == 2 ? c.w1 : c.w2)
+                                     : (slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3);
+        for (int q = 0; q < 3; q++) t[q] = kk * w * w * n2[q];
+      }
+    }
+  }
+  for (int q = 0; q < 3; q++) dterm[(size_t)k * 3 + q] = t[q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_diag_vsum(uint32_t nv, const uint32_t* __restrict__ cverts,
+                                                      const uint32_t* __restrict__ v_off,
+                                                      const double* __restrict__ dterm, double* __restrict__ diag) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nv * 3) return;
+  const uint32_t v = cverts[t / 3];
+  const int q = t % 3;
+  const size_t o = (size_t)v * 3 + q;
+  diag[o] = ordered_sum<3, 0>(dterm + q, v_off[v], v_off[v + 1], diag[o]);
+}
+
 // meritEnergy (contact-assembly.ts:419-428)
 __global__ void __launch_bounds__(kBlock) k_merit(DGeom G, const double* __restrict__ x,
                                                   const Frozen* __restrict__ act, uint32_t C, jsm::dd* partials,
@@ -1000,107 +1030,173 @@ __global__ void k_count_below(const uint64_t* keys, uint32_t n, uint32_t* out) {
 
 // killApproachVelocity (newton.ts:238-303): the reference sweeps the active
 // list once in order.  A contact may run as soon as every earlier contact that
-// shares one of its vertices has run (it is then at the head of each of its
-// vertices' incidence lists), so rounds of ready contacts reproduce the
-// sequential result exactly.  One work-group; every round runs at least the
-// earliest pending contact, so the loop ends within C rounds.
+// shares one of its vertices has run, i.e. when it is at the head of each of
+// its vertices' incidence lists (which are in list order).  Contacts that are
+// ready together share no vertex, so each round runs its whole frontier in
+// parallel and the result is the sequential sweep's.  A finished contact
+// advances the heads of its vertices and proposes the next contact of each
+// list for the following round, so a round costs its frontier, not C.  One
+// work-group; every round runs at least the earliest pending contact, so the
+// loop ends within C rounds.
+__device__ __forceinline__ int kill_verts(const Frozen& c, int* vs) {
+  if (c.kind == 2) { vs[0] = c.p; vs[1] = vs[2] = vs[3] = -1; }
+  else if (c.kind == 0) { vs[0] = c.p; vs[1] = c.a; vs[2] = c.b; vs[3] = c.c; }
+  else { vs[0] = c.a; vs[1] = c.b; vs[2] = c.c; vs[3] = c.d; }
+  return 4;
+}
+
+__device__ __forceinline__ bool kill_ready(const Frozen* __restrict__ act, const uint32_t* __restrict__ v_off,
+                                           const uint32_t* __restrict__ v_ent, const uint32_t* heads, uint32_t i) {
+  const Frozen c = act[i];
+  int vs[4];
+  kill_verts(c, vs);
+  for (int s = 0; s < 4; s++) {
+    if (vs[s] < 0) continue;
+    const uint32_t h = heads[vs[s]];
+    if (h >= v_off[vs[s] + 1] || (v_ent[h] >> 3) != i) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void kill_apply(const Frozen& c, double* v) {
+  if (c.kind == 2) {
+    const double vy = v[(size_t)c.p * 3 + 1];
+    if (vy < 0) v[(size_t)c.p * 3 + 1] = 0;
+  } else if (c.kind == 0) {
+    const size_t p = c.p;
+    const double vp[3] = {v[p * 3], v[p * 3 + 1], v[p * 3 + 2]};
+    double tx = 0, ty = 0, tz = 0;
+    const double ws[3] = {c.w0, c.w1, c.w2};
+    const int vs[3] = {c.a, c.b, c.c};
+    for (int k = 0; k < 3; k++) {
+      if (vs[k] < 0) continue;
+      const size_t q = vs[k];
+      tx += ws[k] * v[q * 3];
+      ty += ws[k] * v[q * 3 + 1];
+      tz += ws[k] * v[q * 3 + 2];
+    }
+    const double rel = (vp[0] - tx) * c.nx + (vp[1] - ty) * c.ny + (vp[2] - tz) * c.nz;
+    if (rel < 0) {
+      const double j = -0.5 * rel;
+      v[p * 3] += j * c.nx;
+      v[p * 3 + 1] += j * c.ny;
+      v[p * 3 + 2] += j * c.nz;
+      for (int k = 0; k < 3; k++) {
+        if (vs[k] < 0) continue;
+        const size_t q = vs[k];
+        v[q * 3] -= j * ws[k] * c.nx;
+        v[q * 3 + 1] -= j * ws[k] * c.ny;
+        v[q * 3 + 2] -= j * ws[k] * c.nz;
+      }
+    }
+  } else {
+    double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, Cc[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+    for (int q = 0; q < 3; q++) {
+      if (c.a >= 0) A[q] = v[(size_t)c.a * 3 + q];
+      if (c.b >= 0) B[q] = v[(size_t)c.b * 3 + q];
+      if (c.c >= 0) Cc[q] = v[(size_t)c.c * 3 + q];
+      if (c.d >= 0) D[q] = v[(size_t)c.d * 3 + q];
+    }
+    double ab[3], cd[3];
+    for (int q = 0; q < 3; q++) {
+      ab[q] = (A[q] + B[q]) / 2;
+      cd[q] = (Cc[q] + D[q]) / 2;
+    }
+    const double rel = (ab[0] - cd[0]) * c.nx + (ab[1] - cd[1]) * c.ny + (ab[2] - cd[2]) * c.nz;
+    if (rel < 0) {
+      const double j = -0.25 * rel;
+      const int plus[2] = {c.a, c.b}, minus[2] = {c.c, c.d};
+      for (int vv : plus) {
+        if (vv < 0) continue;
+        v[(size_t)vv * 3] += j * c.nx;
+        v[(size_t)vv * 3 + 1] += j * c.ny;
+        v[(size_t)vv * 3 + 2] += j * c.nz;
+      }
+      for (int vv : minus) {
+        if (vv < 0) continue;
+        v[(size_t)vv * 3] -= j * c.nx;
+        v[(size_t)vv * 3 + 1] -= j * c.ny;
+        v[(size_t)vv * 3 + 2] -= j * c.nz;
+      }
+    }
+  }
+}
+
+// heads = v_off[0..n) on entry; flag[C] = 0; listA / listB hold C entries.
+// Run-ahead: the thread that ran contact i owns i's cloth vertices for the
+// rest of the round (no other ready contact shares them), so it may go on
+// with the next contact of i's first vertex when that contact is ready and
+// touches only those vertices: long runs of contacts on one vertex or edge
+// (a vertex against many static triangles) then take one round, not one each.
 __global__ void __launch_bounds__(1024) k_kill_approach(const Frozen* __restrict__ act, uint32_t C,
+                                                        const uint32_t* __restrict__ v_off,
                                                         const uint32_t* __restrict__ v_ent, uint32_t* heads,
-                                                        uint8_t* state, double* v) {
-  __shared__ uint32_t s_left;
-  for (uint32_t round = 0; round <= C; round++) {
-    if (threadIdx.x == 0) s_left = 0;
+                                                        uint32_t* flag, uint32_t* listA, uint32_t* listB, double* v,
+                                                        uint32_t* rounds_out) {
+  __shared__ uint32_t nA, nB;
+  if (threadIdx.x == 0) nA = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < C; i += blockDim.x)
+    if (kill_ready(act, v_off, v_ent, heads, i)) listA[atomicAdd(&nA, 1u)] = i;
+  __syncthreads();
+  uint32_t round = 1;
+  for (; round <= C && nA > 0; round++) {
+    if (threadIdx.x == 0) nB = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) {
-      if (state[i] == 2) continue;
-      const Frozen c = act[i];
-      int vs[4];
-      if (c.kind == 2) { vs[0] = c.p; vs[1] = vs[2] = vs[3] = -1; }
-      else if (c.kind == 0) { vs[0] = c.p; vs[1] = c.a; vs[2] = c.b; vs[3] = c.c; }
-      else { vs[0] = c.a; vs[1] = c.b; vs[2] = c.c; vs[3] = c.d; }
-      bool ready = true;
-      for (int s = 0; s < 4 && ready; s++)
-        if (vs[s] >= 0 && (v_ent[heads[vs[s]]] >> 3) != i) ready = false;
-      state[i] = ready ? 1 : 0;
-      atomicAdd(&s_left, 1u);
+    const uint32_t na = nA;
+    for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) {
+      uint32_t i = listA[k];
+      int own[4];
+      kill_verts(act[i], own);
+      for (uint32_t guard = 0; guard <= C; guard++) {
+        const Frozen c = act[i];
+        kill_apply(c, v);
+        int vs[4];
+        kill_verts(c, vs);
+        for (int s = 0; s < 4; s++)
+          if (vs[s] >= 0) ++heads[vs[s]];
+        // run ahead: a ready contact at the head of one of the owned
+        // vertices' lists whose cloth vertices are all owned
+        bool found = false;
+        uint32_t j = 0;
+        for (int s = 0; s < 4 && !found; s++) {
+          const int u = own[s];
+          if (u < 0) continue;
+          const uint32_t h = heads[u];
+          if (h >= v_off[u + 1]) continue;
+          j = v_ent[h] >> 3;
+          int ws[4];
+          kill_verts(act[j], ws);
+          bool inside = true;
+          for (int q = 0; q < 4 && inside; q++) {
+            if (ws[q] < 0) continue;
+            inside = ws[q] == own[0] || ws[q] == own[1] || ws[q] == own[2] || ws[q] == own[3];
+          }
+          found = inside && kill_ready(act, v_off, v_ent, heads, j);
+        }
+        if (!found) break;
+        i = j;
+      }
+      for (int s = 0; s < 4; s++) {
+        if (own[s] < 0) continue;
+        const uint32_t h = heads[own[s]];
+        if (h < v_off[own[s] + 1]) {
+          const uint32_t nxt = v_ent[h] >> 3;
+          if (atomicExch(&flag[nxt], round) != round) listB[atomicAdd(&nB, 1u)] = nxt;
+        }
+      }
     }
     __syncthreads();
-    if (s_left == 0) break;
-    for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) {
-      if (state[i] != 1) continue;
-      const Frozen c = act[i];
-      if (c.kind == 2) {
-        const double vy = v[(size_t)c.p * 3 + 1];
-        if (vy < 0) v[(size_t)c.p * 3 + 1] = 0;
-        heads[c.p]++;
-      } else if (c.kind == 0) {
-        const size_t p = c.p;
-        const double vp[3] = {v[p * 3], v[p * 3 + 1], v[p * 3 + 2]};
-        double tx = 0, ty = 0, tz = 0;
-        const double ws[3] = {c.w0, c.w1, c.w2};
-        const int vs[3] = {c.a, c.b, c.c};
-        for (int k = 0; k < 3; k++) {
-          if (vs[k] < 0) continue;
-          const size_t q = vs[k];
-          tx += ws[k] * v[q * 3];
-          ty += ws[k] * v[q * 3 + 1];
-          tz += ws[k] * v[q * 3 + 2];
-        }
-        const double rel = (vp[0] - tx) * c.nx + (vp[1] - ty) * c.ny + (vp[2] - tz) * c.nz;
-        if (rel < 0) {
-          const double j = -0.5 * rel;
-          v[p * 3] += j * c.nx;
-          v[p * 3 + 1] += j * c.ny;
-          v[p * 3 + 2] += j * c.nz;
-          for (int k = 0; k < 3; k++) {
-            if (vs[k] < 0) continue;
-            const size_t q = vs[k];
-            v[q * 3] -= j * ws[k] * c.nx;
-            v[q * 3 + 1] -= j * ws[k] * c.ny;
-            v[q * 3 + 2] -= j * ws[k] * c.nz;
-          }
-        }
-        heads[p]++;
-        for (int k = 0; k < 3; k++)
-          if (vs[k] >= 0) heads[vs[k]]++;
-      } else {
-        double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, Cc[3] = {0, 0, 0}, D[3] = {0, 0, 0};
-        for (int q = 0; q < 3; q++) {
-          if (c.a >= 0) A[q] = v[(size_t)c.a * 3 + q];
-          if (c.b >= 0) B[q] = v[(size_t)c.b * 3 + q];
-          if (c.c >= 0) Cc[q] = v[(size_t)c.c * 3 + q];
-          if (c.d >= 0) D[q] = v[(size_t)c.d * 3 + q];
-        }
-        double ab[3], cd[3];
-        for (int q = 0; q < 3; q++) {
-          ab[q] = (A[q] + B[q]) / 2;
-          cd[q] = (Cc[q] + D[q]) / 2;
-        }
-        const double rel = (ab[0] - cd[0]) * c.nx + (ab[1] - cd[1]) * c.ny + (ab[2] - cd[2]) * c.nz;
-        if (rel < 0) {
-          const double j = -0.25 * rel;
-          const int plus[2] = {c.a, c.b}, minus[2] = {c.c, c.d};
-          for (int vv : plus) {
-            if (vv < 0) continue;
-            v[(size_t)vv * 3] += j * c.nx;
-            v[(size_t)vv * 3 + 1] += j * c.ny;
-            v[(size_t)vv * 3 + 2] += j * c.nz;
-          }
-          for (int vv : minus) {
-            if (vv < 0) continue;
-            v[(size_t)vv * 3] -= j * c.nx;
-            v[(size_t)vv * 3 + 1] -= j * c.ny;
-            v[(size_t)vv * 3 + 2] -= j * c.nz;
-          }
-        }
-        const int all[4] = {c.a, c.b, c.c, c.d};
-        for (int vv : all)
-          if (vv >= 0) heads[vv]++;
-      }
-      state[i] = 2;
+    if (threadIdx.x == 0) nA = 0;
+    __syncthreads();
+    const uint32_t nb = nB;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
+      const uint32_t i = listB[k];
+      if (kill_ready(act, v_off, v_ent, heads, i)) listA[atomicAdd(&nA, 1u)] = i;
     }
     __syncthreads();
   }
+  if (threadIdx.x == 0) *rounds_out = round - 1;
 }
 
 // checkTrial prims (contact-assembly.ts:497-535): exact distance at the trial
@@ -1221,11 +1317,12 @@ struct ContactImpl {
   uint32_t act_cap = 0, C = 0;
   // incidence
   DevBuf<uint64_t> ekeys, ekeys_t;
-  DevBuf<uint32_t> evals, evals_t, v_off, v_ent;
+  DevBuf<uint32_t> evals, evals_t, v_off, v_ent, ent_vtx;
   // per-contact evaluation
-  DevBuf<uint8_t> inr, kstate;
-  DevBuf<double> cg, fr, coeff, dd, hterm;
-  DevBuf<uint32_t> heads;
+  DevBuf<uint8_t> inr;
+  DevBuf<uint32_t> kflag, klistA, klistB;
+  DevBuf<double> cg, fr, coeff, dd, hvsum, hterm;
+  DevBuf<uint32_t> heads, cverts;
   // lagged friction table
   DevBuf<uint64_t> lag_keys, lag_keys_t;
   DevBuf<uint32_t> lag_idx, lag_idx_t, lag_count;
@@ -1359,6 +1456,8 @@ struct ContactImpl {
     CS_HIP(hipMemsetAsync(cs.ptr, 0, sizeof(CScalars), st));
     v_off.alloc(n + 1);
     heads.alloc(n);
+    cverts.alloc(std::max<uint32_t>(n, 1));
+    hvsum.alloc((size_t)std::max<uint32_t>(n, 1) * 3);
     lag_count.alloc(1);
     ensure_sort(m);
     grow_pairs(std::max<uint32_t>(1024, m * 4), std::max<uint32_t>(256, ms * 8));
@@ -1382,9 +1481,9 @@ struct ContactImpl {
     avals.alloc(cap); avals_t.alloc(cap);
     ekeys.alloc((size_t)cap * 4); ekeys_t.alloc((size_t)cap * 4);
     evals.alloc((size_t)cap * 4); evals_t.alloc((size_t)cap * 4); v_ent.alloc((size_t)cap * 4);
-    inr.alloc(cap); kstate.alloc(cap);
+    ent_vtx.alloc((size_t)cap * 4); hterm.alloc((size_t)cap * 24);
+    inr.alloc(cap); kflag.alloc(cap); klistA.alloc(cap); klistB.alloc(cap);
     cg.alloc(cap); fr.alloc((size_t)cap * 3); coeff.alloc(cap); dd.alloc(cap);
-    hterm.alloc((size_t)cap * 12);
     act_cap = cap;
     ensure_sort((size_t)cap * 4);
   }
@@ -1554,7 +1653,8 @@ struct ContactImpl {
   }
 
   int cshift = 3;
-  uint32_t E = 0;  // vertex -> contact incidence entries
+  uint32_t E = 0;   // vertex -> contact incidence entries
+  uint32_t nv = 0;  // vertices with entries
   void build_incidence() {
     CS_HIP(hipMemsetAsync(v_off.ptr, 0, (size_t)(n + 1) * sizeof(uint32_t), st));
     E = 0;
@@ -1575,10 +1675,19 @@ struct ContactImpl {
     sync_cs();
     const uint32_t cnt = hcs->entry_count;
     E = cnt;
+    nv = 0;
     if (cnt) {
-      hipLaunchKernelGGL(k_entry_ent, dim3(grid_for(cnt)), dim3(kBlock), 0, st, sk, cnt, cshift, v_ent.ptr);
+      hipLaunchKernelGGL(k_entry_ent, dim3(grid_for(cnt)), dim3(kBlock), 0, st, sk, cnt, cshift, v_ent.ptr,
+                         ent_vtx.ptr);
       hipLaunchKernelGGL(k_entry_offsets, dim3(grid_for((size_t)n + 1)), dim3(kBlock), 0, st, sk, cnt, cshift, n,
                          v_off.ptr);
+      CS_HIP(hipMemsetAsync(&cs.ptr->nverts, 0, sizeof(uint32_t), st));
+      hipLaunchKernelGGL(k_contact_verts, dim3(grid_for(cnt)), dim3(kBlock), 0, st, sk, cnt, cshift, cverts.ptr,
+                         cs.ptr);
+      CS_HIP(hipMemsetAsync(hvsum.ptr, 0, (size_t)n * 3 * sizeof(double), st));
+      CS_LAUNCH_CHECK();
+      sync_cs();
+      nv = hcs->nverts;
     }
     CS_LAUNCH_CHECK();
   }
@@ -1596,8 +1705,13 @@ struct ContactImpl {
       f.j[1] = FinishJob{ddpart.ptr + kMaxRedBlocks, red_grid(C), FIN_DD, 0, &cs.ptr->red[3]};
       launch_finish(st, f, 2);
     }
-    hipLaunchKernelGGL(k_contact_grad, dim3(grid_for((size_t)n * kWave)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr,
-                       v_ent.ptr, cev(), ceg);
+    CS_HIP(hipMemsetAsync(ceg, 0, (size_t)n * 3 * sizeof(double), st));
+    if (E) {
+      hipLaunchKernelGGL(k_grad_terms, dim3(grid_for(E)), dim3(kBlock), 0, st, E, act.ptr, v_ent.ptr, cev(),
+                         hterm.ptr);
+      hipLaunchKernelGGL(k_grad_vsum, dim3(grid_for((size_t)nv * 3)), dim3(kBlock), 0, st, nv, cverts.ptr, v_off.ptr,
+                         hterm.ptr, ceg);
+    }
     CS_LAUNCH_CHECK();
     if (C) {
       sync_cs();
@@ -1610,8 +1724,12 @@ struct ContactImpl {
 
   void add_diag_estimate(double* dg) {
     if (!C) return;
-    hipLaunchKernelGGL(k_contact_diag, dim3(grid_for((size_t)n * kWave)), dim3(kBlock), 0, st, n, act.ptr, v_off.ptr,
-                       v_ent.ptr, cev(), dg);
+    if (E) {
+      hipLaunchKernelGGL(k_diag_terms, dim3(grid_for(E)), dim3(kBlock), 0, st, E, act.ptr, v_ent.ptr, cev(),
+                         hterm.ptr);
+      hipLaunchKernelGGL(k_diag_vsum, dim3(grid_for((size_t)nv * 3)), dim3(kBlock), 0, st, nv, cverts.ptr, v_off.ptr,
+                         hterm.ptr, dg);
+    }
     CS_LAUNCH_CHECK();
   }
 
@@ -1677,8 +1795,12 @@ struct ContactImpl {
     update_active_set(x);
     if (!C) return;
     CS_HIP(hipMemcpyAsync(heads.ptr, v_off.ptr, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-    CS_HIP(hipMemsetAsync(kstate.ptr, 0, C, st));
-    hipLaunchKernelGGL(k_kill_approach, dim3(1), dim3(1024), 0, st, act.ptr, C, v_ent.ptr, heads.ptr, kstate.ptr, v);
+    CS_HIP(hipMemsetAsync(kflag.ptr, 0, (size_t)C * sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_kill_approach, dim3(1), dim3(1024), 0, st, act.ptr, C, v_off.ptr, v_ent.ptr, heads.ptr,
+                       kflag.ptr, klistA.ptr, klistB.ptr, v, &cs.ptr->kill_rounds);
+    CS_LAUNCH_CHECK();
+    sync_cs();
+    diag.filter_rounds = hcs->kill_rounds;
     CS_LAUNCH_CHECK();
   }
 
@@ -1741,6 +1863,7 @@ uint64_t ContactSystem::active_count() const { return im_->C; }
 
 uint64_t ContactSystem::candidate_count() const { return im_->diag.candidate_pairs; }
 uint64_t ContactSystem::incidence_count() const { return im_->E; }
+uint64_t ContactSystem::incidence_vertices() const { return im_->nv; }
 
 uint64_t ContactSystem::copy_active(int32_t* ids6, double* vals7, uint64_t cap) {
   const uint64_t C = im_->C;
@@ -1827,7 +1950,11 @@ DContactHvp ContactSystem::hvp_hook() {
   DContactHvp h;
   h.C = im_->C;
   h.E = im_->E;
+  h.nv = im_->nv;
+  h.ent_vtx = im_->ent_vtx.ptr;
   h.term = im_->hterm.ptr;
+  h.cverts = im_->cverts.ptr;
+  h.vsum = im_->hvsum.ptr;
   h.act = im_->act.ptr;
   h.inr = im_->inr.ptr;
   h.coeff = im_->coeff.ptr;

@@ -6,6 +6,8 @@
 //   inv_dm         f32 [4m] [a,b,c,d] (the reference stores Float32)
 //   area           f32 [m];  mass f32 [n]
 //   vt_off/vt_rec  CSR vertex -> (triangle << 2 | corner), ascending triangle
+//   vt_corner      the same CSR with each triangle's three corner ids inlined
+//                  (uint4), so an HVP gather needs no second indirection
 //   hv, h*         hinge corners u32 [4h] + rest/len/areaSum doubles
 //   vh_off/vh_rec  CSR vertex -> (hinge << 2 | corner), ascending hinge
 // Every per-vertex sum walks its CSR list in element order, which reproduces
@@ -25,6 +27,7 @@ struct DMesh {
   const float* mass;
   const uint32_t* vt_off;
   const uint32_t* vt_rec;
+  const uint4* vt_corner;   // per CSR entry: the triangle's corners i0, i1, i2 and its record
   const uint32_t* hv;
   const double* hrest;
   const double* hlen;
@@ -47,6 +50,7 @@ struct DMesh32 {
   const float* mass;
   const uint32_t* vt_off;
   const uint32_t* vt_rec;
+  const uint4* vt_corner;
   const uint8_t* pinmask;
 };
 

@@ -23,7 +23,7 @@ struct MxScalars {
   int32_t done, iters;
 };
 
-enum { MX_RZ = 0, MX_BB = 1, MX_RR = 2, MX_PAP = 3, MX_NPART = 4 };
+enum { MX_RZ = 0, MX_BB = 1, MX_RR = 2, MX_PAP = 3, MX_NPART = 3 + kMaxHvpBlocks / kMaxRedBlocks };
 
 struct MxArgs {
   DMesh32 me;
@@ -44,7 +44,8 @@ struct MxArgs {
   const Frozen32* act;
   const uint32_t* v_off;
   const uint32_t* v_ent;
-  float* term;  // [3E] per-entry terms of the current p
+  const uint32_t* ent_vtx;  // [E] vertex of each incidence entry
+  float* scan;              // [3E] per-vertex running sums of the entry terms within each 64-entry chunk
 };
 
 __device__ __forceinline__ float* mpart(const MxArgs& a, int which) {
@@ -88,8 +89,9 @@ __device__ __forceinline__ void ld3f(const float* x, uint32_t v, float* o) {
 }
 
 // Analytic membrane HVP corner contribution (membrane-hvp.ts:18-75) in FP32.
-__device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint32_t t, int corner, const float* p, float* out) {
-  const uint32_t i0 = a.me.idx[(size_t)t * 3], i1 = a.me.idx[(size_t)t * 3 + 1], i2 = a.me.idx[(size_t)t * 3 + 2];
+__device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const float* p, float* out) {
+  const uint32_t i0 = cr.x, i1 = cr.y, i2 = cr.z, t = cr.w >> 2;
+  const int corner = cr.w & 3;
   float x0[3], x1[3], x2[3], v0[3], v1[3], v2[3];
   ld3f(a.x, i0, x0); ld3f(a.x, i1, x1); ld3f(a.x, i2, x2);
   ld3f(p, i0, v0); ld3f(p, i1, v1); ld3f(p, i2, v2);
@@ -247,16 +249,29 @@ __global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) 
   }
 }
 
-// One thread per incidence entry: its contact term for the current p.
-__global__ void __launch_bounds__(kBlock) k_mx_terms(MxArgs a) {
+// Contact HVP terms of all incidence entries (one thread each) and, within
+// each wave's 64 entries, their running sums per vertex (entries are sorted
+// by vertex): the sum of a vertex's entries is then the last running sum in
+// each 64-entry chunk its range touches.
+__global__ void __launch_bounds__(kBlock) k_mx_cscan(MxArgs a) {
   if (block_latched(&a.s->done)) return;
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.E) return;
-  float t[3];
-  contact_term32(a, a.v_ent[e], a.p, t);
-  a.term[(size_t)e * 3] = t[0];
-  a.term[(size_t)e * 3 + 1] = t[1];
-  a.term[(size_t)e * 3 + 2] = t[2];
+  const int lane = threadIdx.x % kWave;
+  float t[3] = {0.f, 0.f, 0.f};
+  uint32_t u = 0xffffffffu;
+  if (e < a.E) {
+    contact_term32(a, a.v_ent[e], a.p, t);
+    u = a.ent_vtx[e];
+  }
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t uu = __shfl_up(u, d, kWave);
+    float y[3];
+    for (int i = 0; i < 3; i++) y[i] = __shfl_up(t[i], d, kWave);
+    if (lane >= d && uu == u)
+      for (int i = 0; i < 3; i++) t[i] += y[i];
+  }
+  if (e < a.E)
+    for (int i = 0; i < 3; i++) a.scan[(size_t)e * 3 + i] = t[i];
 }
 
 constexpr int kMxLanes = 8;  // lanes per vertex in the HVP gather
@@ -277,18 +292,19 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
     if (valid) {
       const uint32_t e = a.me.vt_off[v + 1];
       for (uint32_t q = a.me.vt_off[v] + sub; q < e; q += kMxLanes) {
-        const uint32_t rec = a.me.vt_rec[q];
         float gc[3];
-        hvp_corner32(a, rec >> 2, rec & 3, a.p, gc);
+        hvp_corner32(a, a.me.vt_corner[q], a.p, gc);
         acc[0] += gc[0]; acc[1] += gc[1]; acc[2] += gc[2];
       }
-      if (a.act) {
-        const uint32_t ce = a.v_off[v + 1];
-        for (uint32_t q = a.v_off[v] + sub; q < ce; q += kMxLanes) {
-          acc[0] += a.term[(size_t)q * 3];
-          acc[1] += a.term[(size_t)q * 3 + 1];
-          acc[2] += a.term[(size_t)q * 3 + 2];
-        }
+      if (a.scan) {
+        const uint32_t cb = a.v_off[v], ce = a.v_off[v + 1];
+        if (cb < ce)
+          for (uint32_t k = cb / kWave + sub; k <= (ce - 1) / kWave; k += kMxLanes) {
+            const size_t last = (size_t)min(ce, (k + 1) * (uint32_t)kWave) - 1;
+            acc[0] += a.scan[last * 3];
+            acc[1] += a.scan[last * 3 + 1];
+            acc[2] += a.scan[last * 3 + 2];
+          }
       }
     }
 #pragma unroll
@@ -386,7 +402,7 @@ __global__ void k_mx_out(const float* sol, double* dx, size_t n3) {
 struct MixedImpl {
   hipStream_t st;
   uint32_t n;
-  DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, term;
+  DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, scan;
   DevBuf<Frozen32> act32;
   DevBuf<MxScalars> s;
   MxArgs last{};
@@ -395,7 +411,7 @@ struct MixedImpl {
 
 static unsigned mx_grid(uint32_t n) { return red_grid(n); }
 static unsigned mx_hvp_grid(uint32_t n) {
-  return (unsigned)std::min<size_t>(kMaxRedBlocks, ((size_t)n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
+  return (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
 }
 
 MixedSolve::MixedSolve(const cs_scene_desc& d, const cs_options&, hipStream_t st) : im_(new MixedImpl) {
@@ -419,7 +435,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   const uint32_t E = C ? in.contact.E : 0;
   if (C) {
     M.act32.alloc(C);
-    M.term.alloc((size_t)std::max<uint32_t>(E, 1) * 3);
+    M.scan.alloc((size_t)std::max<uint32_t>(E, 1) * 3);
     hipLaunchKernelGGL(k_mx_contacts, dim3(grid_for(C)), dim3(kBlock), 0, st, in.contact.act, in.contact.inr,
                        in.contact.coeff, C, M.act32.ptr);
   }
@@ -428,7 +444,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   CS_LAUNCH_CHECK();
   MxArgs a;
   a.me = DMesh32{in.me.n, in.me.m, in.me.idx, in.me.inv_dm, in.me.area, in.me.mass, in.me.vt_off, in.me.vt_rec,
-                 in.me.pinmask};
+                 in.me.vt_corner, in.me.pinmask};
   a.c00 = (float)in.mat.c00; a.c11 = (float)in.mat.c11; a.c01 = (float)in.mat.c01; a.g = (float)in.mat.g;
   a.thick = (float)in.mat.thick;
   a.inv_h2 = (float)in.inv_h2;
@@ -439,7 +455,8 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.partials = M.partials.ptr;
   a.C = C;
   a.E = E;
-  a.term = C ? M.term.ptr : nullptr;
+  a.ent_vtx = in.contact.ent_vtx;
+  a.scan = E ? M.scan.ptr : nullptr;
   a.act = C ? M.act32.ptr : nullptr;
   a.v_off = in.contact.v_off;
   a.v_ent = in.contact.v_ent;
@@ -451,7 +468,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   CS_LAUNCH_CHECK();
   const int chunk = 8;
   for (int k = 0; k < in.max_iters; k++) {
-    if (E) hipLaunchKernelGGL(k_mx_terms, dim3(grid_for(E)), b, 0, st, a);
+    if (E) hipLaunchKernelGGL(k_mx_cscan, dim3(grid_for(E)), b, 0, st, a);
     if (timer) timer(ctx, k, true);
     hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, k);
     if (timer) timer(ctx, k, false);

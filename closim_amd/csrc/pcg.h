@@ -17,13 +17,17 @@ struct Frozen {
 // active-list order.
 struct DContactHvp {
   uint32_t C = 0;
-  uint32_t E = 0;                   // incidence entries
+  uint32_t E = 0;                    // incidence entries (sorted by vertex, then list order)
+  uint32_t nv = 0;                   // vertices with incidence entries
   const Frozen* act = nullptr;
-  const uint8_t* inr = nullptr;     // 1 when the contact is inside dHat
-  const double* coeff = nullptr;    // kappa * max(b''(d), 0), 0 when d >= dHat
-  const uint32_t* v_off = nullptr;  // n+1
-  const uint32_t* v_ent = nullptr;  // contact << 3 | slot
-  double* term = nullptr;           // [3E] per-entry HVP terms of the current p
+  const uint8_t* inr = nullptr;      // 1 when the contact is inside dHat
+  const double* coeff = nullptr;     // kappa * max(b''(d), 0), 0 when d >= dHat
+  const uint32_t* v_off = nullptr;   // n+1
+  const uint32_t* v_ent = nullptr;   // [E] contact << 3 | slot
+  const uint32_t* ent_vtx = nullptr; // [E] vertex of each entry
+  const uint32_t* cverts = nullptr;  // [nv] the vertices with entries
+  double* term = nullptr;            // [3E] per-entry terms of the current p
+  double* vsum = nullptr;            // [3n] contact HVP per vertex (0 off cverts)
 };
 
 // Device scalars of one PCG solve (written by block 0 of the kernel that

@@ -56,12 +56,12 @@ __device__ __forceinline__ double contact_dd(const DContactHvp& H, uint32_t ci, 
 
 // Contribution of one incident triangle corner to the membrane HVP at the
 // vertex: analytic, or the two gradient corners of the FD operator.
-__device__ __forceinline__ void hvp_corner(const PcgArgs& a, uint32_t rec, const double* __restrict__ p, double h,
+__device__ __forceinline__ void hvp_corner(const PcgArgs& a, uint4 cr, const double* __restrict__ p, double h,
                                            double* g0, double* g1) {
   const DMesh& me = a.me;
-  const uint32_t t = rec >> 2;
-  const int corner = rec & 3;
-  const uint32_t i0 = me.idx[(size_t)t * 3], i1 = me.idx[(size_t)t * 3 + 1], i2 = me.idx[(size_t)t * 3 + 2];
+  const uint32_t t = cr.w >> 2;
+  const int corner = cr.w & 3;
+  const uint32_t i0 = cr.x, i1 = cr.y, i2 = cr.z;
   double x0[3], x1[3], x2[3], p0[3], p1[3], p2[3];
   ld3(a.x, i0, x0); ld3(a.x, i1, x1); ld3(a.x, i2, x2);
   ld3(p, i0, p0); ld3(p, i1, p1); ld3(p, i2, p2);
@@ -129,7 +129,7 @@ __device__ __forceinline__ void hvp_vertex8(const PcgArgs& a, uint32_t v, bool v
   for (uint32_t r0 = b; r0 < e; r0 += kLanesPerVertex) {
     double g0[3] = {0, 0, 0}, g1[3] = {0, 0, 0};
     const uint32_t q = r0 + sub;
-    if (q < e) hvp_corner(a, me.vt_rec[q], p, h, g0, g1);
+    if (q < e) hvp_corner(a, me.vt_corner[q], p, h, g0, g1);
     const uint32_t cnt = min((uint32_t)kLanesPerVertex, e - r0);
     for (uint32_t j = 0; j < cnt; j++) {
 #pragma unroll
@@ -149,33 +149,20 @@ __device__ __forceinline__ void hvp_vertex8(const PcgArgs& a, uint32_t v, bool v
   const double m = valid ? (double)me.mass[v] : 0.0;
 #pragma unroll
   for (int i = 0; i < 3; i++) out[i] = m * (valid ? p[(size_t)v * 3 + i] : 0.0) * a.inv_h2 + Hp[i];
-  if (a.contact.act) {
-    // applyHvp (contact-assembly.ts:371-392): out.fill(0) then addDistGrad per
-    // active contact in list order (terms of entries with dd == 0 are +0 and
-    // leave the sum unchanged); then out += cHp.
-    double c[3] = {0, 0, 0};
-    const uint32_t cb = valid ? a.contact.v_off[v] : 0, ce = valid ? a.contact.v_off[v + 1] : 0;
-    for (uint32_t r0 = cb; r0 < ce; r0 += kLanesPerVertex) {
-      double t[3] = {0, 0, 0};
-      const uint32_t q = r0 + sub;
-      if (q < ce) {
-        t[0] = a.contact.term[(size_t)q * 3];
-        t[1] = a.contact.term[(size_t)q * 3 + 1];
-        t[2] = a.contact.term[(size_t)q * 3 + 2];
-      }
-      const uint32_t cnt = min((uint32_t)kLanesPerVertex, ce - r0);
-      for (uint32_t j = 0; j < cnt; j++) {
-        c[0] += __shfl(t[0], base + (int)j, kWave);
-        c[1] += __shfl(t[1], base + (int)j, kWave);
-        c[2] += __shfl(t[2], base + (int)j, kWave);
-      }
-    }
-    out[0] += c[0]; out[1] += c[1]; out[2] += c[2];
+  if (a.contact.act && valid) {
+    // applyHvp (contact-assembly.ts:371-392): cHp summed per vertex in list
+    // order by k_contact_vsum, then out += cHp
+    out[0] += a.contact.vsum[(size_t)v * 3];
+    out[1] += a.contact.vsum[(size_t)v * 3 + 1];
+    out[2] += a.contact.vsum[(size_t)v * 3 + 2];
   }
 }
 
-// Per incidence entry: the frozen barrier HVP term of its contact at its
-// stencil slot for the current p (one thread per entry).
+// applyHvp's contact part (out.fill(0), then addDistGrad per active contact
+// in list order, skipping dd == 0) in two passes: every incidence entry's
+// term in parallel (a skipped entry stores +0, which leaves a sum that
+// started at +0 unchanged), then one thread per contact vertex and component
+// adds its entries in list order.
 __global__ void __launch_bounds__(kBlock) k_contact_terms(DContactHvp H, const double* __restrict__ p,
                                                           const int32_t* done) {
   if (block_latched(done)) return;
@@ -186,6 +173,15 @@ __global__ void __launch_bounds__(kBlock) k_contact_terms(DContactHvp H, const d
   H.term[(size_t)e * 3] = t[0];
   H.term[(size_t)e * 3 + 1] = t[1];
   H.term[(size_t)e * 3 + 2] = t[2];
+}
+
+__global__ void __launch_bounds__(kBlock) k_contact_vsum(DContactHvp H, const int32_t* done) {
+  if (block_latched(done)) return;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= H.nv * 3) return;
+  const uint32_t v = H.cverts[t / 3];
+  const int q = t % 3;
+  H.vsum[(size_t)v * 3 + q] = ordered_sum<3, 0>(H.term + q, H.v_off[v], H.v_off[v + 1], 0.0);
 }
 
 // z = M^-1 r at vertex v (Jacobi with the diag > 1e-12 guard, or 3x3 block).
@@ -357,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a, int k) {
 
 unsigned pcg_grid(const PcgArgs& a) { return red_grid(a.me.n); }
 unsigned pcg_hvp_grid(const PcgArgs& a) {
-  return (unsigned)std::min<size_t>(kMaxRedBlocks, ((size_t)a.me.n + kVertsPerBlock - 1) / kVertsPerBlock);
+  return (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)a.me.n + kVertsPerBlock - 1) / kVertsPerBlock);
 }
 
 void pcg_init(hipStream_t st, const PcgArgs& a) {
@@ -398,9 +394,11 @@ void pcg_iteration(hipStream_t st, const PcgArgs& a, int k) {
 
 // The HVP grid differs from the vector grid; its partial count is passed on.
 void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k) {
-  if (a.contact.act && a.contact.E)
-    hipLaunchKernelGGL(k_contact_terms, dim3(grid_for(a.contact.E)), dim3(kBlock), 0, st, a.contact, a.p,
+  if (a.contact.act && a.contact.nv) {
+    hipLaunchKernelGGL(k_contact_terms, dim3(grid_for(a.contact.E)), dim3(kBlock), 0, st, a.contact, a.p, &a.s->done);
+    hipLaunchKernelGGL(k_contact_vsum, dim3(grid_for((size_t)a.contact.nv * 3)), dim3(kBlock), 0, st, a.contact,
                        &a.s->done);
+  }
   hipLaunchKernelGGL(k_pcg_hvp, dim3(pcg_hvp_grid(a)), dim3(kBlock), 0, st, a, k, pcg_grid(a));
   CS_LAUNCH_CHECK();
 }

