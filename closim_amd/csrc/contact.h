@@ -21,6 +21,10 @@ class ContactSystem {
   ~ContactSystem();
   void begin_step(const double* x0);                       // beginStep :75-81
   bool check_trial(const double* xt);                      // checkTrial :489-553
+  // Declare the segment (xa -> xb) on which the following check_trial /
+  // update_active_set positions lie until the next call or begin_step;
+  // their candidate pairs are then selected from one superset traversal.
+  void set_segment(const double* xa, const double* xb);
   void update_active_set(const double* x);                 // updateActiveSet :164-220
   void energy_grad(const double* x, double* ceg);          // energyGrad :262-285
   void add_diag_estimate(double* diag, const double* x);   // addDiagEstimate :436-471

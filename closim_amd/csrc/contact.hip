@@ -199,16 +199,18 @@ struct PairStage {
 };
 
 // sweptTriAabb (aabb.ts:9-31): min / max over the three vertices at x0 and
-// x1 in the reference's visiting order, then -/+ pad.
+// x1 in the reference's visiting order, then -/+ pad.  With x2 the box also
+// covers a third configuration (segment supersets, never a reference box).
 __device__ __forceinline__ void swept_box(const DGeom& G, const double* x0, const double* x1, uint32_t t, double pad,
-                                          double* lo, double* hi, int32_t* v) {
+                                          double* lo, double* hi, int32_t* v, const double* x2 = nullptr) {
   double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
   double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+  const int nq = x2 ? 3 : 2;
   for (int c = 0; c < 3; c++) {
     const uint32_t vi = G.idx[(size_t)t * 3 + c];
     v[c] = (int32_t)vi;
-    const double* xs[2] = {x0, x1};
-    for (int q = 0; q < 2; q++) {
+    const double* xs[3] = {x0, x1, x2};
+    for (int q = 0; q < nq; q++) {
       for (int k = 0; k < 3; k++) {
         const double pk = xs[q][(size_t)vi * 3 + k];
         if (pk < mn[k]) mn[k] = pk;
@@ -296,9 +298,9 @@ __device__ __forceinline__ void store_box(double* dst, const double* lo, const d
 // wave's range: writes their records and the level-1 box; levels 2 and 3
 // come from xor-shuffles over 8 and 64 lanes.
 __global__ void __launch_bounds__(kBlock) k_refit8(DGeom G, const double* __restrict__ x0,
-                                                   const double* __restrict__ x1, double pad,
-                                                   const uint32_t* __restrict__ order, Leaf* leaves, double* nodes,
-                                                   Levels LV) {
+                                                   const double* __restrict__ x1, const double* __restrict__ x2,
+                                                   double pad, const uint32_t* __restrict__ order, Leaf* leaves,
+                                                   double* nodes, Levels LV) {
   const uint32_t lane = threadIdx.x % kWave;
   const uint32_t wbase = (blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * (kWave * kFan);
   double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
@@ -308,7 +310,7 @@ __global__ void __launch_bounds__(kBlock) k_refit8(DGeom G, const double* __rest
     if (j >= G.m) break;
     Leaf R;
     const uint32_t t = order[j];
-    swept_box(G, x0, x1, t, pad, R.lo, R.hi, R.v);
+    swept_box(G, x0, x1, t, pad, R.lo, R.hi, R.v, x2);
     R.tri = t;
     leaves[j] = R;
     for (int k = 0; k < 3; k++) {
@@ -518,6 +520,58 @@ __global__ void __launch_bounds__(kBlock) k_traverse_static(const Leaf* __restri
   if (overflow && (threadIdx.x % kWave) == 0) atomicOr(&cs->nonfinite, 2u);
 }
 
+// Candidate pairs of one build from the superset of its segment: a trial
+// or iterate x on the segment (x_a, x_b) has every swept box (x_step -> x)
+// inside the segment box (x_step, x_a, x_b), so the superset holds every
+// pair of the build, and the reference's own predicate (exact swept boxes,
+// inclusive overlap; shared-vertex exclusion holds already) selects them.
+__global__ void __launch_bounds__(kBlock) k_filter_pairs(DGeom G, const double* __restrict__ xs,
+                                                         const double* __restrict__ x, double pad,
+                                                         const uint2* __restrict__ sup, uint32_t nsup, uint2* out,
+                                                         uint32_t cap, uint32_t* counter) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nsup; base += gridDim.x * blockDim.x) {
+    const uint32_t k = base + threadIdx.x;
+    bool keep = false;
+    uint2 pr = make_uint2(0, 0);
+    if (k < nsup) {
+      pr = sup[k];
+      double alo[3], ahi[3], blo[3], bhi[3];
+      int32_t av[3], bv[3];
+      swept_box(G, xs, x, pr.x, pad, alo, ahi, av);
+      swept_box(G, xs, x, pr.y, pad, blo, bhi, bv);
+      const double A[6] = {alo[0], alo[1], alo[2], ahi[0], ahi[1], ahi[2]};
+      keep = box_overlap6(A, blo, bhi);
+    }
+    const uint32_t slot = wave_alloc(keep, counter);
+    if (keep && slot < cap) out[slot] = pr;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_filter_spairs(DGeom G, const double* __restrict__ xs,
+                                                          const double* __restrict__ x, double pad,
+                                                          const double* __restrict__ sbox,
+                                                          const uint2* __restrict__ sup, uint32_t nsup, uint2* out,
+                                                          uint32_t cap, uint32_t* counter) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nsup; base += gridDim.x * blockDim.x) {
+    const uint32_t k = base + threadIdx.x;
+    bool keep = false;
+    uint2 pr = make_uint2(0, 0);
+    if (k < nsup) {
+      pr = sup[k];
+      double alo[3], ahi[3];
+      int32_t av[3];
+      swept_box(G, xs, x, pr.x, pad, alo, ahi, av);
+      const double A[6] = {alo[0], alo[1], alo[2], ahi[0], ahi[1], ahi[2]};
+      const double* b = sbox + (size_t)pr.y * 6;
+      const double slo[3] = {b[0] - pad, b[1] - pad, b[2] - pad};
+      const double shi[3] = {b[3] + pad, b[4] + pad, b[5] + pad};
+      keep = box_overlap6(A, slo, shi);
+    }
+    const uint32_t slot = wave_alloc(keep, counter);
+    if (keep && slot < cap) out[slot] = pr;
+  }
+}
+
 // ------------------------------------------------------------- prims
 
 // One candidate primitive (expandTriPair self-collision.ts:23-41 for cloth
@@ -532,13 +586,13 @@ __device__ __forceinline__ uint64_t rkey(const DGeom& G, uint64_t kind, uint64_t
 }
 
 __device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint32_t np, const uint2* spairs, uint32_t nsp,
-                                        uint64_t tid, Prim& q) {
+                                        uint32_t tid, Prim& q) {
   const int TE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
   // prim ranges: cloth VT (6 per pair), cloth EE (9 per pair), static VT (3),
   // static EE (9), so that a wave mostly runs one closest-point routine
-  if (tid < (uint64_t)np * 15) {
-    const bool vt = tid < (uint64_t)np * 6;
-    const uint64_t t1 = vt ? tid : tid - (uint64_t)np * 6;
+  if (tid < np * 15) {
+    const bool vt = tid < np * 6;
+    const uint32_t t1 = vt ? tid : tid - np * 6;
     const uint2 pr = pairs[vt ? t1 / 6 : t1 / 9];
     const int k = vt ? (int)(t1 % 6) : 6 + (int)(t1 % 9);
     const uint32_t tA = pr.x, tB = pr.y;
@@ -560,9 +614,9 @@ __device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint
       q.r2 = de2;
     }
   } else {
-    const uint64_t t2 = tid - (uint64_t)np * 15;
-    const bool vt = t2 < (uint64_t)nsp * 3;
-    const uint64_t t3 = vt ? t2 : t2 - (uint64_t)nsp * 3;
+    const uint32_t t2 = tid - np * 15;
+    const bool vt = t2 < nsp * 3;
+    const uint32_t t3 = vt ? t2 : t2 - nsp * 3;
     const uint2 pr = spairs[vt ? t3 / 3 : t3 / 9];
     const int k = vt ? (int)(t3 % 3) : 3 + (int)(t3 % 9);
     const uint32_t ct = pr.x, st = pr.y;
@@ -612,12 +666,12 @@ __global__ void __launch_bounds__(kBlock) k_prims_active(DGeom G, const double* 
                                                          const uint2* __restrict__ spairs, uint32_t nsp,
                                                          Frozen* rec, uint64_t* keys, uint64_t* nkeys, uint32_t cap,
                                                          double* partials, CScalars* cs) {
-  const uint64_t total = (uint64_t)np * 15 + (uint64_t)nsp * 12;
+  const uint32_t total = np * 15 + nsp * 12;  // < 2^31, checked by the host
   double minD = __builtin_inf();
   // uniform trip count per block so that the wave-aggregated emission sees
   // every lane
-  for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x; base < total; base += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t tid = base + threadIdx.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t tid = base + threadIdx.x;
     bool emit = false;
     Prim q;
     Frozen f;
@@ -1207,10 +1261,9 @@ __global__ void __launch_bounds__(kBlock) k_prims_trial(DGeom G, const double* _
                                                         const uint2* __restrict__ pairs, uint32_t np,
                                                         const uint2* __restrict__ spairs, uint32_t nsp,
                                                         double* partials, CScalars* cs) {
-  const uint64_t total = (uint64_t)np * 15 + (uint64_t)nsp * 12;
+  const uint32_t total = np * 15 + nsp * 12;  // < 2^31, checked by the host
   double minD = __builtin_inf(), minT = __builtin_inf();
-  for (uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; tid < total;
-       tid += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < total; tid += gridDim.x * blockDim.x) {
     Prim q;
     prim_of(G, pairs, np, spairs, nsp, tid, q);
     const int ids[4] = {q.kind == 0 ? q.p : q.a, q.kind == 0 ? q.a : q.b, q.kind == 0 ? q.b : q.c,
@@ -1307,6 +1360,11 @@ struct ContactImpl {
   DevBuf<uint32_t> mvals, mvals_t;
   const uint32_t* order = nullptr;
   bool need_sort = true;
+  DevBuf<double> sbox;  // unpadded static triangle boxes by triangle id
+  // candidate superset of the current segment (see set_segment)
+  DevBuf<uint2> pairs_sup, spairs_sup;
+  uint32_t sup_cap = 0, ssup_cap = 0, np_sup = 0, nsp_sup = 0;
+  bool sup_valid = false;
   // candidates
   DevBuf<uint2> pairs, spairs;
   uint32_t pair_cap = 0, spair_cap = 0, np = 0, nsp = 0;
@@ -1461,6 +1519,10 @@ struct ContactImpl {
     lag_count.alloc(1);
     ensure_sort(m);
     grow_pairs(std::max<uint32_t>(1024, m * 4), std::max<uint32_t>(256, ms * 8));
+    sup_cap = std::max<uint32_t>(1024, m * 8);
+    ssup_cap = std::max<uint32_t>(256, ms * 16);
+    pairs_sup.alloc(sup_cap);
+    spairs_sup.alloc(ssup_cap);
     grow_active(std::max<uint32_t>(4096, n));
     CS_HIP(hipStreamSynchronize(st));
   }
@@ -1557,6 +1619,15 @@ struct ContactImpl {
       }
     if (ms) sleaves.upload(sorted.data(), ms, st);
     else sleaves.alloc(1);
+    {
+      std::vector<double> sb(std::max<size_t>((size_t)ms * 6, 6));
+      for (uint32_t t = 0; t < ms; t++)
+        for (int k = 0; k < 3; k++) {
+          sb[(size_t)t * 6 + k] = sl[t].lo[k];
+          sb[(size_t)t * 6 + 3 + k] = sl[t].hi[k];
+        }
+      sbox.upload(sb.data(), sb.size(), st);
+    }
     snodes.upload(nb.data(), nb.size(), st);
   }
 
@@ -1573,6 +1644,50 @@ struct ContactImpl {
     need_sort = false;
   }
 
+  // LBVH refit over (x_step, xa[, xb]) and both traversals into the given
+  // pair buffers (grown and re-run on overflow).
+  void traverse(const double* xa, const double* xb, double pad, DevBuf<uint2>& P, uint32_t& pcap,
+                DevBuf<uint2>& SP, uint32_t& spcap, uint32_t& np_out, uint32_t& nsp_out) {
+    const DGeom G = geom();
+    const unsigned waves8 = (unsigned)((m + kWave * kFan - 1) / (kWave * kFan));
+    hipLaunchKernelGGL(k_refit8, dim3((waves8 + 3) / 4), dim3(kBlock), 0, st, G, xstep, xa, xb, pad, order,
+                       leaves.ptr, nodes.ptr, LV);
+    if (LV.L > 3) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, nodes.ptr, LV);
+    CS_LAUNCH_CHECK();
+    for (int attempt = 0; attempt < 3; attempt++) {
+      CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 2 * sizeof(uint32_t), st));
+      if (m > 1)
+        hipLaunchKernelGGL(k_traverse_self, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, nodes.ptr, LV, P.ptr,
+                           pcap, cs.ptr);
+      if (ms)
+        hipLaunchKernelGGL(k_traverse_static, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, m, sleaves.ptr,
+                           snodes.ptr, SL, pad, SP.ptr, spcap, cs.ptr);
+      CS_LAUNCH_CHECK();
+      sync_cs();
+      if (hcs->nonfinite & 2u) throw OverflowError("LBVH traversal stack overflow");
+      np_out = hcs->pair_count;
+      nsp_out = hcs->spair_count;
+      if (np_out <= pcap && nsp_out <= spcap) return;
+      if (np_out > pcap) { pcap = (uint32_t)(np_out * 1.25) + 1024; P.alloc(pcap); }
+      if (nsp_out > spcap) { spcap = (uint32_t)(nsp_out * 1.25) + 256; SP.alloc(spcap); }
+    }
+    throw OverflowError("candidate pair buffer could not be grown");
+  }
+
+  // The segment (xa -> xb) that the next builds lie on: line-search trials
+  // x + alpha dx (alpha in (0, 1]) and the accepted iterate, or the predictor
+  // bisection probes.  Every such build's swept boxes lie inside the boxes
+  // over (x_step, xa, xb) padded by max(dHat, dMin), so one traversal gives a
+  // superset of all their candidate pairs.
+  void set_segment(const double* xa, const double* xb) {
+    sup_valid = false;
+    if (m == 0) return;
+    if (need_sort) sort_leaves(xb);
+    const double pad = std::max(prm.d_hat, prm.d_min);
+    traverse(xa, xb, pad, pairs_sup, sup_cap, spairs_sup, ssup_cap, np_sup, nsp_sup);
+    sup_valid = true;
+  }
+
   // buildCandidates (contact-assembly.ts:87-121) minus the expansion into
   // prims, which the prim kernels do on the fly.
   void build_candidates(const double* x, double pad) {
@@ -1585,30 +1700,25 @@ struct ContactImpl {
       diag.candidate_pairs = 0;
       return;
     }
-    if (need_sort) sort_leaves(x);
-    const unsigned waves8 = (unsigned)((m + kWave * kFan - 1) / (kWave * kFan));
-    hipLaunchKernelGGL(k_refit8, dim3((waves8 + 3) / 4), dim3(kBlock), 0, st, G, xstep, x, pad, order, leaves.ptr,
-                       nodes.ptr, LV);
-    if (LV.L > 3) hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, nodes.ptr, LV);
-    CS_LAUNCH_CHECK();
-    for (int attempt = 0; attempt < 3; attempt++) {
+    if (sup_valid) {
+      grow_pairs(std::max(pair_cap, np_sup), std::max(spair_cap, nsp_sup));
       CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 2 * sizeof(uint32_t), st));
-      if (m > 1)
-        hipLaunchKernelGGL(k_traverse_self, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, nodes.ptr, LV,
-                           pairs.ptr, pair_cap, cs.ptr);
-      if (ms)
-        hipLaunchKernelGGL(k_traverse_static, dim3(grid_for(m)), dim3(kBlock), 0, st, leaves.ptr, m, sleaves.ptr,
-                           snodes.ptr, SL, pad, spairs.ptr, spair_cap, cs.ptr);
+      if (np_sup)
+        hipLaunchKernelGGL(k_filter_pairs, dim3(red_grid(np_sup)), dim3(kBlock), 0, st, G, xstep, x, pad,
+                           pairs_sup.ptr, np_sup, pairs.ptr, pair_cap, &cs.ptr->pair_count);
+      if (nsp_sup)
+        hipLaunchKernelGGL(k_filter_spairs, dim3(red_grid(nsp_sup)), dim3(kBlock), 0, st, G, xstep, x, pad,
+                           sbox.ptr, spairs_sup.ptr, nsp_sup, spairs.ptr, spair_cap, &cs.ptr->spair_count);
       CS_LAUNCH_CHECK();
       sync_cs();
-      if (hcs->nonfinite & 2u) throw OverflowError("LBVH traversal stack overflow");
       np = hcs->pair_count;
       nsp = hcs->spair_count;
-      if (np <= pair_cap && nsp <= spair_cap) break;
-      grow_pairs(std::max(pair_cap, (uint32_t)(np * 1.25) + 1024), std::max(spair_cap, (uint32_t)(nsp * 1.25) + 256));
-      if (attempt == 2) throw OverflowError("candidate pair buffer could not be grown");
+    } else {
+      if (need_sort) sort_leaves(x);
+      traverse(x, nullptr, pad, pairs, pair_cap, spairs, spair_cap, np, nsp);
     }
     diag.candidate_pairs = (uint64_t)np * 15 + (uint64_t)nsp * 12;
+    if (diag.candidate_pairs >= (1ull << 31)) throw OverflowError("more than 2^31 candidate primitives");
   }
 
   void update_active_set(const double* x) {
@@ -1845,12 +1955,14 @@ ContactSystem::~ContactSystem() { delete im_; }
 void ContactSystem::begin_step(const double* x0) {
   im_->xstep = x0;
   im_->need_sort = true;
+  im_->sup_valid = false;
   im_->lag_L = 0;
   im_->lag_vbase = 0;
   im_->C = 0;
   im_->reset_diag();
 }
 bool ContactSystem::check_trial(const double* xt) { return im_->check_trial(xt); }
+void ContactSystem::set_segment(const double* xa, const double* xb) { im_->set_segment(xa, xb); }
 void ContactSystem::update_active_set(const double* x) { im_->update_active_set(x); }
 void ContactSystem::energy_grad(const double* x, double* ceg) { im_->energy_grad(x, ceg); }
 void ContactSystem::add_diag_estimate(double* diag, const double*) { im_->add_diag_estimate(diag); }

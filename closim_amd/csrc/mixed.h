@@ -20,6 +20,7 @@ struct MixedInputs {
   DContactHvp contact; // FP64 frozen contacts (act + coeff)
   int max_iters;
   double tol;
+  Schwarz* sw = nullptr;  // additive Schwarz with FP32 factors (else binv / diag)
 };
 
 struct MixedImpl;

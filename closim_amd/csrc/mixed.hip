@@ -7,6 +7,7 @@
 // deterministic run to run.
 #include "fem.h"
 #include "mixed.h"
+#include "schwarz.h"
 
 namespace cloth {
 
@@ -37,6 +38,7 @@ struct MxArgs {
   float* p;
   float* Ap;
   float tol;
+  int ext;  // z comes from the Schwarz apply between the kernels
   MxScalars* s;
   float* partials;  // MX_NPART * kMaxRedBlocks
   // contacts
@@ -221,19 +223,40 @@ __global__ void __launch_bounds__(kBlock) k_mx_init(MxArgs a) {
     const bool pinned = a.me.pinmask[v] != 0;
     float r3[3], z3[3];
     ld3f(a.r, v, r3);
-    precond32(a, v, r3, z3);
+    if (!a.ext) {
+      precond32(a, v, r3, z3);
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const size_t j = (size_t)v * 3 + i;
+        if (pinned) z3[i] = 0.f;
+        a.z[j] = z3[i];
+        a.p[j] = z3[i];
+        rz += r3[i] * z3[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) bb += r3[i] * r3[i];
+  }
+  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ));
+  block_partial_f(bb, mpart(a, MX_BB));
+}
+
+// z from the Schwarz apply: pin filter, p = z (init), r.z partials.
+__global__ void __launch_bounds__(kBlock) k_mx_ext_z(MxArgs a, int init) {
+  if (!init && block_latched(&a.s->done)) return;
+  float rz = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+    const bool pinned = a.me.pinmask[v] != 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       const size_t j = (size_t)v * 3 + i;
-      if (pinned) z3[i] = 0.f;
-      a.z[j] = z3[i];
-      a.p[j] = z3[i];
-      rz += r3[i] * z3[i];
-      bb += r3[i] * r3[i];
+      const float zi = pinned ? 0.f : a.z[j];
+      a.z[j] = zi;
+      if (init) a.p[j] = zi;
+      rz += a.r[j] * zi;
     }
   }
   block_partial_f(rz, mpart(a, MX_RZ));
-  block_partial_f(bb, mpart(a, MX_BB));
 }
 
 __global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) {
@@ -354,16 +377,18 @@ __global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned 
       r3[i] = ri;
       rr += ri * ri;
     }
-    precond32(a, v, r3, z3);
+    if (!a.ext) {
+      precond32(a, v, r3, z3);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      if (pinned) z3[i] = 0.f;
-      a.z[(size_t)v * 3 + i] = z3[i];
-      rz += r3[i] * z3[i];
+      for (int i = 0; i < 3; i++) {
+        if (pinned) z3[i] = 0.f;
+        a.z[(size_t)v * 3 + i] = z3[i];
+        rz += r3[i] * z3[i];
+      }
     }
   }
   block_partial_f(rr, mpart(a, MX_RR));
-  block_partial_f(rz, mpart(a, MX_RZ));
+  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ));
 }
 
 __global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a, int k) {
@@ -451,6 +476,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.x = M.x32.ptr; a.diag = M.d32.ptr; a.binv = in.binv ? M.binv32.ptr : nullptr;
   a.sol = M.sol.ptr; a.r = M.r.ptr; a.z = M.z.ptr; a.p = M.p.ptr; a.Ap = M.Ap.ptr;
   a.tol = (float)in.tol;
+  a.ext = in.sw ? 1 : 0;
   a.s = M.s.ptr;
   a.partials = M.partials.ptr;
   a.C = C;
@@ -464,6 +490,10 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   M.has_last = true;
   const dim3 g(mx_grid(n)), gh(mx_hvp_grid(n)), b(kBlock);
   hipLaunchKernelGGL(k_mx_init, g, b, 0, st, a);
+  if (in.sw) {
+    in.sw->apply32(a.r, a.z, nullptr);
+    hipLaunchKernelGGL(k_mx_ext_z, g, b, 0, st, a, 1);
+  }
   hipLaunchKernelGGL(k_mx_decide, dim3(1), b, 0, st, a, mx_grid(n));
   CS_LAUNCH_CHECK();
   const int chunk = 8;
@@ -473,6 +503,10 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
     hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, k);
     if (timer) timer(ctx, k, false);
     hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k, mx_hvp_grid(n));
+    if (in.sw) {
+      in.sw->apply32(a.r, a.z, &M.s.ptr->done);
+      hipLaunchKernelGGL(k_mx_ext_z, g, b, 0, st, a, 0);
+    }
     hipLaunchKernelGGL(k_mx_dir, g, b, 0, st, a, k);
     CS_LAUNCH_CHECK();
     if ((k + 1) % chunk == 0 || k + 1 == in.max_iters) {

@@ -38,7 +38,9 @@ struct PcgScalars {
   int32_t done, iters;
 };
 
-enum PrecondKind { PRE_JACOBI = 0, PRE_BLOCK = 1 };
+enum PrecondKind { PRE_JACOBI = 0, PRE_BLOCK = 1, PRE_SCHWARZ = 2 };
+
+class Schwarz;
 
 struct PcgArgs {
   DMesh me;
@@ -57,8 +59,9 @@ struct PcgArgs {
   int precond;         // PrecondKind
   double tol;
   PcgScalars* s;
-  jsm::dd* partials;   // >= 5 * kMaxRedBlocks
+  jsm::dd* partials;   // >= 4 * kMaxRedBlocks + kMaxHvpBlocks (the p.Ap slots last)
   DContactHvp contact;
+  Schwarz* sw;         // PRE_SCHWARZ: applied between the kernels (host side only)
 };
 
 void pcg_init(hipStream_t st, const PcgArgs& a);

@@ -14,6 +14,7 @@
 // decisions; every block takes them from the same merged values.
 #include "elements.h"
 #include "pcg.h"
+#include "schwarz.h"
 
 namespace cloth {
 
@@ -214,23 +215,50 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(PcgArgs a) {
       a.sol[j] = 0.0;
       a.r[j] = r3[i];
     }
-    precond_vertex(a, v, r3, z3);
+    if (a.precond != PRE_SCHWARZ) {
+      precond_vertex(a, v, r3, z3);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const size_t j = (size_t)v * 3 + i;
-      if (pinned) z3[i] = 0.0;
-      a.z[j] = z3[i];
-      a.p[j] = z3[i];
-      jsm::dd_add_prod(rz, r3[i], z3[i]);
-      jsm::dd_add_prod(pp, z3[i], z3[i]);
+      for (int i = 0; i < 3; i++) {
+        const size_t j = (size_t)v * 3 + i;
+        if (pinned) z3[i] = 0.0;
+        a.z[j] = z3[i];
+        a.p[j] = z3[i];
+        jsm::dd_add_prod(rz, r3[i], z3[i]);
+        jsm::dd_add_prod(pp, z3[i], z3[i]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) jsm::dd_add_prod(rr, r3[i], r3[i]);
   }
   block_partial_dd<kBlock>(bb, part(a, PART_BB));
   block_partial_dd<kBlock>(rr, part(a, PART_RR));
+  if (a.precond != PRE_SCHWARZ) {
+    block_partial_dd<kBlock>(rz, part(a, PART_RZ));
+    block_partial_dd<kBlock>(pp, part(a, PART_PP));
+  }
+}
+
+// External preconditioner (Schwarz): z was written by the apply; the pin
+// filter on z (pcg.ts:46, :84), then p = z (init only), r.z and p.p partials.
+__global__ void __launch_bounds__(kBlock) k_pcg_ext_z(PcgArgs a, int init) {
+  if (!init && block_latched(&a.s->done)) return;
+  jsm::dd rz = jsm::dd_zero(), pp = jsm::dd_zero();
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+    const bool pinned = a.me.pinmask[v] != 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const size_t j = (size_t)v * 3 + i;
+      const double zi = pinned ? 0.0 : a.z[j];
+      a.z[j] = zi;
+      jsm::dd_add_prod(rz, a.r[j], zi);
+      if (init) {
+        a.p[j] = zi;
+        jsm::dd_add_prod(pp, zi, zi);
+      }
+    }
+  }
   block_partial_dd<kBlock>(rz, part(a, PART_RZ));
-  block_partial_dd<kBlock>(pp, part(a, PART_PP));
+  if (init) block_partial_dd<kBlock>(pp, part(a, PART_PP));
 }
 
 // After init, one block: tolerance, initial residual and rz, early return.
@@ -307,16 +335,18 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k, unsigne
       r3[i] = ri;
       jsm::dd_add_prod(rr, ri, ri);
     }
-    precond_vertex(a, v, r3, z3);
+    if (a.precond != PRE_SCHWARZ) {
+      precond_vertex(a, v, r3, z3);
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      if (pinned) z3[i] = 0.0;
-      a.z[(size_t)v * 3 + i] = z3[i];
-      jsm::dd_add_prod(rz, r3[i], z3[i]);
+      for (int i = 0; i < 3; i++) {
+        if (pinned) z3[i] = 0.0;
+        a.z[(size_t)v * 3 + i] = z3[i];
+        jsm::dd_add_prod(rz, r3[i], z3[i]);
+      }
     }
   }
   block_partial_dd<kBlock>(rr, part(a, PART_RR));
-  block_partial_dd<kBlock>(rz, part(a, PART_RZ));
+  if (a.precond != PRE_SCHWARZ) block_partial_dd<kBlock>(rz, part(a, PART_RZ));
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_dir(PcgArgs a, int k) {
@@ -359,6 +389,11 @@ unsigned pcg_hvp_grid(const PcgArgs& a) {
 void pcg_init(hipStream_t st, const PcgArgs& a) {
   hipLaunchKernelGGL(k_pcg_init, dim3(pcg_grid(a)), dim3(kBlock), 0, st, a);
   CS_LAUNCH_CHECK();
+  if (a.precond == PRE_SCHWARZ) {
+    a.sw->apply64(a.r, a.z, nullptr);
+    hipLaunchKernelGGL(k_pcg_ext_z, dim3(pcg_grid(a)), dim3(kBlock), 0, st, a, 1);
+    CS_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_pcg_init_decide, dim3(1), dim3(kBlock), 0, st, a, pcg_grid(a));
   CS_LAUNCH_CHECK();
 }
@@ -406,6 +441,10 @@ void pcg_iteration_hvp(hipStream_t st, const PcgArgs& a, int k) {
 void pcg_iteration_rest(hipStream_t st, const PcgArgs& a, int k) {
   const dim3 g(pcg_grid(a)), b(kBlock);
   hipLaunchKernelGGL(k_pcg_update, g, b, 0, st, a, k, pcg_hvp_grid(a));
+  if (a.precond == PRE_SCHWARZ) {
+    a.sw->apply64(a.r, a.z, &a.s->done);
+    hipLaunchKernelGGL(k_pcg_ext_z, g, b, 0, st, a, 0);
+  }
   hipLaunchKernelGGL(k_pcg_dir, g, b, 0, st, a, k);
   CS_LAUNCH_CHECK();
 }

@@ -16,6 +16,7 @@
 #include "fem.h"
 #include "mixed.h"
 #include "pcg.h"
+#include "schwarz.h"
 
 namespace cloth {
 
@@ -87,7 +88,7 @@ struct Solver {
   DevBuf<float> inv_dm, area, mass;
   DevBuf<double> hrest, hlen, hasum;
   // state
-  DevBuf<double> pos, vel, x0, yhat, x, xt, g, b, dx, diag, ceg;
+  DevBuf<double> pos, vel, x0, yhat, x, xt, xe, g, b, dx, diag, ceg;
   // pcg
   DevBuf<double> r, z, p, Ap, binv;
   DevBuf<jsm::dd> partials;
@@ -104,6 +105,7 @@ struct Solver {
   // subsystems
   std::unique_ptr<ContactSystem> contact;
   std::unique_ptr<MixedSolve> mixed;
+  std::unique_ptr<Schwarz> schwarz;
   // timing
   hipEvent_t ev_pool[2 * 4096];
   int ev_used = 0;
@@ -236,7 +238,7 @@ struct Solver {
     const size_t n3 = (size_t)n * 3;
     pos.upload(d->positions, n3, st);
     vel.upload(d->velocities, n3, st);
-    for (auto* bf : {&x0, &yhat, &x, &xt, &g, &b, &dx, &diag, &ceg, &r, &z, &p, &Ap}) bf->alloc(n3);
+    for (auto* bf : {&x0, &yhat, &x, &xt, &xe, &g, &b, &dx, &diag, &ceg, &r, &z, &p, &Ap}) bf->alloc(n3);
     partials.alloc(8 * kMaxRedBlocks);
     mpartials.alloc(4 * kMaxRedBlocks);
     scal.alloc(1);
@@ -252,6 +254,8 @@ struct Solver {
     has_contact = d->contact_enabled != 0;
     if (has_contact) contact.reset(new ContactSystem(*d, st));
     if (opt.precision == CS_PREC_MIXED) mixed.reset(new MixedSolve(*d, opt, st));
+    if (opt.precond == CS_PRECOND_SCHWARZ)
+      schwarz.reset(new Schwarz(*d, opt.schwarz_target, opt.schwarz_min, opt.schwarz_max, st));
     CS_HIP(hipStreamSynchronize(st));
   }
 
@@ -301,6 +305,7 @@ struct Solver {
     int lsTrials = 0;
     if (contact) {
       contact->begin_step(x0.ptr);
+      contact->set_segment(x0.ptr, x.ptr);  // predictor probes lie on x0 -> yhat
       if (!contact->check_trial(x.ptr)) {
         // predictor bisection (newton.ts:53-74)
         double lo = 0, hi = 1;
@@ -330,7 +335,9 @@ struct Solver {
       used++;
       const double* cg = nullptr;
       if (contact) {
-        contact->update_active_set(x.ptr);
+        // the first iteration's x is the one the active set was just built at
+        // (newton.ts:99 and :124 build it twice with the same inputs)
+        if (ni > 0) contact->update_active_set(x.ptr);
         contact->energy_grad(x.ptr, ceg.ptr);
         contact->add_diag_estimate(diag.ptr, x.ptr);
         cg = ceg.ptr;
@@ -371,6 +378,12 @@ struct Solver {
         const double s = TRUST / maxDx;
         launch_scale(st, dx.ptr, s, n3);
         gtdx *= s;
+      }
+      // every trial x + alpha dx (and the accepted iterate) lies on x -> x + dx
+      if (contact) {
+        launch_trial(st, x.ptr, dx.ptr, 1.0, xe.ptr, n3);
+        enforce_pins(xe.ptr);
+        contact->set_segment(x.ptr, xe.ptr);
       }
       // Armijo (newton.ts:184-219)
       double alpha = 1;
@@ -445,7 +458,10 @@ struct Solver {
     a.sol = dx.ptr;
     a.r = r.ptr; a.z = z.ptr; a.p = p.ptr; a.Ap = Ap.ptr;
     a.hvp_mode = opt.hvp_mode == CS_HVP_ANALYTIC ? HVP_ANALYTIC : HVP_FD;
-    a.precond = opt.precond == CS_PRECOND_BLOCK_JACOBI ? PRE_BLOCK : PRE_JACOBI;
+    a.precond = opt.precond == CS_PRECOND_BLOCK_JACOBI ? PRE_BLOCK
+                : opt.precond == CS_PRECOND_SCHWARZ       ? PRE_SCHWARZ
+                                                          : PRE_JACOBI;
+    a.sw = schwarz.get();
     a.tol = 1e-3;  // newton.ts:148
     a.s = &ds()->pcg;
     a.partials = partials.ptr;
@@ -467,6 +483,11 @@ struct Solver {
       launch_block_factors(st, in.me, in.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
       ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
       in.binv = binv.ptr;
+    } else if (schwarz) {
+      const int e0 = ev_mark();
+      schwarz->factor(in.me, in.mat, x.ptr, diag.ptr, true);
+      ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
+      in.sw = schwarz.get();
     }
     in.dx = dx.ptr;
     if (contact) in.contact = contact->hvp_hook();
@@ -509,6 +530,10 @@ struct Solver {
     if (a.precond == PRE_BLOCK) {
       const int e0 = ev_mark();
       launch_block_factors(st, a.me, a.mat, x.ptr, diag.ptr, binv.ptr, nullptr);
+      ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
+    } else if (a.precond == PRE_SCHWARZ) {
+      const int e0 = ev_mark();
+      schwarz->factor(a.me, a.mat, x.ptr, diag.ptr, false);
       ev_span(CS_KERNEL_PRECOND, e0, ev_mark());
     }
     pcg_init(st, a);
@@ -627,7 +652,10 @@ int cs_create(const cs_scene_desc* scene, const cs_options* opt, cs_solver** out
     return fail(CS_E_INVALID, "negative iteration count");
   if (opt->precision != CS_PREC_FP64 && opt->precision != CS_PREC_MIXED)
     return fail(CS_E_INVALID, "unknown precision");
-  if (opt->precond == CS_PRECOND_SCHWARZ) return fail(CS_E_UNSUPPORTED, "additive Schwarz is not built in this library");
+  if (opt->precond < CS_PRECOND_JACOBI || opt->precond > CS_PRECOND_SCHWARZ)
+    return fail(CS_E_INVALID, "unknown preconditioner");
+  if (opt->precond == CS_PRECOND_SCHWARZ && (opt->schwarz_target < 1 || opt->schwarz_min < 0))
+    return fail(CS_E_INVALID, "Schwarz domain sizes must be positive");
   *out = nullptr;
   auto* h = new cs_solver;
   const int rc = guarded([&]() -> int {
@@ -776,6 +804,21 @@ int cs_reset_timing(cs_solver* s) {
     s->s.kt_count[k] = 0;
   }
   return CS_OK;
+}
+
+int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
+                         uint32_t* local_of, uint32_t* domains) {
+  (void)max_size;
+  if (!scene || !scene->indices || (!scene->positions && !scene->rest_positions) || !domain_of || !local_of)
+    return fail(CS_E_INVALID, "null argument");
+  if (target < 1 || min_size < 0) return fail(CS_E_INVALID, "Schwarz domain sizes must be positive");
+  for (size_t k = 0; k < (size_t)scene->n_tris * 3; k++)
+    if (scene->indices[k] >= scene->n_verts) return fail(CS_E_INVALID, "triangle index out of range");
+  return guarded([&]() -> int {
+    const int D = cloth::schwarz_partition_host(*scene, target, min_size, domain_of, local_of);
+    if (domains) *domains = (uint32_t)D;
+    return CS_OK;
+  });
 }
 
 int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t* ids6, double* vals7,

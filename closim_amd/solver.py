@@ -56,6 +56,8 @@ def load_library(path=LIB_PATH):
         "cs_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_uint64), _dp]),
         "cs_reset_timing": (C.c_int, [C.c_void_p]),
         "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+        "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cs_probe_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, _dp]),
@@ -81,6 +83,20 @@ def device_info(device=0):
     buf = C.create_string_buffer(256)
     _check(load_library().cs_device_info(device, buf, 256))
     return buf.value.decode()
+
+
+def schwarz_partition(scene, target=16, min_size=8, max_size=32):
+    """buildSchwarzDomains of the library (host only): (domain_of, local_of, count)."""
+    desc, keep = to_desc(scene)
+    n = scene.mesh.count
+    dom = np.zeros(n, np.uint32)
+    loc = np.zeros(n, np.uint32)
+    cnt = C.c_uint32()
+    _check(load_library().cs_schwarz_partition(C.byref(desc), target, min_size, max_size,
+                                               dom.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                               loc.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(cnt)))
+    del keep
+    return dom, loc, cnt.value
 
 
 def probe_ccd(x0, x1, kinds, ids4, thickness, device=0):

@@ -225,6 +225,11 @@ int cs_reset_timing(cs_solver* s);
 int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
 
 /* ---- component probes for parity tests (no reference counterpart) ------- */
+/* buildSchwarzDomains (src/solver/schwarz.ts:44-140) on the host, without a
+ * device: vertex -> domain id and local index (n each); *domains = count. */
+int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
+                         uint32_t* local_of, uint32_t* domains);
+
 /* Begin a step at x0 and build the frozen active set at x (updateActiveSet,
  * contact-assembly.ts:164-220).  Writes min(count, cap) records in active-list
  * order: ids6 = (kind, p, a, b, c, d), vals7 = (w0, w1, w2, w3, nx, ny, nz). */

@@ -725,7 +725,8 @@ void apply_schwarz(const SchwarzFactors& f, const SchwarzDomains& dom, const dou
       }
       for (size_t ii = dw; ii-- > 0;) {
         double s = y[ii];
-        for (size_t j = ii + 1; j < dw; j++) s -= M[j * dw + ii] * sol[j];
+        for (size_t j = ii + 1; j < dw; j++)
+          s -= (g_fault == ORC_FAULT_SCHWARZ_BACK_ROW ? M[ii * dw + j] : M[j * dw + ii]) * sol[j];
         sol[ii] = s / M[ii * dw + ii];
       }
     } else {
@@ -2387,6 +2388,19 @@ int orc_schwarz_domains(orc_solver* o, int target, int minsz, int maxsz, uint32_
   const SchwarzDomains d = build_schwarz_domains(s.mesh, s.mesh.restPos.data(), target, minsz, maxsz);
   std::copy(d.domainOf.begin(), d.domainOf.end(), domain_of);
   std::copy(d.localOf.begin(), d.localOf.end(), local_of);
+  return (int)d.members.size();
+}
+
+int orc_schwarz_apply(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                      const double* r, double* z, uint8_t* flags) {
+  Solver& s = o->s;
+  const SchwarzDomains d = build_schwarz_domains(s.mesh, s.mesh.restPos.data(), target, minsz, maxsz);
+  std::vector<std::vector<uint32_t>> v2t(s.mesh.n);
+  for (uint32_t t = 0; t < s.mesh.m; t++)
+    for (int k = 0; k < 3; k++) v2t[s.mesh.idx[t * 3 + k]].push_back(t);
+  const SchwarzFactors f = build_schwarz_factors(s.mesh, s.mat, x, diag, d, v2t);
+  apply_schwarz(f, d, r, z, s.mesh.n);
+  std::copy(f.flag.begin(), f.flag.end(), flags);
   return (int)d.members.size();
 }
 

@@ -85,12 +85,17 @@ int orc_check_trial(orc_solver* s, const double* x0, const double* xT, double* m
 /* Schwarz domains (src/solver/schwarz.ts:44-140) for tests. */
 int orc_schwarz_domains(orc_solver* s, int target, int minsz, int maxsz, uint32_t* domain_of,
                         uint32_t* local_of);
+/* buildSchwarzFactors at x with Jacobi diagonal `diag`, then applySchwarz to r
+ * (src/solver/schwarz.ts:154-309); flags[d] = 1 Cholesky, 0 Jacobi fallback.
+ * Returns the domain count. */
+int orc_schwarz_apply(orc_solver* s, int target, int minsz, int maxsz, const double* x, const double* diag,
+                      const double* r, double* z, uint8_t* flags);
 
 /* Fault hooks: plant a known mistake in the oracle so tests can show that the
  * checks that pin it are discriminating.  0 = none. */
 void orc_set_fault(int fault);
 enum { ORC_FAULT_NONE = 0, ORC_FAULT_SHEAR_4G = 1, ORC_FAULT_CCD_NO_FALLBACK = 2,
-       ORC_FAULT_BARRIER_SIGN = 3, ORC_FAULT_EE_TRANSPOSE = 4 };
+       ORC_FAULT_BARRIER_SIGN = 3, ORC_FAULT_EE_TRANSPOSE = 4, ORC_FAULT_SCHWARZ_BACK_ROW = 5 };
 
 #ifdef __cplusplus
 }

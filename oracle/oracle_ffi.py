@@ -71,6 +71,8 @@ def lib():
                                               C.POINTER(C.c_int64)]),
             "orc_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
             "orc_schwarz_domains": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _up, _up]),
+            "orc_schwarz_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                                            C.POINTER(C.c_uint8)]),
             "orc_set_fault": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
@@ -192,6 +194,22 @@ class OracleSolver:
         if k > cap:
             return self.active_set(x0, x, cap=int(k), candidates_dmin=candidates_dmin)
         return ids[:k].copy(), vals[:k].copy(), nc.value
+
+    def schwarz_domains(self, target=16, minsz=8, maxsz=32):
+        n = len(self.getPositions()) // 3
+        dom = np.zeros(n, np.uint32)
+        loc = np.zeros(n, np.uint32)
+        cnt = lib().orc_schwarz_domains(self.h, target, minsz, maxsz, dom.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        loc.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return dom, loc, cnt
+
+    def schwarz_apply(self, x, diag, r, target=16, minsz=8, maxsz=32):
+        x = d3(x); diag = d3(diag); r = d3(r)
+        z = np.zeros_like(r)
+        flags = np.zeros(len(x) // 3 + 1, np.uint8)
+        cnt = lib().orc_schwarz_apply(self.h, target, minsz, maxsz, P(x), P(diag), P(r), P(z),
+                                      flags.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return z, flags[:cnt].copy()
 
     def check_trial(self, x0, xt):
         md = C.c_double()

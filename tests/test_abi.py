@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from closim_amd import _abi
@@ -73,3 +74,27 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert L.cs_last_error() == b"CpuSolver not initialized"
     assert L.cs_get_positions(None, None, 0) == _abi.CS_E_NOT_INIT
     assert L.cs_abi_version() == 1
+
+
+@pytest.mark.parametrize("make,sizes", [
+    ("small_strip", (16, 8, 32)),
+    ("strip_pull", (16, 8, 32)),
+    ("strip_pull", (64, 1, 64)),
+    ("crumpled", (24, 8, 48)),
+])
+def test_schwarz_partition_matches_oracle(make, sizes):
+    """buildSchwarzDomains (schwarz.ts:44-140): the library's host partition is
+    the oracle's, vertex by vertex (no device needed)."""
+    import oracle_ffi as O
+    from closim_amd import scenes as S
+    from closim_amd.solver import schwarz_partition
+    from contact_scenes import crumpled_sheet
+    mk = crumpled_sheet if make == "crumpled" else getattr(S, make)
+    dom, loc, cnt = schwarz_partition(mk(), *sizes)
+    o = O.OracleSolver(mk())
+    od, ol, oc = o.schwarz_domains(*sizes)
+    assert cnt == oc and cnt > 1
+    assert np.array_equal(dom, od)
+    assert np.array_equal(loc, ol)
+    sizes_seen = np.bincount(dom)
+    assert sizes_seen.max() < sizes[0] + sizes[1]  # a merged remainder has fewer than min vertices

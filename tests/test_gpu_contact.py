@@ -146,3 +146,11 @@ def test_crumpled_fold_trajectory_matches_oracle():
 
 def test_stacked_patches_block_jacobi_matches_oracle():
     run_pair(stacked_patches, 8, dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI, hvp_mode=_abi.CS_HVP_ANALYTIC))
+
+
+def test_floor_patch_schwarz_matches_oracle():
+    run_pair(S.floor_patch, 20, dict(precond=_abi.CS_PRECOND_SCHWARZ))
+
+
+def test_crumpled_fold_schwarz_matches_oracle():
+    run_pair(crumpled_sheet, 6, dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64)))

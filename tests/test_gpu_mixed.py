@@ -39,6 +39,7 @@ SCENES = [
     (S.small_strip, 20, _abi.CS_PRECOND_BLOCK_JACOBI),
     (S.floor_patch, 20, _abi.CS_PRECOND_BLOCK_JACOBI),
     (S.static_plate_patch, 20, _abi.CS_PRECOND_BLOCK_JACOBI),
+    (S.small_strip, 20, _abi.CS_PRECOND_SCHWARZ),
 ]
 
 # Lagged friction is not the gradient of the merit energy, so Newton with
@@ -49,6 +50,8 @@ CONVERGED_SCENES = [
     (S.small_strip, 20, _abi.CS_PRECOND_BLOCK_JACOBI),
     (lambda: S.floor_patch(mu=0.0), 20, _abi.CS_PRECOND_BLOCK_JACOBI),
     (lambda: S.static_plate_patch(mu=0.0), 20, _abi.CS_PRECOND_BLOCK_JACOBI),
+    (S.small_strip, 20, _abi.CS_PRECOND_SCHWARZ),
+    (lambda: S.floor_patch(mu=0.0), 20, _abi.CS_PRECOND_SCHWARZ),
 ]
 
 

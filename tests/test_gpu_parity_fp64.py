@@ -68,3 +68,20 @@ def test_strip_fp64_analytic_and_block_jacobi_match_oracle():
     kw = dict(hvp_mode=_abi.CS_HVP_ANALYTIC, precond=_abi.CS_PRECOND_BLOCK_JACOBI)
     errs, g, o = run_pair(S.small_strip, 10, kw)
     assert max(errs) <= REL_TOL
+
+
+def test_strip_pull_fp64_schwarz_matches_oracle_per_step():
+    # Additive Schwarz (schwarz.ts): 16-vertex domains on the driven strip.
+    def drive(solver, scene, k):
+        for i, p in S.strip_pull_targets(scene, k):
+            solver.pinVertex(i, p)
+    kw = dict(grad_tol=1e-6, precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(16, 8, 32))
+    errs, g, o = run_pair(S.strip_pull, 30, kw, drive)
+    assert max(errs) <= REL_TOL
+
+
+def test_small_strip_fp64_schwarz_large_domains_matches_oracle():
+    # 64-vertex domains (192x192 local Cholesky) with the analytic HVP.
+    kw = dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64), hvp_mode=_abi.CS_HVP_ANALYTIC)
+    errs, g, o = run_pair(S.small_strip, 10, kw)
+    assert max(errs) <= REL_TOL
