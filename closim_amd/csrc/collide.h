@@ -13,11 +13,14 @@ struct Closest {
   double s, t, dist, rx, ry, rz;
 };
 
+// kDist = false leaves dist at 0 for callers that decide from |r|^2 first
+template <bool kDist = true>
 __device__ __forceinline__ Closest pack_cp(double s, double t, double rx, double ry, double rz) {
-  return Closest{s, t, jsm::hypot3(rx, ry, rz), rx, ry, rz};
+  return Closest{s, t, kDist ? jsm::hypot3(rx, ry, rz) : 0.0, rx, ry, rz};
 }
 
 // closestPointVertexTriangle() (Ericson 5.1.5)
+template <bool kDist = true>
 __device__ inline Closest closest_vt(const double* P, const double* A, const double* B, const double* Cc) {
   const double px = P[0], py = P[1], pz = P[2], ax = A[0], ay = A[1], az = A[2];
   const double bx = B[0], by = B[1], bz = B[2], cx = Cc[0], cy = Cc[1], cz = Cc[2];
@@ -26,30 +29,30 @@ __device__ inline Closest closest_vt(const double* P, const double* A, const dou
   const double apx = px - ax, apy = py - ay, apz = pz - az;
   const double d1 = abx * apx + aby * apy + abz * apz;
   const double d2 = acx * apx + acy * apy + acz * apz;
-  if (d1 <= 0 && d2 <= 0) return pack_cp(0, 0, px - ax, py - ay, pz - az);
+  if (d1 <= 0 && d2 <= 0) return pack_cp<kDist>(0, 0, px - ax, py - ay, pz - az);
   const double bpx = px - bx, bpy = py - by, bpz = pz - bz;
   const double d3 = abx * bpx + aby * bpy + abz * bpz;
   const double d4 = acx * bpx + acy * bpy + acz * bpz;
-  if (d3 >= 0 && d4 <= d3) return pack_cp(1, 0, px - bx, py - by, pz - bz);
+  if (d3 >= 0 && d4 <= d3) return pack_cp<kDist>(1, 0, px - bx, py - by, pz - bz);
   const double vc = d1 * d4 - d3 * d2;
   if (vc <= 0 && d1 >= 0 && d3 <= 0) {
     const double v = d1 / (d1 - d3);
-    return pack_cp(v, 0, px - (ax + v * abx), py - (ay + v * aby), pz - (az + v * abz));
+    return pack_cp<kDist>(v, 0, px - (ax + v * abx), py - (ay + v * aby), pz - (az + v * abz));
   }
   const double cpx = px - cx, cpy = py - cy, cpz = pz - cz;
   const double d5 = abx * cpx + aby * cpy + abz * cpz;
   const double d6 = acx * cpx + acy * cpy + acz * cpz;
-  if (d6 >= 0 && d5 <= d6) return pack_cp(0, 1, px - cx, py - cy, pz - cz);
+  if (d6 >= 0 && d5 <= d6) return pack_cp<kDist>(0, 1, px - cx, py - cy, pz - cz);
   const double vb = d5 * d2 - d1 * d6;
   if (vb <= 0 && d2 >= 0 && d6 <= 0) {
     const double w = d2 / (d2 - d6);
-    return pack_cp(0, w, px - (ax + w * acx), py - (ay + w * acy), pz - (az + w * acz));
+    return pack_cp<kDist>(0, w, px - (ax + w * acx), py - (ay + w * acy), pz - (az + w * acz));
   }
   const double va = d3 * d6 - d5 * d4;
   if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0) {
     const double w = (d4 - d3) / (d4 - d3 + (d5 - d6));
     const double qx = bx + w * (cx - bx), qy = by + w * (cy - by), qz = bz + w * (cz - bz);
-    return pack_cp(1 - w, w, px - qx, py - qy, pz - qz);
+    return pack_cp<kDist>(1 - w, w, px - qx, py - qy, pz - qz);
   }
   const double denom = 1 / (va + vb + vc);
   const double v = vb * denom;
@@ -57,12 +60,13 @@ __device__ inline Closest closest_vt(const double* P, const double* A, const dou
   const double qx = ax + abx * v + acx * w;
   const double qy = ay + aby * v + acy * w;
   const double qz = az + abz * v + acz * w;
-  return pack_cp(v, w, px - qx, py - qy, pz - qz);
+  return pack_cp<kDist>(v, w, px - qx, py - qy, pz - qz);
 }
 
 __device__ __forceinline__ double clamp_(double v, double lo, double hi) { return v < lo ? lo : v > hi ? hi : v; }
 
 // closestPointEdgeEdge() (Ericson 5.1.9)
+template <bool kDist = true>
 __device__ inline Closest closest_ee(const double* A, const double* B, const double* Cc, const double* D) {
   const double ax = A[0], ay = A[1], az = A[2], bx = B[0], by = B[1], bz = B[2];
   const double cx = Cc[0], cy = Cc[1], cz = Cc[2], dx = D[0], dy = D[1], dz = D[2];
@@ -96,7 +100,7 @@ __device__ inline Closest closest_ee(const double* A, const double* B, const dou
   const double axp = ax + d1x * s, ayp = ay + d1y * s, azp = az + d1z * s;
   const double cxp = cx + d2x * t, cyp = cy + d2y * t, czp = cz + d2z * t;
   const double rx2 = axp - cxp, ry2 = ayp - cyp, rz2 = azp - czp;
-  return Closest{s, t, jsm::hypot3(rx2, ry2, rz2), rx2, ry2, rz2};
+  return Closest{s, t, kDist ? jsm::hypot3(rx2, ry2, rz2) : 0.0, rx2, ry2, rz2};
 }
 
 __device__ inline int solve_quadratic01(double a2, double a1, double a0, double* out) {
@@ -205,6 +209,82 @@ __device__ __forceinline__ double stencil_dev(const Seg4& S) {
     if (m > dev) dev = m;
   }
   return dev;
+}
+
+__device__ __forceinline__ Closest ee_at(const double* a0, const double* da, const double* b0, const double* db,
+                                         const double* c0, const double* dc, const double* d0, const double* dd,
+                                         double t);
+
+// The head of vtCCD / eeCCD: resting contact at t = 0 (TOI 0) and the
+// relative-motion early-out (Infinity), ccd-vt.ts:26-52 and ccd-ee.ts:41-59,
+// with the same arithmetic as the full routines below.  Returns false when
+// the cubic roots and the sampled fallback must decide.
+__device__ inline bool ccd_head(const Seg4& S, int kind, double thickness, double& toi) {
+  double d0;
+  if (kind == 0) {
+    d0 = closest_vt(S.x0[0], S.x0[1], S.x0[2], S.x0[3]).dist;
+  } else {
+    double a0[3], da[3], b0[3], db[3], c0[3], dc[3], e0[3], de[3];
+    for (int i = 0; i < 3; i++) {
+      a0[i] = S.x0[0][i]; da[i] = S.x1[0][i] - S.x0[0][i];
+      b0[i] = S.x0[1][i]; db[i] = S.x1[1][i] - S.x0[1][i];
+      c0[i] = S.x0[2][i]; dc[i] = S.x1[2][i] - S.x0[2][i];
+      e0[i] = S.x0[3][i]; de[i] = S.x1[3][i] - S.x0[3][i];
+    }
+    d0 = ee_at(a0, da, b0, db, c0, dc, e0, de, 0).dist;
+  }
+  if (d0 <= thickness) {
+    toi = 0;
+    return true;
+  }
+  const double dev = stencil_dev(S);
+  if (d0 - 4 * dev > thickness) {
+    toi = __builtin_inf();
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double norm2(const Closest& c) { return c.rx * c.rx + c.ry * c.ry + c.rz * c.rz; }
+
+// ccd_head with the two comparisons of the closest distance d0 decided from
+// |r|^2 wherever a relative margin of 1e-9 separates them from their
+// thresholds (the hypot of the reference is within a few ulp of |r|); the
+// exact head runs only in the band around a threshold or for non-finite or
+// extreme inputs.  Same results as ccd_head.
+__device__ inline bool ccd_head_fast(const Seg4& S, int kind, double thickness, double& toi) {
+  Closest c0;
+  if (kind == 0) {
+    c0 = closest_vt<false>(S.x0[0], S.x0[1], S.x0[2], S.x0[3]);
+  } else {
+    // eeCCD evaluates its t = 0 configuration as x0 + dx * 0
+    double A[4][3];
+    for (int v = 0; v < 4; v++)
+      for (int i = 0; i < 3; i++) A[v][i] = S.x0[v][i] + (S.x1[v][i] - S.x0[v][i]) * 0;
+    c0 = closest_ee<false>(A[0], A[1], A[2], A[3]);
+  }
+  const double r2 = norm2(c0);
+  const double lo = 1 - 1e-9, hi = 1 + 1e-9;
+  if (!(r2 >= 1e-200 && r2 <= 1e200) || !(thickness >= 1e-100 && thickness <= 1e100))
+    return ccd_head(S, kind, thickness, toi);
+  const double t2 = thickness * thickness;
+  if (r2 < t2 * lo) {  // d0 < thickness
+    toi = 0;
+    return true;
+  }
+  // the second comparison subtracts 4 dev from d0: keep d0 <= 1e6 thickness
+  // so that its rounding stays far below the margin
+  if (!(r2 > t2 * hi) || r2 > t2 * 1e12) return ccd_head(S, kind, thickness, toi);
+  const double dev = stencil_dev(S);
+  const double b = thickness + 4 * dev;
+  if (!(b <= 1e100)) return ccd_head(S, kind, thickness, toi);
+  const double b2 = b * b;
+  if (r2 > b2 * hi) {  // d0 - 4 dev > thickness
+    toi = __builtin_inf();
+    return true;
+  }
+  if (r2 < b2 * lo) return false;
+  return ccd_head(S, kind, thickness, toi);
 }
 
 // vtCCD() ccd-vt.ts:20-115; slots 0..3 = p, a, b, c.

@@ -47,6 +47,7 @@ struct CScalars {
   unsigned tickets[16];
   uint64_t mink[16];   // okey images of LtMin reductions (all ones = no value)
   uint32_t pair_count, spair_count, active_count, entry_count, nonfinite, floor_fail, nverts, kill_rounds;
+  uint32_t tail_count;
   uint32_t kind_count[4];
   uint64_t nan_key, cross_key;
 };
@@ -520,58 +521,6 @@ __global__ void __launch_bounds__(kBlock) k_traverse_static(const Leaf* __restri
   if (overflow && (threadIdx.x % kWave) == 0) atomicOr(&cs->nonfinite, 2u);
 }
 
-// Candidate pairs of one build from the superset of its segment: a trial
-// or iterate x on the segment (x_a, x_b) has every swept box (x_step -> x)
-// inside the segment box (x_step, x_a, x_b), so the superset holds every
-// pair of the build, and the reference's own predicate (exact swept boxes,
-// inclusive overlap; shared-vertex exclusion holds already) selects them.
-__global__ void __launch_bounds__(kBlock) k_filter_pairs(DGeom G, const double* __restrict__ xs,
-                                                         const double* __restrict__ x, double pad,
-                                                         const uint2* __restrict__ sup, uint32_t nsup, uint2* out,
-                                                         uint32_t cap, uint32_t* counter) {
-  for (uint32_t base = blockIdx.x * blockDim.x; base < nsup; base += gridDim.x * blockDim.x) {
-    const uint32_t k = base + threadIdx.x;
-    bool keep = false;
-    uint2 pr = make_uint2(0, 0);
-    if (k < nsup) {
-      pr = sup[k];
-      double alo[3], ahi[3], blo[3], bhi[3];
-      int32_t av[3], bv[3];
-      swept_box(G, xs, x, pr.x, pad, alo, ahi, av);
-      swept_box(G, xs, x, pr.y, pad, blo, bhi, bv);
-      const double A[6] = {alo[0], alo[1], alo[2], ahi[0], ahi[1], ahi[2]};
-      keep = box_overlap6(A, blo, bhi);
-    }
-    const uint32_t slot = wave_alloc(keep, counter);
-    if (keep && slot < cap) out[slot] = pr;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_filter_spairs(DGeom G, const double* __restrict__ xs,
-                                                          const double* __restrict__ x, double pad,
-                                                          const double* __restrict__ sbox,
-                                                          const uint2* __restrict__ sup, uint32_t nsup, uint2* out,
-                                                          uint32_t cap, uint32_t* counter) {
-  for (uint32_t base = blockIdx.x * blockDim.x; base < nsup; base += gridDim.x * blockDim.x) {
-    const uint32_t k = base + threadIdx.x;
-    bool keep = false;
-    uint2 pr = make_uint2(0, 0);
-    if (k < nsup) {
-      pr = sup[k];
-      double alo[3], ahi[3];
-      int32_t av[3];
-      swept_box(G, xs, x, pr.x, pad, alo, ahi, av);
-      const double A[6] = {alo[0], alo[1], alo[2], ahi[0], ahi[1], ahi[2]};
-      const double* b = sbox + (size_t)pr.y * 6;
-      const double slo[3] = {b[0] - pad, b[1] - pad, b[2] - pad};
-      const double shi[3] = {b[3] + pad, b[4] + pad, b[5] + pad};
-      keep = box_overlap6(A, slo, shi);
-    }
-    const uint32_t slot = wave_alloc(keep, counter);
-    if (keep && slot < cap) out[slot] = pr;
-  }
-}
-
 // ------------------------------------------------------------- prims
 
 // One candidate primitive (expandTriPair self-collision.ts:23-41 for cloth
@@ -585,59 +534,6 @@ __device__ __forceinline__ uint64_t rkey(const DGeom& G, uint64_t kind, uint64_t
   return (kind << (2 * G.B)) | (f1 << G.B) | f2;
 }
 
-__device__ __forceinline__ void prim_of(const DGeom& G, const uint2* pairs, uint32_t np, const uint2* spairs, uint32_t nsp,
-                                        uint32_t tid, Prim& q) {
-  const int TE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
-  // prim ranges: cloth VT (6 per pair), cloth EE (9 per pair), static VT (3),
-  // static EE (9), so that a wave mostly runs one closest-point routine
-  if (tid < np * 15) {
-    const bool vt = tid < np * 6;
-    const uint32_t t1 = vt ? tid : tid - np * 6;
-    const uint2 pr = pairs[vt ? t1 / 6 : t1 / 9];
-    const int k = vt ? (int)(t1 % 6) : 6 + (int)(t1 % 9);
-    const uint32_t tA = pr.x, tB = pr.y;
-    const int A[3] = {(int)G.idx[(size_t)tA * 3], (int)G.idx[(size_t)tA * 3 + 1], (int)G.idx[(size_t)tA * 3 + 2]};
-    const int B[3] = {(int)G.idx[(size_t)tB * 3], (int)G.idx[(size_t)tB * 3 + 1], (int)G.idx[(size_t)tB * 3 + 2]};
-    if (k < 6) {
-      const bool fromA = k < 3;
-      const int* P = fromA ? A : B;
-      const int* T = fromA ? B : A;
-      const uint32_t tri = fromA ? tB : tA;
-      q.kind = 0; q.p = P[k % 3]; q.a = T[0]; q.b = T[1]; q.c = T[2]; q.d = -1;
-      q.r1 = tri;
-    } else {
-      const int e = k - 6, ea = e / 3, eb = e % 3;
-      q.kind = 1; q.p = -1;
-      q.a = A[TE[ea][0]]; q.b = A[TE[ea][1]]; q.c = B[TE[eb][0]]; q.d = B[TE[eb][1]];
-      const uint32_t de1 = tA * 3 + ea, de2 = tB * 3 + eb;
-      q.r1 = de1;
-      q.r2 = de2;
-    }
-  } else {
-    const uint32_t t2 = tid - np * 15;
-    const bool vt = t2 < nsp * 3;
-    const uint32_t t3 = vt ? t2 : t2 - nsp * 3;
-    const uint2 pr = spairs[vt ? t3 / 3 : t3 / 9];
-    const int k = vt ? (int)(t3 % 3) : 3 + (int)(t3 % 9);
-    const uint32_t ct = pr.x, st = pr.y;
-    const int A[3] = {(int)G.idx[(size_t)ct * 3], (int)G.idx[(size_t)ct * 3 + 1], (int)G.idx[(size_t)ct * 3 + 2]};
-    const int B[3] = {-(int)G.sidx[(size_t)st * 3] - 2, -(int)G.sidx[(size_t)st * 3 + 1] - 2,
-                      -(int)G.sidx[(size_t)st * 3 + 2] - 2};
-    const uint32_t stri = G.m + st;
-    if (k < 3) {
-      q.kind = 0; q.p = A[k]; q.a = B[0]; q.b = B[1]; q.c = B[2]; q.d = -2;
-      q.r1 = stri;
-    } else {
-      const int e = k - 3, ea = e / 3, eb = e % 3;
-      q.kind = 1; q.p = -1;
-      q.a = A[TE[ea][0]]; q.b = A[TE[ea][1]]; q.c = B[TE[eb][0]]; q.d = B[TE[eb][1]];
-      const uint32_t de1 = ct * 3 + ea, de2 = stri * 3 + eb;
-      q.r1 = de1;
-      q.r2 = de2;
-    }
-  }
-}
-
 // Sort key (candidate order) and lagged-friction key of a prim.
 __device__ __forceinline__ uint64_t prim_key(const DGeom& G, const Prim& q) {
   if (q.kind == 0) return rkey(G, 0, (uint64_t)q.p, G.tri_rank[q.r1]);
@@ -649,53 +545,318 @@ __device__ __forceinline__ uint64_t prim_nkey(const DGeom& G, const Prim& q) {
   return rkey(G, 1, u1 < u2 ? u1 : u2, u1 < u2 ? u2 : u1);
 }
 
-__device__ __forceinline__ Closest prim_closest(const DGeom& G, const double* x, const Prim& q) {
-  double P0[3], P1[3], P2[3], P3[3];
-  if (q.kind == 0) {
-    vpos(G, x, q.p, P0); vpos(G, x, q.a, P1); vpos(G, x, q.b, P2); vpos(G, x, q.c, P3);
-    return closest_vt(P0, P1, P2, P3);
+// ------------------------------------------------------------ pair kernels
+//
+// One thread per triangle pair: the pair's six corners are loaded once, at
+// both configurations, and its 15 (cloth/cloth) or 12 (cloth/static) prims
+// (the expansion of expandTriPair, self-collision.ts:23-41 and :105-114) are
+// evaluated from registers, instead of one thread per prim gathering its four
+// corners.  With kFilter the pairs are the step-segment superset and each is
+// first tested with the broadphase's padded swept-box predicate (the same
+// arithmetic as swept_box + box_overlap6), which selects exactly the
+// candidate pairs of a traversal at these positions.
+
+struct PairCorners {
+  int v[6];               // slots 0-2: cloth triangle A, 3-5: triangle B (static corners as -id-2)
+  double x0[6][3], x1[6][3];
+};
+
+__device__ __forceinline__ void load_pair(const DGeom& G, const double* __restrict__ xa, const double* __restrict__ xb,
+                                          uint2 pr, bool stat, PairCorners& P) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int v = (int)G.idx[(size_t)pr.x * 3 + c];
+    P.v[c] = v;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      P.x0[c][k] = xa[(size_t)v * 3 + k];
+      P.x1[c][k] = xb[(size_t)v * 3 + k];
+    }
   }
-  vpos(G, x, q.a, P0); vpos(G, x, q.b, P1); vpos(G, x, q.c, P2); vpos(G, x, q.d, P3);
-  return closest_ee(P0, P1, P2, P3);
+  // triangle B: cloth corners from xa / xb, or static corners (fixed)
+  const uint32_t* bidx = stat ? G.sidx : G.idx;
+  const double* ba = stat ? G.spos : xa;
+  const double* bb = stat ? G.spos : xb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const uint32_t v = bidx[(size_t)pr.y * 3 + c];
+    P.v[3 + c] = stat ? -(int)v - 2 : (int)v;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      P.x0[3 + c][k] = ba[(size_t)v * 3 + k];
+      P.x1[3 + c][k] = bb[(size_t)v * 3 + k];
+    }
+  }
 }
 
-// updateActiveSet (contact-assembly.ts:164-199): closest point per candidate,
-// active when 1e-12 <= d < dHat; min distance over all candidates.
-__global__ void __launch_bounds__(kBlock) k_prims_active(DGeom G, const double* __restrict__ x,
-                                                         const uint2* __restrict__ pairs, uint32_t np,
-                                                         const uint2* __restrict__ spairs, uint32_t nsp,
-                                                         Frozen* rec, uint64_t* keys, uint64_t* nkeys, uint32_t cap,
-                                                         double* partials, CScalars* cs) {
-  const uint32_t total = np * 15 + nsp * 12;  // < 2^31, checked by the host
-  double minD = __builtin_inf();
-  // uniform trip count per block so that the wave-aggregated emission sees
-  // every lane
-  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
-    const uint32_t tid = base + threadIdx.x;
-    bool emit = false;
-    Prim q;
-    Frozen f;
-    if (tid < total) {
-      prim_of(G, pairs, np, spairs, nsp, tid, q);
-      const Closest cp = prim_closest(G, x, q);
-      if (cp.dist < minD) minD = cp.dist;
-      if (!(cp.dist >= G.d_hat || cp.dist < 1e-12)) {
-        emit = true;
-        // frozen VT records carry d = -1 even for static candidates (contact-assembly.ts:181)
-        f.kind = q.kind; f.p = q.p; f.a = q.a; f.b = q.b; f.c = q.c; f.d = q.kind == 0 ? -1 : q.d;
-        f.nx = cp.rx / cp.dist; f.ny = cp.ry / cp.dist; f.nz = cp.rz / cp.dist;
-        if (q.kind == 0) {
-          f.w1 = cp.s; f.w2 = cp.t; f.w0 = 1 - cp.s - cp.t; f.w3 = 0;
-        } else {
-          f.w0 = 1 - cp.s; f.w1 = cp.s; f.w2 = -(1 - cp.t); f.w3 = -cp.t;
-        }
+// swept_box over corner slots base..base+2 (same visiting order)
+__device__ __forceinline__ void corner_box(const PairCorners& P, int base, double pad, double* lo, double* hi) {
+  double mn[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+  double mx[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const double pk = q == 0 ? P.x0[base + c][k] : P.x1[base + c][k];
+        if (pk < mn[k]) mn[k] = pk;
+        if (pk > mx[k]) mx[k] = pk;
       }
     }
-    const uint32_t k = wave_alloc(emit, &cs->active_count);
-    if (emit && k < cap) {
-      rec[k] = f;
-      keys[k] = prim_key(G, q);
-      nkeys[k] = prim_nkey(G, q);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    lo[k] = mn[k] - pad;
+    hi[k] = mx[k] + pad;
+  }
+}
+
+// Candidate pairs of one build from the superset of its segment: a trial
+// or iterate x on the segment (x_a, x_b) has every swept box (x_step -> x)
+// inside the segment box (x_step, x_a, x_b), so the superset holds every
+// pair of the build, and the reference's own predicate (exact swept boxes,
+// inclusive overlap; shared-vertex exclusion holds already) selects them.
+__device__ __forceinline__ bool pair_keep(const PairCorners& P, bool stat, double pad, const double* __restrict__ sbox,
+                                          uint32_t st) {
+  double alo[3], ahi[3];
+  corner_box(P, 0, pad, alo, ahi);
+  const double A[6] = {alo[0], alo[1], alo[2], ahi[0], ahi[1], ahi[2]};
+  if (!stat) {
+    double blo[3], bhi[3];
+    corner_box(P, 3, pad, blo, bhi);
+    return box_overlap6(A, blo, bhi);
+  }
+  const double* b = sbox + (size_t)st * 6;
+  const double slo[3] = {b[0] - pad, b[1] - pad, b[2] - pad};
+  const double shi[3] = {b[3] + pad, b[4] + pad, b[5] + pad};
+  return box_overlap6(A, slo, shi);
+}
+
+// Corner ids by slot from scalar copies: a select over values, never an
+// index into the pair's arrays (which would move them to scratch memory).
+struct Ids6 {
+  int s0, s1, s2, s3, s4, s5;
+  __device__ __forceinline__ int at(int s) const {
+    return s == 0 ? s0 : s == 1 ? s1 : s == 2 ? s2 : s == 3 ? s3 : s == 4 ? s4 : s5;
+  }
+};
+
+__device__ __forceinline__ Ids6 pair_ids(const PairCorners& P) {
+  return Ids6{P.v[0], P.v[1], P.v[2], P.v[3], P.v[4], P.v[5]};
+}
+
+// The prim (kind, corner slots, reference ids) of prim k of a pair: cloth
+// pairs k < 6 VT (A's corners against B, then B's against A), 6..14 EE;
+// static pairs k < 3 VT (cloth corners against the static triangle), 3..11
+// EE.  The set matches expandTriPair; the order inside a pair is free since
+// every consumer reduces with min or sorts by key.
+__device__ __forceinline__ void pair_prim(const DGeom& G, const Ids6 V, uint2 pr, bool stat, int k, Prim& q) {
+  const int nvt = stat ? 3 : 6;
+  if (k < nvt) {
+    // VT: corner k against the other triangle (k < 3: A's corner vs B)
+    const int tb = k < 3 ? 3 : 0;
+    q.kind = 0; q.p = V.at(k); q.a = V.at(tb); q.b = V.at(tb + 1); q.c = V.at(tb + 2);
+    q.d = stat ? -2 : -1;
+    q.r1 = stat ? G.m + pr.y : (k < 3 ? pr.y : pr.x);
+    q.r2 = 0;
+  } else {
+    // EE: edge ea of A against edge eb of B, edges (0,1), (1,2), (2,0)
+    const int e = k - nvt, ea = e / 3, eb = e % 3;
+    q.kind = 1; q.p = -1;
+    q.a = V.at(ea); q.b = V.at(ea == 2 ? 0 : ea + 1);
+    q.c = V.at(3 + eb); q.d = V.at(3 + (eb == 2 ? 0 : eb + 1));
+    q.r1 = pr.x * 3 + ea;
+    q.r2 = (stat ? G.m + pr.y : pr.y) * 3 + eb;
+  }
+}
+
+// Corners of prim k of a pair copied out of the pair's registers.  k is the
+// same for every lane of a wave (a loop counter), so the switch is a scalar
+// branch and every register index is a constant.
+template <int A0, int A1, int A2, int A3, bool kBoth>
+__device__ __forceinline__ void take4(const PairCorners& P, Seg4& S) {
+  constexpr int sl[4] = {A0, A1, A2, A3};
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      S.x1[q][k] = P.x1[sl[q]][k];
+      if (kBoth) S.x0[q][k] = P.x0[sl[q]][k];
+    }
+}
+
+template <bool kBoth>
+__device__ __forceinline__ void prim_seg(const PairCorners& P, bool stat, int k, Seg4& S) {
+  if (!stat) {
+    switch (k) {
+      case 0: take4<0, 3, 4, 5, kBoth>(P, S); break;
+      case 1: take4<1, 3, 4, 5, kBoth>(P, S); break;
+      case 2: take4<2, 3, 4, 5, kBoth>(P, S); break;
+      case 3: take4<3, 0, 1, 2, kBoth>(P, S); break;
+      case 4: take4<4, 0, 1, 2, kBoth>(P, S); break;
+      case 5: take4<5, 0, 1, 2, kBoth>(P, S); break;
+      case 6: take4<0, 1, 3, 4, kBoth>(P, S); break;
+      case 7: take4<0, 1, 4, 5, kBoth>(P, S); break;
+      case 8: take4<0, 1, 5, 3, kBoth>(P, S); break;
+      case 9: take4<1, 2, 3, 4, kBoth>(P, S); break;
+      case 10: take4<1, 2, 4, 5, kBoth>(P, S); break;
+      case 11: take4<1, 2, 5, 3, kBoth>(P, S); break;
+      case 12: take4<2, 0, 3, 4, kBoth>(P, S); break;
+      case 13: take4<2, 0, 4, 5, kBoth>(P, S); break;
+      default: take4<2, 0, 5, 3, kBoth>(P, S); break;
+    }
+  } else {
+    switch (k) {
+      case 0: take4<0, 3, 4, 5, kBoth>(P, S); break;
+      case 1: take4<1, 3, 4, 5, kBoth>(P, S); break;
+      case 2: take4<2, 3, 4, 5, kBoth>(P, S); break;
+      case 3: take4<0, 1, 3, 4, kBoth>(P, S); break;
+      case 4: take4<0, 1, 4, 5, kBoth>(P, S); break;
+      case 5: take4<0, 1, 5, 3, kBoth>(P, S); break;
+      case 6: take4<1, 2, 3, 4, kBoth>(P, S); break;
+      case 7: take4<1, 2, 4, 5, kBoth>(P, S); break;
+      case 8: take4<1, 2, 5, 3, kBoth>(P, S); break;
+      case 9: take4<2, 0, 3, 4, kBoth>(P, S); break;
+      case 10: take4<2, 0, 4, 5, kBoth>(P, S); break;
+      default: take4<2, 0, 5, 3, kBoth>(P, S); break;
+    }
+  }
+}
+
+// Swept boxes of the two triangles (unpadded) further apart than the
+// thickness bound of the prim prefilter, all corners finite: every prim of
+// the pair passes that prefilter too, since a prim's two parts lie inside the
+// two triangles' boxes and the rounded gap is monotone in the boxes.
+__device__ __forceinline__ bool pair_separated(const PairCorners& P, double d_min) {
+  double gap2 = 0;
+  bool all_finite = true;
+#pragma unroll
+  for (int kk = 0; kk < 3; kk++) {
+    double lo1 = __builtin_inf(), hi1 = -__builtin_inf(), lo2 = __builtin_inf(), hi2 = -__builtin_inf();
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const double a0 = P.x0[s][kk], a1 = P.x1[s][kk];
+      all_finite = all_finite && jsm::isfinite_(a0) && jsm::isfinite_(a1);
+      const double l = fmin(a0, a1), h = fmax(a0, a1);
+      if (s < 3) { lo1 = fmin(lo1, l); hi1 = fmax(hi1, h); }
+      else { lo2 = fmin(lo2, l); hi2 = fmax(hi2, h); }
+    }
+    const double g = fmax(fmax(lo2 - hi1, lo1 - hi2), 0.0);
+    gap2 += g * g;
+  }
+  const double thr = d_min * (1 + 1e-6);
+  return all_finite && gap2 > thr * thr;
+}
+
+// Swept boxes of a prim's two parts (VT: the point / the triangle; EE: edge
+// / edge) further apart than the thickness, with a 1e-6 relative margin, all
+// corners finite: no configuration on the segment x0 -> x1 comes within the
+// thickness, and the reference's CCD returns Infinity on every branch (not
+// resting, no validated root, no sample within the thickness).
+__device__ __forceinline__ bool prim_separated(const Seg4& S, int kind, double d_min) {
+  const int split = kind == 0 ? 1 : 2;
+  double gap2 = 0;
+  bool all_finite = true;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double lo1 = __builtin_inf(), hi1 = -__builtin_inf(), lo2 = __builtin_inf(), hi2 = -__builtin_inf();
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const double a0 = S.x0[s][k], a1 = S.x1[s][k];
+      all_finite = all_finite && jsm::isfinite_(a0) && jsm::isfinite_(a1);
+      const double l = fmin(a0, a1), h = fmax(a0, a1);
+      if (s < split) { lo1 = fmin(lo1, l); hi1 = fmax(hi1, h); }
+      else { lo2 = fmin(lo2, l); hi2 = fmax(hi2, h); }
+    }
+    const double g = fmax(fmax(lo2 - hi1, lo1 - hi2), 0.0);
+    gap2 += g * g;
+  }
+  const double thr = d_min * (1 + 1e-6);
+  return all_finite && gap2 > thr * thr;
+}
+
+// A CCD result of checkTrial (contact-assembly.ts:515-533): NaN marks a CCD
+// failure, a TOI in (0, 1 - 1e-9) a crossing; the smallest key of each kind
+// decides which came first in candidate order.
+__device__ __forceinline__ void trial_toi(const DGeom& G, const Prim& q, double toi, double& minT, CScalars* cs) {
+  if (jsm::isnan_(toi)) {
+    atomicMin((unsigned long long*)&cs->nan_key, (unsigned long long)prim_key(G, q));
+  } else {
+    if (toi < minT) minT = toi;
+    if (toi > 0 && toi < 1 - 1e-9) atomicMin((unsigned long long*)&cs->cross_key, (unsigned long long)prim_key(G, q));
+  }
+}
+
+// updateActiveSet (contact-assembly.ts:164-199) per pair: closest point per
+// prim at x, active when 1e-12 <= d < dHat (frozen record, sort key and
+// lagged-friction key emitted wave-aggregated); min distance over all
+// candidates.  With kFilter the pair lists are the superset, filtered with
+// the padded swept boxes from the step start to x.
+template <bool kFilter>
+__global__ void __launch_bounds__(kBlock) k_active_pairs(DGeom G, const double* __restrict__ xs,
+                                                         const double* __restrict__ x, double pad,
+                                                         const uint2* __restrict__ pairs, uint32_t np,
+                                                         const uint2* __restrict__ spairs, uint32_t nsp,
+                                                         const double* __restrict__ sbox, Frozen* rec, uint64_t* keys,
+                                                         uint64_t* nkeys, uint32_t cap, CScalars* cs) {
+  const uint32_t total = np + nsp;
+  double minD = __builtin_inf();
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool stat = i >= np;
+    bool keep = false;
+    PairCorners P;
+    uint2 pr = make_uint2(0, 0);
+    if (i < total) {
+      pr = stat ? spairs[i - np] : pairs[i];
+      load_pair(G, xs, x, pr, stat, P);
+      keep = kFilter ? pair_keep(P, stat, pad, sbox, pr.y) : true;
+    }
+    if (kFilter) {
+      const uint64_t mc = __ballot(keep && !stat), ms = __ballot(keep && stat);
+      if ((threadIdx.x % kWave) == 0) {
+        if (mc) atomicAdd(&cs->pair_count, (uint32_t)__popcll(mc));
+        if (ms) atomicAdd(&cs->spair_count, (uint32_t)__popcll(ms));
+      }
+    }
+    const int nk = stat ? 12 : 15;
+    // uniform trip count so that the wave-aggregated emission sees every lane
+    for (int k = 0; k < 15; k++) {
+      bool emit = false;
+      Prim q;
+      Frozen f;
+      if (keep && k < nk) {
+        pair_prim(G, pair_ids(P), pr, stat, k, q);
+        Seg4 S;
+        prim_seg<false>(P, stat, k, S);
+        Closest cp = q.kind == 0 ? closest_vt<false>(S.x1[0], S.x1[1], S.x1[2], S.x1[3])
+                                 : closest_ee<false>(S.x1[0], S.x1[1], S.x1[2], S.x1[3]);
+        // the exact distance matters only below dHat or below this thread's
+        // minimum; |r|^2 above both (with a 1e-9 margin) rules both out
+        const double r2 = norm2(cp);
+        const double lim = minD < G.d_hat ? G.d_hat : minD;
+        const bool far = G.d_hat >= 1e-100 && lim <= 1e100 && r2 <= 1e200 && r2 > lim * lim * (1 + 1e-9);
+        cp.dist = far ? __builtin_inf() : jsm::hypot3(cp.rx, cp.ry, cp.rz);
+        if (cp.dist < minD) minD = cp.dist;
+        if (!(cp.dist >= G.d_hat || cp.dist < 1e-12)) {
+          emit = true;
+          // frozen VT records carry d = -1 even for static candidates (contact-assembly.ts:181)
+          f.kind = q.kind; f.p = q.p; f.a = q.a; f.b = q.b; f.c = q.c; f.d = q.kind == 0 ? -1 : q.d;
+          f.nx = cp.rx / cp.dist; f.ny = cp.ry / cp.dist; f.nz = cp.rz / cp.dist;
+          if (q.kind == 0) {
+            f.w1 = cp.s; f.w2 = cp.t; f.w0 = 1 - cp.s - cp.t; f.w3 = 0;
+          } else {
+            f.w0 = 1 - cp.s; f.w1 = cp.s; f.w2 = -(1 - cp.t); f.w3 = -cp.t;
+          }
+        }
+      }
+      const uint32_t slot = wave_alloc(emit, &cs->active_count);
+      if (emit && slot < cap) {
+        rec[slot] = f;
+        keys[slot] = prim_key(G, q);
+        nkeys[slot] = prim_nkey(G, q);
+      }
     }
   }
   block_min_to(minD, &cs->mink[0]);
@@ -1083,75 +1244,122 @@ __global__ void k_count_below(const uint64_t* keys, uint32_t n, uint32_t* out) {
 }
 
 // killApproachVelocity (newton.ts:238-303): the reference sweeps the active
-// list once in order.  A contact may run as soon as every earlier contact that
-// shares one of its vertices has run, i.e. when it is at the head of each of
-// its vertices' incidence lists (which are in list order).  Contacts that are
-// ready together share no vertex, so each round runs its whole frontier in
-// parallel and the result is the sequential sweep's.  A finished contact
-// advances the heads of its vertices and proposes the next contact of each
-// list for the following round, so a round costs its frontier, not C.  One
-// work-group; every round runs at least the earliest pending contact, so the
-// loop ends within C rounds.
-__device__ __forceinline__ int kill_verts(const Frozen& c, int* vs) {
-  if (c.kind == 2) { vs[0] = c.p; vs[1] = vs[2] = vs[3] = -1; }
-  else if (c.kind == 0) { vs[0] = c.p; vs[1] = c.a; vs[2] = c.b; vs[3] = c.c; }
-  else { vs[0] = c.a; vs[1] = c.b; vs[2] = c.c; vs[3] = c.d; }
-  return 4;
+// list once in order.  A contact depends only on the earlier contacts that
+// share one of its cloth vertices, so the sweep is a DAG: each contact's
+// successors are the next contacts of its vertices' incidence lists (which
+// are in list order) and it may run once all its predecessors have.  Ready
+// contacts share no vertex (of two contacts on one vertex the later waits for
+// the earlier), so any set of ready contacts can run in parallel and the
+// result is the sequential sweep's, bit for bit.
+//
+// The sweep is latency-bound (its critical path is a large part of C), so it
+// runs in one work-group with the touched vertices' velocities and the
+// predecessor counters in LDS: a thread that finishes a contact decrements
+// its successors' counters and goes on at once with one that became ready
+// (the others start in the next round).
+
+constexpr uint32_t kNoSucc = 0xffffffffu;
+
+// One contact of the sweep, in list order: normal, barycentric weights, the
+// cloth vertices (global id and LDS slot; -1 / 0xffff for none or static),
+// and the next contact of each vertex.
+struct KillRec {
+  double nx, ny, nz, w0, w1, w2;
+  int32_t vid[4];
+  uint16_t slot[4];
+  uint32_t succ[4];
+  uint32_t kind;
+  uint32_t indeg;  // contacts that precede this one on its vertices
+};
+
+__global__ void k_kill_slots(const uint32_t* __restrict__ cverts, uint32_t nv, uint32_t* vslot) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nv) vslot[cverts[k]] = k;
 }
 
-__device__ __forceinline__ bool kill_ready(const Frozen* __restrict__ act, const uint32_t* __restrict__ v_off,
-                                           const uint32_t* __restrict__ v_ent, const uint32_t* heads, uint32_t i) {
+__global__ void k_kill_rec(const Frozen* __restrict__ act, uint32_t C, const uint32_t* __restrict__ vslot,
+                           KillRec* rec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
   const Frozen c = act[i];
-  int vs[4];
-  kill_verts(c, vs);
+  KillRec r;
+  r.nx = c.nx; r.ny = c.ny; r.nz = c.nz;
+  r.w0 = c.w0; r.w1 = c.w1; r.w2 = c.w2;
+  const int vs[4] = {c.kind == 2 ? c.p : c.kind == 0 ? c.p : c.a, c.kind == 2 ? -1 : c.kind == 0 ? c.a : c.b,
+                     c.kind == 2 ? -1 : c.kind == 0 ? c.b : c.c, c.kind == 2 ? -1 : c.kind == 0 ? c.c : c.d};
   for (int s = 0; s < 4; s++) {
-    if (vs[s] < 0) continue;
-    const uint32_t h = heads[vs[s]];
-    if (h >= v_off[vs[s] + 1] || (v_ent[h] >> 3) != i) return false;
+    r.vid[s] = vs[s] >= 0 ? vs[s] : -1;
+    r.slot[s] = vs[s] >= 0 ? (uint16_t)vslot[vs[s]] : (uint16_t)0xffff;
+    r.succ[s] = kNoSucc;
   }
-  return true;
+  r.kind = (uint32_t)c.kind;
+  r.indeg = 0;
+  rec[i] = r;
 }
 
-__device__ __forceinline__ void kill_apply(const Frozen& c, double* v) {
+// Per incidence entry (vertex u, contact i, stencil slot s): the successor of
+// i on u and one predecessor count if i is not u's first contact.
+__global__ void k_kill_links(const uint32_t* __restrict__ v_off, const uint32_t* __restrict__ v_ent,
+                             const uint32_t* __restrict__ ent_vtx, uint32_t E, KillRec* rec) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= E) return;
+  const uint32_t u = ent_vtx[h];
+  const uint32_t e = v_ent[h];
+  const uint32_t i = e >> 3, s = e & 7;
+  if (h + 1 < v_off[u + 1]) rec[i].succ[s] = v_ent[h + 1] >> 3;
+  if (h > v_off[u]) atomicAdd(&rec[i].indeg, 1u);
+}
+
+// The reference's per-contact update (newton.ts:245-301) on velocities reached
+// through V(vid, slot), in its operation order.
+template <class VelAt>
+__device__ __forceinline__ void kill_contact(const KillRec& c, VelAt V) {
   if (c.kind == 2) {
-    const double vy = v[(size_t)c.p * 3 + 1];
-    if (vy < 0) v[(size_t)c.p * 3 + 1] = 0;
+    double* vp = V(c.vid[0], c.slot[0]);
+    if (vp[1] < 0) vp[1] = 0;
   } else if (c.kind == 0) {
-    const size_t p = c.p;
-    const double vp[3] = {v[p * 3], v[p * 3 + 1], v[p * 3 + 2]};
+    double* vp = V(c.vid[0], c.slot[0]);
+    const double p0 = vp[0], p1 = vp[1], p2 = vp[2];
     double tx = 0, ty = 0, tz = 0;
     const double ws[3] = {c.w0, c.w1, c.w2};
-    const int vs[3] = {c.a, c.b, c.c};
+#pragma unroll
     for (int k = 0; k < 3; k++) {
-      if (vs[k] < 0) continue;
-      const size_t q = vs[k];
-      tx += ws[k] * v[q * 3];
-      ty += ws[k] * v[q * 3 + 1];
-      tz += ws[k] * v[q * 3 + 2];
+      if (c.vid[k + 1] < 0) continue;
+      const double* q = V(c.vid[k + 1], c.slot[k + 1]);
+      tx += ws[k] * q[0];
+      ty += ws[k] * q[1];
+      tz += ws[k] * q[2];
     }
-    const double rel = (vp[0] - tx) * c.nx + (vp[1] - ty) * c.ny + (vp[2] - tz) * c.nz;
+    const double rel = (p0 - tx) * c.nx + (p1 - ty) * c.ny + (p2 - tz) * c.nz;
     if (rel < 0) {
       const double j = -0.5 * rel;
-      v[p * 3] += j * c.nx;
-      v[p * 3 + 1] += j * c.ny;
-      v[p * 3 + 2] += j * c.nz;
+      vp[0] += j * c.nx;
+      vp[1] += j * c.ny;
+      vp[2] += j * c.nz;
+#pragma unroll
       for (int k = 0; k < 3; k++) {
-        if (vs[k] < 0) continue;
-        const size_t q = vs[k];
-        v[q * 3] -= j * ws[k] * c.nx;
-        v[q * 3 + 1] -= j * ws[k] * c.ny;
-        v[q * 3 + 2] -= j * ws[k] * c.nz;
+        if (c.vid[k + 1] < 0) continue;
+        double* q = V(c.vid[k + 1], c.slot[k + 1]);
+        q[0] -= j * ws[k] * c.nx;
+        q[1] -= j * ws[k] * c.ny;
+        q[2] -= j * ws[k] * c.nz;
       }
     }
   } else {
     double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, Cc[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+    double* pa = c.vid[0] >= 0 ? V(c.vid[0], c.slot[0]) : nullptr;
+    double* pb = c.vid[1] >= 0 ? V(c.vid[1], c.slot[1]) : nullptr;
+    double* pc = c.vid[2] >= 0 ? V(c.vid[2], c.slot[2]) : nullptr;
+    double* pd = c.vid[3] >= 0 ? V(c.vid[3], c.slot[3]) : nullptr;
+#pragma unroll
     for (int q = 0; q < 3; q++) {
-      if (c.a >= 0) A[q] = v[(size_t)c.a * 3 + q];
-      if (c.b >= 0) B[q] = v[(size_t)c.b * 3 + q];
-      if (c.c >= 0) Cc[q] = v[(size_t)c.c * 3 + q];
-      if (c.d >= 0) D[q] = v[(size_t)c.d * 3 + q];
+      if (pa) A[q] = pa[q];
+      if (pb) B[q] = pb[q];
+      if (pc) Cc[q] = pc[q];
+      if (pd) D[q] = pd[q];
     }
     double ab[3], cd[3];
+#pragma unroll
     for (int q = 0; q < 3; q++) {
       ab[q] = (A[q] + B[q]) / 2;
       cd[q] = (Cc[q] + D[q]) / 2;
@@ -1159,113 +1367,236 @@ __device__ __forceinline__ void kill_apply(const Frozen& c, double* v) {
     const double rel = (ab[0] - cd[0]) * c.nx + (ab[1] - cd[1]) * c.ny + (ab[2] - cd[2]) * c.nz;
     if (rel < 0) {
       const double j = -0.25 * rel;
-      const int plus[2] = {c.a, c.b}, minus[2] = {c.c, c.d};
-      for (int vv : plus) {
-        if (vv < 0) continue;
-        v[(size_t)vv * 3] += j * c.nx;
-        v[(size_t)vv * 3 + 1] += j * c.ny;
-        v[(size_t)vv * 3 + 2] += j * c.nz;
+      double* plus[2] = {pa, pb};
+      double* minus[2] = {pc, pd};
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (!plus[k]) continue;
+        plus[k][0] += j * c.nx;
+        plus[k][1] += j * c.ny;
+        plus[k][2] += j * c.nz;
       }
-      for (int vv : minus) {
-        if (vv < 0) continue;
-        v[(size_t)vv * 3] -= j * c.nx;
-        v[(size_t)vv * 3 + 1] -= j * c.ny;
-        v[(size_t)vv * 3 + 2] -= j * c.nz;
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (!minus[k]) continue;
+        minus[k][0] -= j * c.nx;
+        minus[k][1] -= j * c.ny;
+        minus[k][2] -= j * c.nz;
       }
     }
   }
 }
 
-// heads = v_off[0..n) on entry; flag[C] = 0; listA / listB hold C entries.
-// Run-ahead: the thread that ran contact i owns i's cloth vertices for the
-// rest of the round (no other ready contact shares them), so it may go on
-// with the next contact of i's first vertex when that contact is ready and
-// touches only those vertices: long runs of contacts on one vertex or edge
-// (a vertex against many static triangles) then take one round, not one each.
-__global__ void __launch_bounds__(1024) k_kill_approach(const Frozen* __restrict__ act, uint32_t C,
-                                                        const uint32_t* __restrict__ v_off,
-                                                        const uint32_t* __restrict__ v_ent, uint32_t* heads,
-                                                        uint32_t* flag, uint32_t* listA, uint32_t* listB, double* v,
-                                                        uint32_t* rounds_out) {
+constexpr int kKillBlock = 256;
+
+// The sweep.  kLds: velocities of the nv touched vertices (slot order) and
+// byte-packed predecessor counters in LDS, sized by the host; otherwise the
+// velocities are updated in place in global memory and the counters live in
+// `gcnt`.  listA / listB (LDS, nv entries each: a frontier is vertex-disjoint)
+// hold the round's ready contacts.
+template <bool kLds>
+__global__ void __launch_bounds__(kKillBlock) k_kill_run(const KillRec* __restrict__ rec, uint32_t C,
+                                                         const uint32_t* __restrict__ cverts, uint32_t nv, double* v,
+                                                         uint32_t* gcnt, uint32_t* glist, uint32_t* rounds_out) {
+  extern __shared__ __align__(16) unsigned char kill_lds[];
   __shared__ uint32_t nA, nB;
+  double* sv = reinterpret_cast<double*>(kill_lds);                              // kLds: nv * 3
+  uint32_t* cnt = kLds ? reinterpret_cast<uint32_t*>(sv + (size_t)nv * 3) : gcnt;  // kLds: ceil(C / 4) words
+  uint32_t* listA = kLds ? cnt + (C + 3) / 4 : glist;
+  uint32_t* listB = listA + nv;
+  auto vel = [&](int vid, uint32_t slot) -> double* {
+    return kLds ? sv + (size_t)slot * 3 : v + (size_t)vid * 3;
+  };
+  // decrement contact j's counter; true when it reached zero.  The LDS sweep
+  // orders its velocity writes before the decrements and the reads of a
+  // released contact after them with LDS-only fences (lgkmcnt), so that the
+  // prefetched record load stays in flight; the global sweep needs full
+  // work-group acquire/release.
+  auto release = [&](uint32_t j) -> bool {
+    if (kLds) {
+      const uint32_t sh = 8 * (j & 3);
+      const uint32_t old = __hip_atomic_fetch_sub(&cnt[j >> 2], 1u << sh, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+      return ((old >> sh) & 0xff) == 1;
+    }
+    return __hip_atomic_fetch_sub(&cnt[j], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1;
+  };
   if (threadIdx.x == 0) nA = 0;
+  {
+    // pull the records into this XCD's L2 (one dword per 128-byte line):
+    // the chains below then wait on L2 hits instead of memory
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(rec);
+    const size_t lines = ((size_t)C * sizeof(KillRec) + 127) / 128;
+    uint32_t acc = 0;
+    for (size_t l = threadIdx.x; l < lines; l += blockDim.x) acc ^= words[l * 32];
+    asm volatile("" ::"v"(acc));
+  }
+  if (kLds) {
+    for (uint32_t k = threadIdx.x; k < nv; k += blockDim.x) {
+      const size_t u = cverts[k];
+      sv[(size_t)k * 3] = v[u * 3];
+      sv[(size_t)k * 3 + 1] = v[u * 3 + 1];
+      sv[(size_t)k * 3 + 2] = v[u * 3 + 2];
+    }
+    for (uint32_t w = threadIdx.x; w < (C + 3) / 4; w += blockDim.x) {
+      uint32_t word = 0;
+      for (uint32_t b = 0; b < 4; b++)
+        if (w * 4 + b < C) word |= (rec[w * 4 + b].indeg & 0xff) << (8 * b);
+      cnt[w] = word;
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < C; i += blockDim.x) cnt[i] = rec[i].indeg;
+  }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < C; i += blockDim.x)
-    if (kill_ready(act, v_off, v_ent, heads, i)) listA[atomicAdd(&nA, 1u)] = i;
+    if (rec[i].indeg == 0) listA[atomicAdd(&nA, 1u)] = i;
   __syncthreads();
-  uint32_t round = 1;
-  for (; round <= C && nA > 0; round++) {
+  uint32_t round = 0;
+  while (nA > 0) {
+    round++;
     if (threadIdx.x == 0) nB = 0;
     __syncthreads();
     const uint32_t na = nA;
     for (uint32_t k = threadIdx.x; k < na; k += blockDim.x) {
-      uint32_t i = listA[k];
-      int own[4];
-      kill_verts(act[i], own);
-      for (uint32_t guard = 0; guard <= C; guard++) {
-        const Frozen c = act[i];
-        kill_apply(c, v);
-        int vs[4];
-        kill_verts(c, vs);
-        for (int s = 0; s < 4; s++)
-          if (vs[s] >= 0) ++heads[vs[s]];
-        // run ahead: a ready contact at the head of one of the owned
-        // vertices' lists whose cloth vertices are all owned
-        bool found = false;
-        uint32_t j = 0;
-        for (int s = 0; s < 4 && !found; s++) {
-          const int u = own[s];
-          if (u < 0) continue;
-          const uint32_t h = heads[u];
-          if (h >= v_off[u + 1]) continue;
-          j = v_ent[h] >> 3;
-          int ws[4];
-          kill_verts(act[j], ws);
-          bool inside = true;
-          for (int q = 0; q < 4 && inside; q++) {
-            if (ws[q] < 0) continue;
-            inside = ws[q] == own[0] || ws[q] == own[1] || ws[q] == own[2] || ws[q] == own[3];
+      const uint32_t first = listA[k];
+      KillRec c = rec[first];
+      // guard: a chain never revisits a contact, so it ends within C steps
+      for (uint32_t guard = 0; guard < C; guard++) {
+        // the chain goes on with one of the successors: fetch them all
+        // while this contact runs (an absent one reads a valid record)
+        const uint32_t self = first;
+        KillRec pre[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) pre[s] = rec[c.succ[s] != kNoSucc ? c.succ[s] : self];
+        kill_contact(c, vel);
+        if (kLds) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        int take = -1;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const uint32_t j = c.succ[s];
+          if (j == kNoSucc) continue;
+          // a contact that follows this one on two of its vertices is
+          // released twice
+          if (release(j)) {
+            if (take < 0) take = s;
+            else listB[atomicAdd(&nB, 1u)] = j;
           }
-          found = inside && kill_ready(act, v_off, v_ent, heads, j);
         }
-        if (!found) break;
-        i = j;
-      }
-      for (int s = 0; s < 4; s++) {
-        if (own[s] < 0) continue;
-        const uint32_t h = heads[own[s]];
-        if (h < v_off[own[s] + 1]) {
-          const uint32_t nxt = v_ent[h] >> 3;
-          if (atomicExch(&flag[nxt], round) != round) listB[atomicAdd(&nB, 1u)] = nxt;
-        }
+        if (take < 0) break;
+        if (kLds) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        if (take == 0) c = pre[0];
+        else if (take == 1) c = pre[1];
+        else if (take == 2) c = pre[2];
+        else c = pre[3];
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) nA = 0;
+    if (threadIdx.x == 0) nA = nB;
     __syncthreads();
-    const uint32_t nb = nB;
-    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) {
-      const uint32_t i = listB[k];
-      if (kill_ready(act, v_off, v_ent, heads, i)) listA[atomicAdd(&nA, 1u)] = i;
-    }
+    const uint32_t nb = nA;
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) listA[k] = listB[k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *rounds_out = round - 1;
+  if (kLds) {
+    for (uint32_t k = threadIdx.x; k < nv; k += blockDim.x) {
+      const size_t u = cverts[k];
+      v[u * 3] = sv[(size_t)k * 3];
+      v[u * 3 + 1] = sv[(size_t)k * 3 + 1];
+      v[u * 3 + 2] = sv[(size_t)k * 3 + 2];
+    }
+  }
+  if (threadIdx.x == 0) *rounds_out = round;
 }
 
-// checkTrial prims (contact-assembly.ts:497-535): exact distance at the trial
-// and CCD from step start; the first failure in candidate order decides
-// whether a CCD failure is counted (nan_key vs cross_key).
-__global__ void __launch_bounds__(kBlock) k_prims_trial(DGeom G, const double* __restrict__ xs,
-                                                        const double* __restrict__ xt,
+// checkTrial prims (contact-assembly.ts:497-535) per pair; pairs [0, np)
+// are cloth/cloth, then nsp cloth/static.  Per prim: exact distance at the
+// trial, the swept-box prefilter, then the head of the reference's CCD; the
+// few prims that need the cubic and sampling tail are queued for
+// k_trial_ccd.  With kFilter the lists are the superset and the kept pair
+// counts go to cs->pair_count / spair_count.
+template <bool kFilter>
+__global__ void __launch_bounds__(kBlock) k_trial_pairs(DGeom G, const double* __restrict__ xs,
+                                                        const double* __restrict__ xt, double pad,
                                                         const uint2* __restrict__ pairs, uint32_t np,
                                                         const uint2* __restrict__ spairs, uint32_t nsp,
-                                                        double* partials, CScalars* cs) {
-  const uint32_t total = np * 15 + nsp * 12;  // < 2^31, checked by the host
+                                                        const double* __restrict__ sbox, uint2* tail, CScalars* cs) {
+  const uint32_t total = np + nsp;
   double minD = __builtin_inf(), minT = __builtin_inf();
-  for (uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x; tid < total; tid += gridDim.x * blockDim.x) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const bool stat = i >= np;
+    bool keep = false, sep = true;
+    PairCorners P;
+    uint2 pr = make_uint2(0, 0);
+    if (i < total) {
+      pr = stat ? spairs[i - np] : pairs[i];
+      load_pair(G, xs, xt, pr, stat, P);
+      keep = kFilter ? pair_keep(P, stat, pad, sbox, pr.y) : true;
+      sep = pair_separated(P, G.d_min);
+    }
+    if (kFilter) {
+      const uint64_t mc = __ballot(keep && !stat), ms = __ballot(keep && stat);
+      if ((threadIdx.x % kWave) == 0) {
+        if (mc) atomicAdd(&cs->pair_count, (uint32_t)__popcll(mc));
+        if (ms) atomicAdd(&cs->spair_count, (uint32_t)__popcll(ms));
+      }
+    }
+    const int nk = stat ? 12 : 15;
+    // uniform trip count: the switch in prim_seg and the tail queue's
+    // wave-aggregated allocation see every lane
+    for (int k = 0; k < 15; k++) {
+      bool queue = false;
+      if (keep && k < nk) {
+        Seg4 S;
+        prim_seg<true>(P, stat, k, S);
+        const int kind = k < (stat ? 3 : 6) ? 0 : 1;
+        const Closest c1 = kind == 0 ? closest_vt<false>(S.x1[0], S.x1[1], S.x1[2], S.x1[3])
+                                     : closest_ee<false>(S.x1[0], S.x1[1], S.x1[2], S.x1[3]);
+        // the distance only matters when it can lower this thread's minimum
+        // (or is not finite): |r|^2 above minD^2 (1 + 1e-9) rules that out
+        const double r2 = norm2(c1);
+        if (!(minD >= 1e-100 && minD <= 1e100 && r2 <= 1e200 && r2 > minD * minD * (1 + 1e-9))) {
+          const double d = jsm::hypot3(c1.rx, c1.ry, c1.rz);
+          if (d < minD) minD = d;
+          if (!jsm::isfinite_(d)) atomicOr(&cs->nonfinite, 1u);
+        }
+        if (!sep && !prim_separated(S, kind, G.d_min)) {
+          double toi;
+          if (ccd_head_fast(S, kind, G.d_min, toi)) {
+            if (toi < minT) minT = toi;  // 0 or Infinity: no crossing, no failure
+          } else {
+            queue = true;
+          }
+        }
+      }
+      const uint32_t slot = wave_alloc(queue, &cs->tail_count);
+      if (queue) tail[slot] = make_uint2(i, (uint32_t)k);
+    }
+  }
+  block_min_to(minD, &cs->mink[5]);
+  block_min_to(minT, &cs->mink[6]);
+}
+
+// The CCD tail (cubic roots, sampled fallback) of the prims k_trial_pairs
+// queued; the queue length is read on the device.
+__global__ void __launch_bounds__(kBlock) k_trial_ccd(DGeom G, const double* __restrict__ xs,
+                                                      const double* __restrict__ xt,
+                                                      const uint2* __restrict__ pairs, uint32_t np,
+                                                      const uint2* __restrict__ spairs,
+                                                      const uint2* __restrict__ tail, CScalars* cs) {
+  const uint32_t cnt = cs->tail_count;
+  double minT = __builtin_inf();
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const uint2 it = tail[j];
+    const bool stat = it.x >= np;
+    const uint2 pr = stat ? spairs[it.x - np] : pairs[it.x];
+    PairCorners P;
+    for (int c = 0; c < 3; c++) {
+      P.v[c] = (int)G.idx[(size_t)pr.x * 3 + c];
+      P.v[3 + c] = stat ? -(int)G.sidx[(size_t)pr.y * 3 + c] - 2 : (int)G.idx[(size_t)pr.y * 3 + c];
+    }
     Prim q;
-    prim_of(G, pairs, np, spairs, nsp, tid, q);
+    pair_prim(G, pair_ids(P), pr, stat, (int)it.y, q);
     const int ids[4] = {q.kind == 0 ? q.p : q.a, q.kind == 0 ? q.a : q.b, q.kind == 0 ? q.b : q.c,
                         q.kind == 0 ? q.c : q.d};
     Seg4 S;
@@ -1273,43 +1604,9 @@ __global__ void __launch_bounds__(kBlock) k_prims_trial(DGeom G, const double* _
       vpos(G, xs, ids[s], S.x0[s]);
       vpos(G, xt, ids[s], S.x1[s]);
     }
-    const double d = q.kind == 0 ? closest_vt(S.x1[0], S.x1[1], S.x1[2], S.x1[3]).dist
-                                 : closest_ee(S.x1[0], S.x1[1], S.x1[2], S.x1[3]).dist;
-    if (d < minD) minD = d;
-    if (!jsm::isfinite_(d)) atomicOr(&cs->nonfinite, 1u);
-    // Swept boxes of the prim's two parts (VT: the point / the triangle; EE:
-    // edge / edge) further apart than the thickness, with a 1e-6 relative
-    // margin, mean no configuration on the segment x0 -> x1 comes within the
-    // thickness: the reference's CCD then returns Infinity on every branch
-    // (not resting, no validated root, no sample within the thickness).
-    const int split = q.kind == 0 ? 1 : 2;
-    double gap2 = 0;
-    for (int k = 0; k < 3; k++) {
-      double lo1 = __builtin_inf(), hi1 = -__builtin_inf(), lo2 = __builtin_inf(), hi2 = -__builtin_inf();
-      for (int s = 0; s < 4; s++) {
-        const double a0 = S.x0[s][k], a1 = S.x1[s][k];
-        const double l = fmin(a0, a1), h = fmax(a0, a1);
-        if (s < split) { lo1 = fmin(lo1, l); hi1 = fmax(hi1, h); }
-        else { lo2 = fmin(lo2, l); hi2 = fmax(hi2, h); }
-      }
-      const double g = fmax(fmax(lo2 - hi1, lo1 - hi2), 0.0);
-      gap2 += g * g;
-    }
-    const double thr = G.d_min * (1 + 1e-6);
-    bool all_finite = true;
-    for (int s = 0; s < 4; s++)
-      for (int k = 0; k < 3; k++) all_finite = all_finite && jsm::isfinite_(S.x0[s][k]) && jsm::isfinite_(S.x1[s][k]);
-    if (all_finite && gap2 > thr * thr) continue;  // toi = Infinity
     const double toi = q.kind == 0 ? vt_ccd(S, G.d_min) : ee_ccd(S, G.d_min);
-    if (jsm::isnan_(toi)) {
-      atomicMin((unsigned long long*)&cs->nan_key, (unsigned long long)prim_key(G, q));
-    } else {
-      if (toi < minT) minT = toi;
-      if (toi > 0 && toi < 1 - 1e-9)
-        atomicMin((unsigned long long*)&cs->cross_key, (unsigned long long)prim_key(G, q));
-    }
+    trial_toi(G, q, toi, minT, cs);
   }
-  block_min_to(minD, &cs->mink[5]);
   block_min_to(minT, &cs->mink[6]);
 }
 
@@ -1342,6 +1639,21 @@ static int bits_for(uint64_t v) {
   return b;
 }
 
+// Dynamic LDS available to the LDS velocity-filter sweep: the device's
+// opt-in per-work-group maximum (raised for that kernel), less its static LDS.
+static size_t kill_lds_limit() {
+  int dev = 0, optin = 0, base = 0;
+  CS_HIP(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+  CS_HIP(hipDeviceGetAttribute(&base, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  size_t lim = (size_t)std::max(optin, base);
+  if (lim > (size_t)base &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kill_run<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lim - 64)) != hipSuccess)
+    lim = (size_t)base;
+  return lim > 64 ? lim - 64 : 0;
+}
+
 struct ContactImpl {
   hipStream_t st;
   cs_contact_params prm;
@@ -1367,6 +1679,8 @@ struct ContactImpl {
   bool sup_valid = false;
   // candidates
   DevBuf<uint2> pairs, spairs;
+  DevBuf<uint2> tailq;  // (pair, prim) items for the CCD tail
+  size_t tail_cap = 0;
   uint32_t pair_cap = 0, spair_cap = 0, np = 0, nsp = 0;
   // active set
   DevBuf<Frozen> rec, act;
@@ -1378,7 +1692,9 @@ struct ContactImpl {
   DevBuf<uint32_t> evals, evals_t, v_off, v_ent, ent_vtx;
   // per-contact evaluation
   DevBuf<uint8_t> inr;
-  DevBuf<uint32_t> kflag, klistA, klistB;
+  DevBuf<KillRec> krec;           // velocity-filter sweep records
+  DevBuf<uint32_t> kcnt, klist;    // counters / frontier lists of the global-memory sweep
+  size_t kill_lds_max = 0;         // dynamic LDS the LDS sweep may use
   DevBuf<double> cg, fr, coeff, dd, hvsum, hterm;
   DevBuf<uint32_t> heads, cverts;
   // lagged friction table
@@ -1514,6 +1830,8 @@ struct ContactImpl {
     CS_HIP(hipMemsetAsync(cs.ptr, 0, sizeof(CScalars), st));
     v_off.alloc(n + 1);
     heads.alloc(n);
+    klist.alloc((size_t)2 * std::max<uint32_t>(n, 1));
+    kill_lds_max = kill_lds_limit();
     cverts.alloc(std::max<uint32_t>(n, 1));
     hvsum.alloc((size_t)std::max<uint32_t>(n, 1) * 3);
     lag_count.alloc(1);
@@ -1544,7 +1862,7 @@ struct ContactImpl {
     ekeys.alloc((size_t)cap * 4); ekeys_t.alloc((size_t)cap * 4);
     evals.alloc((size_t)cap * 4); evals_t.alloc((size_t)cap * 4); v_ent.alloc((size_t)cap * 4);
     ent_vtx.alloc((size_t)cap * 4); hterm.alloc((size_t)cap * 24);
-    inr.alloc(cap); kflag.alloc(cap); klistA.alloc(cap); klistB.alloc(cap);
+    inr.alloc(cap); krec.alloc(cap); kcnt.alloc(cap);
     cg.alloc(cap); fr.alloc((size_t)cap * 3); coeff.alloc(cap); dd.alloc(cap);
     act_cap = cap;
     ensure_sort((size_t)cap * 4);
@@ -1690,45 +2008,58 @@ struct ContactImpl {
 
   // buildCandidates (contact-assembly.ts:87-121) minus the expansion into
   // prims, which the prim kernels do on the fly.
-  void build_candidates(const double* x, double pad) {
-    const DGeom G = geom();
+  // Candidate pairs of buildCandidates (contact-assembly.ts:87-121) at x
+  // (swept from the step start, padded by `pad`) minus the expansion into
+  // prims, which the pair kernels do.  Returns true when the pairs are the
+  // step-segment superset, which the pair kernels then filter with the exact
+  // predicate; otherwise pairs / spairs hold the traversal's exact list.
+  struct PairSrc {
+    const uint2* p;
+    uint32_t np;
+    const uint2* s;
+    uint32_t nsp;
+    bool filter;
+  };
+
+  PairSrc candidate_source(const double* x, double pad) {
     clear_counters();
     CS_HIP(hipMemsetAsync(&cs.ptr->nan_key, 0xff, 2 * sizeof(uint64_t), st));
     if (m == 0) {
       np = 0;
       nsp = 0;
-      diag.candidate_pairs = 0;
-      return;
+      return PairSrc{nullptr, 0, nullptr, 0, false};
     }
-    if (sup_valid) {
-      grow_pairs(std::max(pair_cap, np_sup), std::max(spair_cap, nsp_sup));
-      CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 2 * sizeof(uint32_t), st));
-      if (np_sup)
-        hipLaunchKernelGGL(k_filter_pairs, dim3(red_grid(np_sup)), dim3(kBlock), 0, st, G, xstep, x, pad,
-                           pairs_sup.ptr, np_sup, pairs.ptr, pair_cap, &cs.ptr->pair_count);
-      if (nsp_sup)
-        hipLaunchKernelGGL(k_filter_spairs, dim3(red_grid(nsp_sup)), dim3(kBlock), 0, st, G, xstep, x, pad,
-                           sbox.ptr, spairs_sup.ptr, nsp_sup, spairs.ptr, spair_cap, &cs.ptr->spair_count);
-      CS_LAUNCH_CHECK();
-      sync_cs();
+    if (sup_valid) return PairSrc{pairs_sup.ptr, np_sup, spairs_sup.ptr, nsp_sup, true};
+    if (need_sort) sort_leaves(x);
+    traverse(x, nullptr, pad, pairs, pair_cap, spairs, spair_cap, np, nsp);
+    return PairSrc{pairs.ptr, np, spairs.ptr, nsp, false};
+  }
+
+  // The exact candidate counts after a pair kernel ran (scalars synced).
+  void candidate_counts(const PairSrc& src) {
+    if (src.filter) {
       np = hcs->pair_count;
       nsp = hcs->spair_count;
-    } else {
-      if (need_sort) sort_leaves(x);
-      traverse(x, nullptr, pad, pairs, pair_cap, spairs, spair_cap, np, nsp);
     }
     diag.candidate_pairs = (uint64_t)np * 15 + (uint64_t)nsp * 12;
-    if (diag.candidate_pairs >= (1ull << 31)) throw OverflowError("more than 2^31 candidate primitives");
   }
 
   void update_active_set(const double* x) {
-    build_candidates(x, prm.d_hat);
+    const PairSrc src = candidate_source(x, prm.d_hat);
     const DGeom G = geom();
-    const uint64_t total = (uint64_t)np * 15 + (uint64_t)nsp * 12;
+    const uint32_t total = src.np + src.nsp;
     for (int attempt = 0; attempt < 3; attempt++) {
-      CS_HIP(hipMemsetAsync(&cs.ptr->active_count, 0, sizeof(uint32_t), st));
-      hipLaunchKernelGGL(k_prims_active, dim3(red_grid(total)), dim3(kBlock), 0, st, G, x, pairs.ptr, np, spairs.ptr,
-                         nsp, rec.ptr, akeys.ptr, anke_raw.ptr, act_cap, dpart.ptr, cs.ptr);
+      CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 3 * sizeof(uint32_t), st));  // pair, spair, active counts
+      if (total) {
+        if (src.filter)
+          hipLaunchKernelGGL(k_active_pairs<true>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, x,
+                             prm.d_hat, src.p, src.np, src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr,
+                             act_cap, cs.ptr);
+        else
+          hipLaunchKernelGGL(k_active_pairs<false>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, x,
+                             prm.d_hat, src.p, src.np, src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr,
+                             act_cap, cs.ptr);
+      }
       if (has_floor)
         hipLaunchKernelGGL(k_floor_active, dim3(red_grid(n)), dim3(kBlock), 0, st, G, x, rec.ptr, akeys.ptr,
                            anke_raw.ptr, act_cap, dpart.ptr + 1024, cs.ptr);
@@ -1738,8 +2069,9 @@ struct ContactImpl {
       grow_active((uint32_t)(hcs->active_count * 1.25) + 1024);
       if (attempt == 2) throw OverflowError("active contact buffer could not be grown");
     }
+    candidate_counts(src);
     C = hcs->active_count;
-    double minD = total ? mink_value(0) : INFINITY;
+    double minD = mink_value(0);
     if (has_floor && mink_value(1) < minD) minD = mink_value(1);
     // sort the frozen multiset into the reference's candidate order
     if (C) {
@@ -1904,10 +2236,18 @@ struct ContactImpl {
   void kill_approach_velocity(const double* x, double* v) {
     update_active_set(x);
     if (!C) return;
-    CS_HIP(hipMemcpyAsync(heads.ptr, v_off.ptr, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-    CS_HIP(hipMemsetAsync(kflag.ptr, 0, (size_t)C * sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_kill_approach, dim3(1), dim3(1024), 0, st, act.ptr, C, v_off.ptr, v_ent.ptr, heads.ptr,
-                       kflag.ptr, klistA.ptr, klistB.ptr, v, &cs.ptr->kill_rounds);
+    // heads[] (n entries) serves as the vertex -> LDS slot map here
+    hipLaunchKernelGGL(k_kill_slots, dim3(grid_for(nv)), dim3(kBlock), 0, st, cverts.ptr, nv, heads.ptr);
+    hipLaunchKernelGGL(k_kill_rec, dim3(grid_for(C)), dim3(kBlock), 0, st, act.ptr, C, heads.ptr, krec.ptr);
+    hipLaunchKernelGGL(k_kill_links, dim3(grid_for(E)), dim3(kBlock), 0, st, v_off.ptr, v_ent.ptr, ent_vtx.ptr, E,
+                       krec.ptr);
+    const size_t lds = (size_t)nv * 3 * sizeof(double) + (size_t)((C + 3) / 4) * 4 + (size_t)nv * 2 * 4;
+    if (lds <= kill_lds_max)
+      hipLaunchKernelGGL(k_kill_run<true>, dim3(1), dim3(kKillBlock), lds, st, krec.ptr, C, cverts.ptr, nv, v,
+                         kcnt.ptr, klist.ptr, &cs.ptr->kill_rounds);
+    else
+      hipLaunchKernelGGL(k_kill_run<false>, dim3(1), dim3(kKillBlock), 0, st, krec.ptr, C, cverts.ptr, nv, v,
+                         kcnt.ptr, klist.ptr, &cs.ptr->kill_rounds);
     CS_LAUNCH_CHECK();
     sync_cs();
     diag.filter_rounds = hcs->kill_rounds;
@@ -1915,18 +2255,31 @@ struct ContactImpl {
   }
 
   bool check_trial(const double* xt) {
-    build_candidates(xt, prm.d_min);
+    const PairSrc src = candidate_source(xt, prm.d_min);
     const DGeom G = geom();
-    const uint64_t total = (uint64_t)np * 15 + (uint64_t)nsp * 12;
-    if (total)
-      hipLaunchKernelGGL(k_prims_trial, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, xt, pairs.ptr, np,
-                         spairs.ptr, nsp, dpart.ptr, cs.ptr);
+    const uint32_t total = src.np + src.nsp;
+    if (total) {
+      // every prim of every pair at most once in the tail queue
+      if ((size_t)total * 15 > tail_cap) {
+        tail_cap = (size_t)total * 15;
+        tailq.alloc(tail_cap);
+      }
+      if (src.filter)
+        hipLaunchKernelGGL(k_trial_pairs<true>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, xt, prm.d_min,
+                           src.p, src.np, src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
+      else
+        hipLaunchKernelGGL(k_trial_pairs<false>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, xt,
+                           prm.d_min, src.p, src.np, src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
+      hipLaunchKernelGGL(k_trial_ccd, dim3(256), dim3(kBlock), 0, st, G, xstep, xt, src.p, src.np, src.s, tailq.ptr,
+                         cs.ptr);
+    }
     if (has_floor)
       hipLaunchKernelGGL(k_floor_trial, dim3(red_grid(n)), dim3(kBlock), 0, st, G, xstep, xt, dpart.ptr + 2048, cs.ptr);
     CS_LAUNCH_CHECK();
     sync_cs();
-    double minDist = total ? mink_value(5) : INFINITY;
-    double minTOI = total ? mink_value(6) : INFINITY;
+    candidate_counts(src);
+    double minDist = mink_value(5);
+    double minTOI = mink_value(6);
     if (hcs->nonfinite & 1u) {
       diag.ccd_failures++;
       return false;

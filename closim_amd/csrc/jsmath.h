@@ -59,6 +59,7 @@ JSM_HD double hypot3(double a, double b, double c) {
   double ab[3] = {0, 0, 0};
   bool nan = false;
   double mx = 0;
+#pragma unroll
   for (int i = 0; i < 3; i++) {
     if (isnan_(v[i])) {
       nan = true;
@@ -71,6 +72,7 @@ JSM_HD double hypot3(double a, double b, double c) {
   if (nan) return from_bits(0x7ff8000000000000ull);
   if (mx == 0) return 0;
   double sum = 0, comp = 0;
+#pragma unroll
   for (int i = 0; i < 3; i++) {
     const double n = ab[i] / mx;
     const double summand = n * n - comp;
