@@ -7,16 +7,20 @@
 // Factors (per Newton iteration): one work-group per domain assembles the
 // dense local Hessian row by row (each thread owns the rows of one member
 // vertex, so every entry receives its triangle contributions in ascending
-// triangle order, as in the reference), embeds the Jacobi diagonal,
-// symmetrises, and runs the reference's left-looking Cholesky (row sums in
-// the reference's order, rows below the pivot in parallel).  Apply: one wave
-// per domain.  The FP64 apply keeps the reference's operation order (forward
-// substitution column by column, which subtracts in the same order as the
-// row loop; back substitution row by row); the FP32 apply used inside the
-// mixed-precision solve substitutes column by column both ways.
+// triangle order, as in the reference) and embeds the Jacobi diagonal; a
+// second kernel symmetrises while loading the lower triangle into LDS and
+// runs the reference's left-looking Cholesky there (every sum in the
+// reference's order, rows below the pivot in parallel), then stores the
+// factor packed by columns (FP64, and FP32 for the mixed solve).  Apply: one
+// wave per domain, each column read contiguously.  The FP64 apply keeps the
+// reference's operation order (forward substitution column by column, which
+// subtracts in the same order as the row loop; back substitution row by row,
+// products by the wave, subtractions in order by one lane); the FP32 apply
+// substitutes forward by columns and backward by wave dot products.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "elements.h"
@@ -29,9 +33,13 @@ struct SchwarzImpl {
   uint32_t n = 0, D = 0, dmax = 0, DW = 0;
   std::vector<uint32_t> h_domain_of, h_local_of;
   DevBuf<uint32_t> members, size, domain_of, local_of;
-  DevBuf<double> A;      // D * DW * DW, row stride dw of each domain
-  DevBuf<float> A32;     // FP32 copy of the factors
+  DevBuf<double> A;      // assembled local Hessians, D * DW * DW, row stride dw of each domain
+  DevBuf<double> L;      // factors, D * T (T = DW (DW + 1) / 2), packed lower triangle by columns
+  DevBuf<float> L32;     // FP32 copy of L for the mixed-precision apply (reciprocal diagonal)
+  DevBuf<float> L32T;    // the same packed by rows, for its back substitution
+  DevBuf<double> W;      // factorisation workspace when the triangle does not fit in LDS
   DevBuf<uint8_t> flag;  // 1 = Cholesky, 0 = Jacobi fallback
+  size_t lds_max = 0;    // dynamic LDS the factor kernel may use
 };
 
 // ---------------------------------------------------------------- host
@@ -228,14 +236,23 @@ struct SwArgs {
   const uint32_t* domain_of;
   const uint32_t* local_of;
   double* A;
-  float* A32;
+  double* L;
+  float* L32;
+  float* L32T;
+  double* W;
   uint8_t* flag;
 };
 
 __device__ __forceinline__ double* dom_mat(const SwArgs& s, uint32_t d) { return s.A + (size_t)d * s.DW * s.DW; }
+__host__ __device__ __forceinline__ size_t tri_size(uint32_t dw) { return (size_t)dw * (dw + 1) / 2; }
+// column j of a packed lower triangle stored by columns (rows j..dw-1)
+__device__ __forceinline__ size_t col_off(uint32_t dw, uint32_t j) { return (size_t)j * dw - (size_t)j * (j - 1) / 2; }
+// row i of a packed lower triangle stored by rows (columns 0..i)
+__device__ __forceinline__ size_t row_off(uint32_t i) { return (size_t)i * (i + 1) / 2; }
 
 // Local Hessian of domain d: membrane 3x3 blocks of intra-domain corner
-// pairs, Jacobi diagonal, symmetrised (schwarz.ts:170-210).
+// pairs and the Jacobi diagonal (schwarz.ts:170-203); the symmetrisation
+// (:204-210) happens as the factor kernel loads it.
 __global__ void __launch_bounds__(kSwBlock) k_sw_assemble(SwArgs s, DMesh me, DMat mat, const double* __restrict__ x,
                                                           const double* __restrict__ diag) {
   const uint32_t d = blockIdx.x;
@@ -264,151 +281,525 @@ __global__ void __launch_bounds__(kSwBlock) k_sw_assemble(SwArgs s, DMesh me, DM
     }
     for (int a = 0; a < 3; a++) A[(size_t)(l * 3 + a) * dw + (l * 3 + a)] += diag[(size_t)v * 3 + a];
   }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < dw; i += blockDim.x)
-    for (uint32_t j = i + 1; j < dw; j++) {
-      const double v = 0.5 * (A[(size_t)i * dw + j] + A[(size_t)j * dw + i]);
-      A[(size_t)i * dw + j] = v;
-      A[(size_t)j * dw + i] = v;
-    }
 }
 
-// Dense Cholesky in place (schwarz.ts:212-249), or the Jacobi fallback.
-__global__ void __launch_bounds__(kSwBlock) k_sw_cholesky(SwArgs s, const double* __restrict__ diag) {
-  const uint32_t d = blockIdx.x;
-  const uint32_t dv = s.size[d], dw = dv * 3;
-  double* L = dom_mat(s, d);
+// The FP32 factor of the mixed solve: packed by columns (forward
+// substitution) and by rows (back substitution), diagonal as reciprocal.
+__device__ __forceinline__ void store32(const SwArgs& s, uint32_t d, uint32_t dw, uint32_t i, uint32_t k, double v) {
+  const float f = i == k ? (float)(1.0 / v) : (float)v;
+  s.L32[(size_t)d * tri_size(s.DW) + col_off(dw, k) + (i - k)] = f;
+  s.L32T[(size_t)d * tri_size(s.DW) + row_off(i) + k] = f;
+}
+
+constexpr int kSwFactorBlock = 256;
+
+// Symmetrise (schwarz.ts:204-210), then the reference's left-looking
+// Cholesky (:212-235) or its Jacobi fallback (:236-249), one work-group per
+// domain.  The triangle being factored lives in LDS (kLds) or in the W
+// workspace, packed by columns, which is also the layout the substitutions
+// read: at step k thread i reads its row's L_it (consecutive threads,
+// consecutive addresses) and the broadcast L_kt.  Thread t owns rows t and
+// t + 256.  Row k's pivot sum  A_kk - sum_s L_ks^2  is kept by its owner as
+// a running value, updated right after each L_ks is computed, i.e. in the
+// reference's order s = 0, 1, ..., so that the pivot is ready when column k
+// starts.
+template <bool kLds>
+__global__ void __launch_bounds__(kSwFactorBlock) k_sw_factor(SwArgs s, const double* __restrict__ diag,
+                                                              int want32) {
+  extern __shared__ double sw_lds[];
   __shared__ int ok;
   __shared__ double tiny;
+  const uint32_t d = blockIdx.x;
+  const uint32_t dv = s.size[d], dw = dv * 3;
+  const double* A = dom_mat(s, d);
+  double* T = kLds ? sw_lds : s.W + (size_t)d * tri_size(s.DW);
+  constexpr uint32_t kRows = 2;  // rows per thread: dw <= 2 * kSwFactorBlock (checked by the host)
+  // load the lower triangle, symmetrised: s = 0.5 (A_cr + A_rc)
+  for (uint32_t c = 0; c < dw; c++)
+    for (uint32_t r = c + threadIdx.x; r < dw; r += blockDim.x)
+      T[col_off(dw, c) + (r - c)] =
+          r == c ? A[(size_t)r * dw + r] : 0.5 * (A[(size_t)c * dw + r] + A[(size_t)r * dw + c]);
+  __syncthreads();
   if (threadIdx.x == 0) {
     double scale = 0;
-    for (uint32_t i = 0; i < dw; i++) scale = jsm::jmax(scale, jsm::fabs_(L[(size_t)i * dw + i]));
+    for (uint32_t i = 0; i < dw; i++) scale = jsm::jmax(scale, jsm::fabs_(T[col_off(dw, i)]));
     tiny = 1e-20 * (1 + scale);
     ok = 1;
   }
+  double piv[kRows];
+#pragma unroll
+  for (uint32_t q = 0; q < kRows; q++) {
+    const uint32_t r = threadIdx.x + q * blockDim.x;
+    piv[q] = r < dw ? T[col_off(dw, r)] : 0.0;
+  }
   __syncthreads();
   for (uint32_t k = 0; k < dw; k++) {
-    if (threadIdx.x == 0) {
-      double sum = L[(size_t)k * dw + k];
-      for (uint32_t q = 0; q < k; q++) sum -= L[(size_t)k * dw + q] * L[(size_t)k * dw + q];
+    // the owner of row k finishes the pivot
+    if (k % blockDim.x == threadIdx.x) {
+      const double sum = piv[k / blockDim.x];
       if (!(sum > tiny)) ok = 0;
-      else L[(size_t)k * dw + k] = jsm::sqrt_(sum);
+      else T[col_off(dw, k)] = jsm::sqrt_(sum);
     }
     __syncthreads();
     if (!ok) break;
-    const double lkk = L[(size_t)k * dw + k];
-    for (uint32_t i = k + 1 + threadIdx.x; i < dw; i += blockDim.x) {
-      double s2 = L[(size_t)i * dw + k];
-      for (uint32_t q = 0; q < k; q++) s2 -= L[(size_t)i * dw + q] * L[(size_t)k * dw + q];
-      L[(size_t)i * dw + k] = s2 / lkk;
+    const double lkk = T[col_off(dw, k)];
+#pragma unroll
+    for (uint32_t q = 0; q < kRows; q++) {
+      const uint32_t i = threadIdx.x + q * blockDim.x;
+      if (i <= k || i >= dw) continue;
+      double s2 = T[col_off(dw, k) + (i - k)];
+      // L_it at off_t + (i - t), L_kt at off_t + (k - t), off_t = col_off(t);
+      // loads issued eight ahead of the (ordered) subtractions
+      size_t off = 0;
+      uint32_t t = 0;
+      for (; t + 8 <= k; t += 8) {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          a[u] = T[off + (i - (t + u))];
+          b[u] = T[off + (k - (t + u))];
+          off += dw - (t + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) s2 -= a[u] * b[u];
+      }
+      for (; t < k; t++) {
+        s2 -= T[off + (i - t)] * T[off + (k - t)];
+        off += dw - t;
+      }
+      const double lik = s2 / lkk;
+      T[col_off(dw, k) + (i - k)] = lik;
+      piv[q] -= lik * lik;
     }
     __syncthreads();
   }
-  if (!ok) {
-    // symmetric Jacobi-diagonal fallback (schwarz.ts:236-249)
-    for (uint32_t e = threadIdx.x; e < dw * dw; e += blockDim.x) L[e] = 0;
-    __syncthreads();
+  double* Lg = s.L + (size_t)d * tri_size(s.DW);
+  float* L32 = s.L32 + (size_t)d * tri_size(s.DW);
+  if (ok) {
+    for (size_t e = threadIdx.x; e < tri_size(dw); e += blockDim.x) Lg[e] = T[e];
+    if (want32)
+      for (uint32_t k = 0; k < dw; k++)
+        for (uint32_t i = k + threadIdx.x; i < dw; i += blockDim.x) store32(s, d, dw, i, k, T[col_off(dw, k) + (i - k)]);
+  } else {
+    // symmetric Jacobi-diagonal fallback: the inverse diagonal in the
+    // diagonal slots
     for (uint32_t i = threadIdx.x; i < dw; i += blockDim.x) {
       const uint32_t v = s.members[(size_t)d * s.dmax + i / 3];
       const double dd = diag[(size_t)v * 3 + i % 3];
-      L[(size_t)i * dw + i] = dd > 1e-12 ? 1 / dd : 0;
+      const double inv = dd > 1e-12 ? 1 / dd : 0;
+      Lg[col_off(dw, i)] = inv;
+      if (want32) {
+        L32[col_off(dw, i)] = (float)inv;
+        s.L32T[(size_t)d * tri_size(s.DW) + row_off(i) + i] = (float)inv;
+      }
     }
   }
   if (threadIdx.x == 0) s.flag[d] = ok ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kSwBlock) k_sw_to32(SwArgs s) {
+// The same factorisation right-looking, with the triangle in registers
+// (domains up to 16 NB rows): 256 threads as a 16 x 16 grid, thread
+// (tr, tc) owning elements (16 a + tr, 16 b + tc), a >= b.  Step t takes the
+// pivot, divides column t, and subtracts L_it L_kt from every element (i, k)
+// with k > t.  Each element therefore receives exactly the reference's
+// operations in the reference's order: its subtractions for s = 0, 1, ...,
+// k - 1 (mul, then sub), then the division by L_kk (or, on the diagonal, the
+// square root).  Only column t passes through LDS.
+template <int NB>
+__global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __restrict__ diag, int want32) {
+  __shared__ double colbuf[2][NB * 16];
+  __shared__ double red[256];
+  __shared__ int ok;
+  __shared__ double tiny;
   const uint32_t d = blockIdx.x;
   const uint32_t dv = s.size[d], dw = dv * 3;
-  const double* L = dom_mat(s, d);
-  float* L32 = s.A32 + (size_t)d * s.DW * s.DW;
-  for (uint32_t e = threadIdx.x; e < dw * dw; e += blockDim.x) L32[e] = (float)L[e];
-}
-
-// z = A_d^-1 r on every domain, FP64 (solveLocal + applySchwarz,
-// schwarz.ts:255-309).  One wave per domain; rows i of the domain live in
-// lane i % 64.
-__global__ void __launch_bounds__(kWave) k_sw_apply64(SwArgs s, const double* __restrict__ r, double* z,
-                                                      const int32_t* done) {
-  if (done && *done) return;
-  const uint32_t d = blockIdx.x;
-  const uint32_t dv = s.size[d], dw = dv * 3;
-  const double* M = dom_mat(s, d);
-  extern __shared__ double sh[];
-  double* y = sh;            // dw
-  double* sol = sh + s.DW;   // dw
-  const int lane = threadIdx.x;
-  const uint32_t* mem = s.members + (size_t)d * s.dmax;
-  if (s.flag[d]) {
-    // forward: y_i = (rhs_i - sum_{j<i} M_ij y_j) / M_ii, the subtractions in
-    // increasing j (column by column, rows in parallel)
-    for (uint32_t i = lane; i < dw; i += kWave) y[i] = r[(size_t)mem[i / 3] * 3 + i % 3];
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t j = 0; j < dw; j++) {
-      if ((uint32_t)lane == j % kWave) y[j] = y[j] / M[(size_t)j * dw + j];
-      __builtin_amdgcn_wave_barrier();
-      const double yj = y[j];
-      for (uint32_t i = j + 1 + (((uint32_t)lane - (j + 1)) & (kWave - 1)); i < dw; i += kWave)
-        y[i] -= M[(size_t)i * dw + j] * yj;
-      __builtin_amdgcn_wave_barrier();
+  const uint32_t tr = threadIdx.x % 16, tc = threadIdx.x / 16;
+  const double* A = dom_mat(s, d);
+  double e[NB][NB];
+  double dmax = 0;
+#pragma unroll
+  for (int a = 0; a < NB; a++)
+#pragma unroll
+    for (int b = 0; b <= a; b++) {
+      const uint32_t i = 16 * a + tr, k = 16 * b + tc;
+      double v = 0;
+      if (i < dw && k <= i) {
+        // symmetrised (schwarz.ts:204-210): 0.5 (A_ki + A_ik)
+        v = i == k ? A[(size_t)i * dw + i] : 0.5 * (A[(size_t)k * dw + i] + A[(size_t)i * dw + k]);
+        if (i == k) dmax = jsm::jmax(dmax, jsm::fabs_(v));
+      }
+      e[a][b] = v;
     }
-    // back: x_i = (y_i - sum_{j>i} M_ji x_j) / M_ii, j increasing, one lane
-    if (lane == 0) {
-      for (uint32_t ii = dw; ii-- > 0;) {
-        double acc = y[ii];
-        for (uint32_t j = ii + 1; j < dw; j++) acc -= M[(size_t)j * dw + ii] * sol[j];
-        sol[ii] = acc / M[(size_t)ii * dw + ii];
+  red[threadIdx.x] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // Math.max over |A_ii| (schwarz.ts:213-214): order-free, NaN propagates
+    double scale = 0;
+    for (int q = 0; q < 256; q++) scale = jsm::jmax(scale, red[q]);
+    tiny = 1e-20 * (1 + scale);
+    ok = 1;
+  }
+  __syncthreads();
+  bool alive = true;
+#pragma unroll
+  for (int bt = 0; bt < NB; bt++) {
+    for (uint32_t ct = 0; ct < 16 && alive; ct++) {
+      const uint32_t t = 16 * bt + ct;
+      if (t >= dw) break;
+      double* cb = colbuf[t & 1];
+      if (tr == ct && tc == ct) {
+        const double sum = e[bt][bt];
+        if (!(sum > tiny)) {
+          ok = 0;
+        } else {
+          e[bt][bt] = jsm::sqrt_(sum);
+          cb[t] = e[bt][bt];
+        }
+      }
+      __syncthreads();
+      if (!ok) {
+        alive = false;
+        break;
+      }
+      const double ltt = cb[t];
+      if (tc == ct) {
+#pragma unroll
+        for (int a = bt; a < NB; a++) {
+          const uint32_t i = 16 * a + tr;
+          if (i > t && i < dw) {
+            e[a][bt] = e[a][bt] / ltt;
+            cb[i] = e[a][bt];
+          }
+        }
+      }
+      __syncthreads();
+      double lr[NB];
+#pragma unroll
+      for (int a = bt; a < NB; a++) {
+        const uint32_t i = 16 * a + tr;
+        lr[a] = (i > t && i < dw) ? cb[i] : 0.0;
+      }
+#pragma unroll
+      for (int b = bt; b < NB; b++) {
+        const uint32_t k = 16 * b + tc;
+        if (k <= t || k >= dw) continue;
+        const double lk = cb[k];
+#pragma unroll
+        for (int a = b; a < NB; a++) {
+          const uint32_t i = 16 * a + tr;
+          if (i >= k && i < dw) e[a][b] -= lr[a] * lk;
+        }
       }
     }
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    for (uint32_t i = lane; i < dw; i += kWave) {
-      double acc = 0;
-      for (uint32_t j = 0; j < dw; j++) acc += M[(size_t)i * dw + j] * r[(size_t)mem[j / 3] * 3 + j % 3];
-      sol[i] = acc;
-    }
-    __builtin_amdgcn_wave_barrier();
   }
-  // out.fill(0) then out += sol for the (disjoint) members
-  for (uint32_t i = lane; i < dw; i += kWave) z[(size_t)mem[i / 3] * 3 + i % 3] = 0.0 + sol[i];
+  __syncthreads();
+  double* Lg = s.L + (size_t)d * tri_size(s.DW);
+  float* L32 = s.L32 + (size_t)d * tri_size(s.DW);
+  if (ok) {
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+      for (int b = 0; b <= a; b++) {
+        const uint32_t i = 16 * a + tr, k = 16 * b + tc;
+        if (i < dw && k <= i) {
+          Lg[col_off(dw, k) + (i - k)] = e[a][b];
+          if (want32) store32(s, d, dw, i, k, e[a][b]);
+        }
+      }
+  } else {
+    // symmetric Jacobi-diagonal fallback (schwarz.ts:236-249)
+    for (uint32_t i = threadIdx.x; i < dw; i += blockDim.x) {
+      const uint32_t v = s.members[(size_t)d * s.dmax + i / 3];
+      const double dd = diag[(size_t)v * 3 + i % 3];
+      const double inv = dd > 1e-12 ? 1 / dd : 0;
+      Lg[col_off(dw, i)] = inv;
+      if (want32) {
+        L32[col_off(dw, i)] = (float)inv;
+        s.L32T[(size_t)d * tri_size(s.DW) + row_off(i) + i] = (float)inv;
+      }
+    }
+  }
+  if (threadIdx.x == 0) s.flag[d] = ok ? 1 : 0;
 }
 
-// The same apply with the FP32 factors, both substitutions column by column.
-__global__ void __launch_bounds__(kWave) k_sw_apply32(SwArgs s, const float* __restrict__ r, float* z,
+// The substitutions keep row i of a domain in lane i % 64, register slot
+// i / 64 (NC slots cover the widest domain), and read every column of the
+// factor contiguously, in blocks of K columns: the loads of the next block
+// are issued before the current one is used, which hides the memory latency
+// behind K columns of work.
+__device__ __forceinline__ double readlane_f64(double x, uint32_t l) {
+  const long long u = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane_f32(float x, uint32_t l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
+template <class T>
+__device__ __forceinline__ T bcast(T x, uint32_t l) {
+  if constexpr (sizeof(T) == 8) return readlane_f64(x, l);
+  else return readlane_f32(x, l);
+}
+
+// this lane's slot holding row j (meaningful in lane j % 64 only).  The
+// slot is picked with integer masks: a compare-and-select chain over the
+// slots would be folded back into a variable index into the array, which
+// the compiler can only serve from scratch memory.
+template <class T>
+using BitsOf = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+
+template <int NC, class T>
+__device__ __forceinline__ T slot_of(const T (&v)[NC], uint32_t j) {
+  using U = BitsOf<T>;
+  const uint32_t c0 = j / kWave;
+  U x = 0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const U m = (U)0 - (U)(c0 == (uint32_t)c);
+    x |= __builtin_bit_cast(U, v[c]) & m;
+  }
+  return __builtin_bit_cast(T, x);
+}
+template <int NC, class T>
+__device__ __forceinline__ void set_slot(T (&v)[NC], uint32_t j, T x) {
+  using U = BitsOf<T>;
+  const uint32_t c0 = j / kWave;
+  const bool mine = (uint32_t)threadIdx.x == j % kWave;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const U m = (U)0 - (U)(mine && c0 == (uint32_t)c);
+    v[c] = __builtin_bit_cast(T, (__builtin_bit_cast(U, v[c]) & ~m) | (__builtin_bit_cast(U, x) & m));
+  }
+}
+
+// Column j's entries for this lane's rows i >= j (0 elsewhere, and for j
+// outside the domain).
+template <int NC, class T>
+__device__ __forceinline__ void load_col(const T* __restrict__ M, uint32_t dw, int j, T (&out)[NC]) {
+  const bool in = j >= 0 && (uint32_t)j < dw;
+  const T* col = M + (in ? col_off(dw, (uint32_t)j) : 0);
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int i = c * kWave + (int)threadIdx.x;
+    out[c] = (in && i >= j && (uint32_t)i < dw) ? col[i - j] : (T)0;
+  }
+}
+
+template <int NC, int K, class T>
+__device__ __forceinline__ void load_block(const T* __restrict__ M, uint32_t dw, int j0, int dir,
+                                           T (&b)[K][NC]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) load_col<NC, T>(M, dw, j0 + dir * k, b[k]);
+}
+
+// Forward substitution: y_j /= L_jj, then y_i -= L_ij y_j for i > j, column
+// by column; for each row the same operations in the same order as the
+// reference's row loop (schwarz.ts:262-268).
+template <int NC, int K, class T>
+__device__ __forceinline__ void forward_sub(const T* __restrict__ M, uint32_t dw, T (&y)[NC]) {
+  T cur[K][NC], nxt[K][NC];
+  load_block<NC, K, T>(M, dw, 0, 1, cur);
+  for (uint32_t jb = 0; jb < dw; jb += K) {
+    load_block<NC, K, T>(M, dw, (int)(jb + K), 1, nxt);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const uint32_t j = jb + k;
+      if (j < dw) {
+        // the owner lane divides; the quotient is broadcast
+        const T yj = bcast(slot_of<NC, T>(y, j) / slot_of<NC, T>(cur[k], j), j % kWave);
+        set_slot<NC, T>(y, j, yj);
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+          if (c * kWave + threadIdx.x > j) y[c] -= cur[k][c] * yj;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) cur[k][c] = nxt[k][c];
+  }
+}
+
+// z = A_d^-1 r on every domain, FP64, in the reference's operation order
+// (solveLocal + applySchwarz, schwarz.ts:255-309).  One wave per domain.
+// Back substitution row by row in decreasing i (:269-274): the wave forms
+// the products L_ji x_j, lane 0 subtracts them in increasing j from LDS.
+template <int NC>
+__global__ void __launch_bounds__(kWave) k_sw_apply64(SwArgs s, const double* __restrict__ r, double* z,
                                                       const int32_t* done) {
+  constexpr int K = NC <= 3 ? 8 : 4;
   if (done && *done) return;
   const uint32_t d = blockIdx.x;
   const uint32_t dv = s.size[d], dw = dv * 3;
-  const float* M = s.A32 + (size_t)d * s.DW * s.DW;
-  extern __shared__ float shf[];
-  float* y = shf;
+  const double* M = s.L + (size_t)d * tri_size(s.DW);
+  __shared__ __align__(16) double prod[NC * kWave];
   const int lane = threadIdx.x;
   const uint32_t* mem = s.members + (size_t)d * s.dmax;
+  double y[NC], sol[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t i = c * kWave + lane;
+    y[c] = i < dw ? r[(size_t)mem[i / 3] * 3 + i % 3] : 0.0;
+    sol[c] = 0;
+  }
   if (s.flag[d]) {
-    for (uint32_t i = lane; i < dw; i += kWave) y[i] = r[(size_t)mem[i / 3] * 3 + i % 3];
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t j = 0; j < dw; j++) {
-      if ((uint32_t)lane == j % kWave) y[j] = y[j] / M[(size_t)j * dw + j];
-      __builtin_amdgcn_wave_barrier();
-      const float yj = y[j];
-      for (uint32_t i = j + 1 + (((uint32_t)lane - (j + 1)) & (kWave - 1)); i < dw; i += kWave)
-        y[i] -= M[(size_t)i * dw + j] * yj;
-      __builtin_amdgcn_wave_barrier();
+    forward_sub<NC, K, double>(M, dw, y);
+    double cur[K][NC], nxt[K][NC];
+    load_block<NC, K, double>(M, dw, (int)dw - 1, -1, cur);
+    for (uint32_t ib = 0; ib < dw; ib += K) {
+      load_block<NC, K, double>(M, dw, (int)dw - 1 - (int)(ib + K), -1, nxt);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (ib + k >= dw) break;
+        const uint32_t ii = dw - 1 - (ib + k);
+        // products of rows below ii (others unused); lane 0 subtracts them
+        // in increasing row order, reading them from LDS eight at a time
+        // with the next eight already requested
+#pragma unroll
+        for (int c = 0; c < NC; c++) prod[c * kWave + lane] = cur[k][c] * sol[c];
+        const double yi = bcast(slot_of<NC, double>(y, ii), ii % kWave);
+        const double lii = bcast(slot_of<NC, double>(cur[k], ii), ii % kWave);
+        __builtin_amdgcn_wave_barrier();
+        double acc = yi;
+        if (lane == 0) {
+          uint32_t j = ii + 1;
+          if ((j & 1) && j < dw) acc -= prod[j++];
+          if (j + 8 <= dw) {
+            const double2* q = reinterpret_cast<const double2*>(prod + j);
+            double2 a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3];
+            for (; j + 16 <= dw; j += 8) {
+              q = reinterpret_cast<const double2*>(prod + j + 8);
+              const double2 b0 = q[0], b1 = q[1], b2 = q[2], b3 = q[3];
+              __builtin_amdgcn_sched_barrier(0);
+              acc -= a0.x; acc -= a0.y; acc -= a1.x; acc -= a1.y;
+              acc -= a2.x; acc -= a2.y; acc -= a3.x; acc -= a3.y;
+              a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+            }
+            acc -= a0.x; acc -= a0.y; acc -= a1.x; acc -= a1.y;
+            acc -= a2.x; acc -= a2.y; acc -= a3.x; acc -= a3.y;
+            j += 8;
+          }
+          for (; j < dw; j++) acc -= prod[j];
+        }
+        acc = bcast(acc, 0);
+        set_slot<NC, double>(sol, ii, acc / lii);
+        __builtin_amdgcn_wave_barrier();
+      }
+#pragma unroll
+      for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) cur[k][c] = nxt[k][c];
     }
-    for (uint32_t j = dw; j-- > 0;) {
-      if ((uint32_t)lane == j % kWave) y[j] = y[j] / M[(size_t)j * dw + j];
-      __builtin_amdgcn_wave_barrier();
-      const float xj = y[j];
-      for (uint32_t i = lane; i < j; i += kWave) y[i] -= M[(size_t)j * dw + i] * xj;
-      __builtin_amdgcn_wave_barrier();
-    }
-    for (uint32_t i = lane; i < dw; i += kWave) z[(size_t)mem[i / 3] * 3 + i % 3] = y[i];
   } else {
-    for (uint32_t i = lane; i < dw; i += kWave) {
-      float acc = 0;
-      for (uint32_t j = 0; j < dw; j++) acc += M[(size_t)i * dw + j] * r[(size_t)mem[j / 3] * 3 + j % 3];
-      z[(size_t)mem[i / 3] * 3 + i % 3] = acc;
+    // the fallback's dense product with the (diagonal) inverse, row by row
+    for (uint32_t j = 0; j < dw; j++) {
+      const double rj = r[(size_t)mem[j / 3] * 3 + j % 3];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const uint32_t i = c * kWave + lane;
+        const double inv = i < dw ? M[col_off(dw, i)] : 0.0;
+        sol[c] += (j == i ? inv : 0.0) * rj;
+      }
     }
+  }
+  // out.fill(0) then out += sol for the (disjoint) members
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t i = c * kWave + lane;
+    if (i < dw) z[(size_t)mem[i / 3] * 3 + i % 3] = 0.0 + sol[c];
+  }
+}
+
+// Row ii's entries for this lane's rows i <= ii of a triangle packed by rows.
+template <int NC, class T>
+__device__ __forceinline__ void load_row(const T* __restrict__ M, uint32_t dw, int ii, T (&out)[NC]) {
+  const bool in = ii >= 0 && (uint32_t)ii < dw;
+  const T* row = M + (in ? row_off((uint32_t)ii) : 0);
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int i = c * kWave + (int)threadIdx.x;
+    out[c] = (in && i <= ii) ? row[i] : (T)0;
+  }
+}
+
+// The same operator with the FP32 factors (mixed-precision inner solve), in
+// any order: both substitutions column-oriented (forward over L by columns,
+// back over L^T, i.e. L by rows), the diagonal stored as reciprocals, so a
+// step is one multiply, one broadcast and a multiply-subtract per row.
+template <int NC>
+__global__ void __launch_bounds__(kWave) k_sw_apply32(SwArgs s, const float* __restrict__ r, float* z,
+                                                      const int32_t* done) {
+  constexpr int K = NC <= 3 ? 16 : 8;
+  if (done && *done) return;
+  const uint32_t d = blockIdx.x;
+  const uint32_t dv = s.size[d], dw = dv * 3;
+  const float* M = s.L32 + (size_t)d * tri_size(s.DW);
+  const float* MT = s.L32T + (size_t)d * tri_size(s.DW);
+  const int lane = threadIdx.x;
+  const uint32_t* mem = s.members + (size_t)d * s.dmax;
+  float y[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t i = c * kWave + lane;
+    y[c] = i < dw ? r[(size_t)mem[i / 3] * 3 + i % 3] : 0.0f;
+  }
+  if (s.flag[d]) {
+    {
+      float cur[K][NC], nxt[K][NC];
+      load_block<NC, K, float>(M, dw, 0, 1, cur);
+      for (uint32_t jb = 0; jb < dw; jb += K) {
+        load_block<NC, K, float>(M, dw, (int)(jb + K), 1, nxt);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const uint32_t j = jb + k;
+          if (j < dw) {
+            const float yj = bcast(slot_of<NC, float>(y, j) * slot_of<NC, float>(cur[k], j), j % kWave);
+            set_slot<NC, float>(y, j, yj);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+              if (c * kWave + (uint32_t)lane > j) y[c] -= cur[k][c] * yj;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+          for (int c = 0; c < NC; c++) cur[k][c] = nxt[k][c];
+      }
+    }
+    float cur[K][NC], nxt[K][NC];
+#pragma unroll
+    for (int k = 0; k < K; k++) load_row<NC, float>(MT, dw, (int)dw - 1 - k, cur[k]);
+    for (uint32_t ib = 0; ib < dw; ib += K) {
+#pragma unroll
+      for (int k = 0; k < K; k++) load_row<NC, float>(MT, dw, (int)dw - 1 - (int)(ib + K) - k, nxt[k]);
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (ib + k < dw) {
+          const uint32_t ii = dw - 1 - (ib + k);
+          const float xi = bcast(slot_of<NC, float>(y, ii) * slot_of<NC, float>(cur[k], ii), ii % kWave);
+          set_slot<NC, float>(y, ii, xi);
+#pragma unroll
+          for (int c = 0; c < NC; c++)
+            if (c * kWave + (uint32_t)lane < ii) y[c] -= cur[k][c] * xi;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) cur[k][c] = nxt[k][c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const uint32_t i = c * kWave + lane;
+      if (i < dw) y[c] = M[col_off(dw, i)] * y[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const uint32_t i = c * kWave + lane;
+    if (i < dw) z[(size_t)mem[i / 3] * 3 + i % 3] = y[c];
   }
 }
 
@@ -426,7 +817,8 @@ Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size,
   S.D = (uint32_t)mem.size();
   for (auto& m : mem) S.dmax = std::max<uint32_t>(S.dmax, (uint32_t)m.size());
   S.DW = S.dmax * 3;
-  if (S.DW * 2 * sizeof(double) > 60 * 1024) throw std::invalid_argument("Schwarz domain too large for the apply kernel");
+  // the factor kernel keeps up to two rows per thread
+  if (S.DW > 2 * kSwFactorBlock) throw std::invalid_argument("Schwarz domains above 170 vertices are not supported");
   std::vector<uint32_t> flat((size_t)std::max<uint32_t>(S.D, 1) * std::max<uint32_t>(S.dmax, 1), 0xffffffffu),
       sz(std::max<uint32_t>(S.D, 1), 0);
   for (uint32_t k = 0; k < S.D; k++) {
@@ -438,7 +830,22 @@ Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size,
   S.domain_of.upload(S.h_domain_of.data(), S.n, st);
   S.local_of.upload(S.h_local_of.data(), S.n, st);
   S.A.alloc((size_t)std::max<uint32_t>(S.D, 1) * S.DW * S.DW);
+  S.L.alloc((size_t)std::max<uint32_t>(S.D, 1) * tri_size(S.DW));
   S.flag.alloc(std::max<uint32_t>(S.D, 1));
+  // the triangle being factored stays in LDS when it fits the device's
+  // per-work-group maximum; otherwise in a global workspace
+  int dev = 0, optin = 0, base = 0;
+  CS_HIP(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+  CS_HIP(hipDeviceGetAttribute(&base, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t need = tri_size(S.DW) * sizeof(double);
+  const size_t stat = 256;  // the kernel's own shared scalars, with room
+  S.lds_max = (size_t)base > stat ? (size_t)base - stat : 0;
+  if (need > S.lds_max && (size_t)optin > (size_t)base &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sw_factor<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)((size_t)optin - stat)) == hipSuccess)
+    S.lds_max = (size_t)optin - stat;
+  if (need > S.lds_max) S.W.alloc((size_t)std::max<uint32_t>(S.D, 1) * tri_size(S.DW));
 }
 
 Schwarz::~Schwarz() { delete im_; }
@@ -447,32 +854,73 @@ static SwArgs sw_args(SchwarzImpl& S) {
   SwArgs a;
   a.D = S.D; a.dmax = S.dmax; a.DW = S.DW;
   a.members = S.members.ptr; a.size = S.size.ptr; a.domain_of = S.domain_of.ptr; a.local_of = S.local_of.ptr;
-  a.A = S.A.ptr; a.A32 = S.A32.ptr; a.flag = S.flag.ptr;
+  a.A = S.A.ptr; a.L = S.L.ptr; a.L32 = S.L32.ptr; a.L32T = S.L32T.ptr; a.W = S.W.ptr; a.flag = S.flag.ptr;
   return a;
 }
 
 void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool want32) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
-  if (want32) S.A32.alloc((size_t)S.D * S.DW * S.DW);
+  if (want32 && !S.L32.ptr) {
+    S.L32.alloc((size_t)S.D * tri_size(S.DW));
+    S.L32T.alloc((size_t)S.D * tri_size(S.DW));
+  }
   const SwArgs a = sw_args(S);
   hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwBlock), 0, S.st, a, me, mat, x, diag);
-  hipLaunchKernelGGL(k_sw_cholesky, dim3(S.D), dim3(kSwBlock), 0, S.st, a, diag);
-  if (want32) hipLaunchKernelGGL(k_sw_to32, dim3(S.D), dim3(kSwBlock), 0, S.st, a);
+  const int w32 = want32 ? 1 : 0;
+  const uint32_t nb = (S.DW + 15) / 16;
+  const size_t need = tri_size(S.DW) * sizeof(double);
+#define SW_RL(C, NB)                                                                                         \
+  case C:                                                                                                    \
+    hipLaunchKernelGGL(k_sw_factor_rl<NB>, dim3(S.D), dim3(256), 0, S.st, a, diag, w32);                     \
+    break;
+  // (the 8-block instance would keep its tile in scratch memory; 9 blocks
+  // cover the same domains from registers)
+  switch (nb) {
+    SW_RL(1, 1) SW_RL(2, 2) SW_RL(3, 3) SW_RL(4, 4) SW_RL(5, 5) SW_RL(6, 6) SW_RL(7, 7) SW_RL(8, 9) SW_RL(9, 9)
+    SW_RL(10, 10) SW_RL(11, 11) SW_RL(12, 12)
+    default:
+      // wider domains: the triangle in LDS (or the workspace), left-looking
+      if (need <= S.lds_max)
+        hipLaunchKernelGGL(k_sw_factor<true>, dim3(S.D), dim3(kSwFactorBlock), need, S.st, a, diag, w32);
+      else
+        hipLaunchKernelGGL(k_sw_factor<false>, dim3(S.D), dim3(kSwFactorBlock), 0, S.st, a, diag, w32);
+  }
+#undef SW_RL
   CS_LAUNCH_CHECK();
 }
 
 void Schwarz::apply64(const double* r, double* z, const int32_t* done) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
-  hipLaunchKernelGGL(k_sw_apply64, dim3(S.D), dim3(kWave), 2 * S.DW * sizeof(double), S.st, sw_args(S), r, z, done);
+  const SwArgs a = sw_args(S);
+  switch ((S.DW + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL(k_sw_apply64<1>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 2: hipLaunchKernelGGL(k_sw_apply64<2>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 3: hipLaunchKernelGGL(k_sw_apply64<3>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 4: hipLaunchKernelGGL(k_sw_apply64<4>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 5: hipLaunchKernelGGL(k_sw_apply64<5>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 6: hipLaunchKernelGGL(k_sw_apply64<6>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 7: hipLaunchKernelGGL(k_sw_apply64<7>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    default: hipLaunchKernelGGL(k_sw_apply64<8>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+  }
   CS_LAUNCH_CHECK();
 }
 
 void Schwarz::apply32(const float* r, float* z, const int32_t* done) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
-  hipLaunchKernelGGL(k_sw_apply32, dim3(S.D), dim3(kWave), S.DW * sizeof(float), S.st, sw_args(S), r, z, done);
+  const SwArgs a = sw_args(S);
+  switch ((S.DW + kWave - 1) / kWave) {
+    case 1: hipLaunchKernelGGL(k_sw_apply32<1>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 2: hipLaunchKernelGGL(k_sw_apply32<2>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 3: hipLaunchKernelGGL(k_sw_apply32<3>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 4: hipLaunchKernelGGL(k_sw_apply32<4>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 5: hipLaunchKernelGGL(k_sw_apply32<5>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 6: hipLaunchKernelGGL(k_sw_apply32<6>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    case 7: hipLaunchKernelGGL(k_sw_apply32<7>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+    default: hipLaunchKernelGGL(k_sw_apply32<8>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
+  }
   CS_LAUNCH_CHECK();
 }
 
