@@ -51,6 +51,17 @@ def make_scene(name):
 
 PROFILE_NAME = {"sphere": "sphere_drape_256", "fold": "crumple_fold_512", "strip": "strip_pull"}
 
+
+def profile_traffic(workload):
+    """HBM bytes per k_mx_hvp launch from the committed counter summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if there is one."""
+    prof = os.path.join(ROOT, "profiles", f"k_mx_hvp_{PROFILE_NAME.get(workload, workload)}.json")
+    if not os.path.exists(prof):
+        return None, None
+    with open(prof) as f:
+        pj = json.load(f)
+    return pj["traffic_bytes_per_launch"], f"profiles/{pj['summary']} (FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated)"
+
 # Accuracy at convergence.  On the drape and fold sheets the reference's own
 # Newton loop does not converge at any cap (its FD Hessian-vector product
 # stalls at |g| ~ 5e-5, and with the analytic one the line search stalls on
@@ -86,6 +97,38 @@ def converged_comparison(steps, h):
         gm.close()
         gd.close()
     return out
+
+
+def fold_section(steps, h):
+    """The 512x512 self-contact fold (additive Schwarz on 64-vertex domains):
+    mixed and FP64 steps/s from the same initial state, one warmup step each,
+    and the mixed HVP roofline on that mesh."""
+    from closim_amd import _abi
+    popts = dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64))
+    res = {}
+    for prec in ("mixed", "fp64"):
+        sc = make_scene("fold")
+        g = solver_for(sc, prec, popts, timing=False)
+        raw_step(g, h)
+        t = timed_steps(g, h, steps)
+        st = g.lastStats
+        ms, nbytes = g.time_hvp(200)
+        g._copy_out()
+        res[prec] = dict(steps_per_s=steps / t, ms_per_step=1000 * t / steps, newton_used_last_step=st["newton_used"],
+                         pcg_iters_last_step=st["pcg_iters"], hvp_mean_us=1000 * ms,
+                         hvp_gbs=nbytes / (ms / 1000) / 1e9 if ms > 0 else 0.0,
+                         positions=g.getPositions().copy())
+        print(f"[bench] fold {prec}: {steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
+        g.close()
+    pm, pd = res["mixed"].pop("positions"), res["fp64"].pop("positions")
+    m = res["mixed"]
+    return dict(workload="self_contact_fold_512x512", vertices=262144, triangles=522242,
+                preconditioner="additive-schwarz 64/1/64", warmup=1, steps=steps,
+                mixed=m, fp64=res["fp64"], mixed_over_fp64=res["fp64"]["ms_per_step"] / m["ms_per_step"],
+                pos_rel_err_mixed_vs_fp64=float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd))),
+                roofline_hvp={"bound": "hbm", "achieved": m["hvp_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": m["hvp_gbs"] / HBM_PEAK_GBS, "kernel": "k_mx_hvp",
+                              "traffic": profile_traffic("fold")[0], "traffic_source": profile_traffic("fold")[1]})
 
 
 def solver_for(scene, precision, popts, timing):
@@ -180,6 +223,8 @@ def main():
                     help="HBM bytes per HVP launch from the FETCH_SIZE/WRITE_SIZE counter runs")
     ap.add_argument("--accuracy-steps", type=int, default=None,
                     help="steps of the converged small-scene comparison (default 20, 0 = skip)")
+    ap.add_argument("--fold-steps", type=int, default=3,
+                    help="timed steps of the 512x512 self-contact fold section on the sphere workload (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (for counter calibration runs)")
     args = ap.parse_args()
@@ -258,12 +303,8 @@ def main():
     achieved = hvp_bytes / mean_s / 1e9 if mean_s > 0 else 0.0
     traffic = args.traffic
     traffic_src = "--traffic" if traffic is not None else None
-    prof = os.path.join(ROOT, "profiles", f"k_mx_hvp_{PROFILE_NAME.get(args.workload, args.workload)}.json")
-    if traffic is None and os.path.exists(prof):
-        with open(prof) as f:
-            pj = json.load(f)
-        traffic = pj["traffic_bytes_per_launch"]
-        traffic_src = f"profiles/{pj['summary']} (FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated)"
+    if traffic is None:
+        traffic, traffic_src = profile_traffic(args.workload)
     # PCIe-inclusive rate (positions + velocities copied out each step)
     t0 = time.perf_counter()
     for _ in range(2):
@@ -300,6 +341,10 @@ def main():
         "pcie_inclusive_steps_per_s": pcie_rate,
     }
     out.update(res)
+    if args.workload == "sphere" and args.fold_steps > 0 and gd:
+        gm.close()
+        gd.close()
+        out["fold"] = fold_section(args.fold_steps, h)
     if args.copy_calibration:
         copy_calibration()
     if not args.no_cpu:

@@ -20,14 +20,17 @@ class Schwarz {
   // Domains from the rest positions (mesh.restPositions) and the triangles.
   Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st);
   ~Schwarz();
-  // buildSchwarzFactors (schwarz.ts:154-252) at x with the Jacobi diagonal;
-  // also keeps an FP32 copy of the factors when `want32`.
-  void factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool want32);
+  // buildSchwarzFactors (schwarz.ts:154-252) at x with the Jacobi diagonal:
+  // the FP64 factors in the reference's operation order, or (fp32) the FP32
+  // factors of the mixed-precision solve only.
+  void factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool fp32);
   // applySchwarz (schwarz.ts:290-309), FP64, in the reference's operation order.
   void apply64(const double* r, double* z, const int32_t* done);
   // The same operator with the FP32 factors (mixed-precision inner solve).
   void apply32(const float* r, float* z, const int32_t* done);
   uint32_t domain_count() const;
+  // 1 = Cholesky, 0 = Jacobi fallback, per domain, of the last factor call
+  void flags(uint8_t* out) const;
   uint32_t max_domain() const;
   // Host copies of the partition (parity probes).
   void partition(uint32_t* domain_of, uint32_t* local_of) const;

@@ -397,17 +397,37 @@ __global__ void __launch_bounds__(kSwFactorBlock) k_sw_factor(SwArgs s, const do
   if (threadIdx.x == 0) s.flag[d] = ok ? 1 : 0;
 }
 
+__device__ __forceinline__ double readlane_f64(double x, uint32_t l) {
+  const long long u = __builtin_bit_cast(long long, x);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane_f32(float x, uint32_t l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+}
 // The same factorisation right-looking, with the triangle in registers
 // (domains up to 16 NB rows): 256 threads as a 16 x 16 grid, thread
-// (tr, tc) owning elements (16 a + tr, 16 b + tc), a >= b.  Step t takes the
-// pivot, divides column t, and subtracts L_it L_kt from every element (i, k)
-// with k > t.  Each element therefore receives exactly the reference's
-// operations in the reference's order: its subtractions for s = 0, 1, ...,
-// k - 1 (mul, then sub), then the division by L_kk (or, on the diagonal, the
-// square root).  Only column t passes through LDS.
-template <int NB>
+// (tr, tc) owning elements (16 a + tr, 16 b + tc), a >= b.  Step t: the 16
+// threads of column t (one wave's lanes) take the pivot by a lane read and
+// divide the column, which goes to LDS; after one barrier every thread
+// subtracts L_it L_kt from its elements (i, k) with k > t.
+//
+// kF32 = false: the FP64 factor of the reference.  Each element receives
+// exactly the reference's operations in the reference's order: its
+// subtractions for s = 0, 1, ..., k - 1 (mul, then sub), then the division by
+// L_kk (on the diagonal, the square root), so it is bit-identical to the
+// left-looking loop of schwarz.ts:212-235.
+// kF32 = true: the FP32 factor of the mixed solve (reciprocal pivots).
+//
+// Elements above the diagonal and rows or columns past the domain are
+// updated too, with whatever the column buffer holds: they are never read
+// and never written out, and sparing them the per-element predicate halves
+// the update.
+template <int NB, bool kF32>
 __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __restrict__ diag, int want32) {
-  __shared__ double colbuf[2][NB * 16];
+  using T = typename std::conditional<kF32, float, double>::type;
+  __shared__ T colbuf[2][NB * 16];
   __shared__ double red[256];
   __shared__ int ok;
   __shared__ double tiny;
@@ -415,7 +435,7 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
   const uint32_t dv = s.size[d], dw = dv * 3;
   const uint32_t tr = threadIdx.x % 16, tc = threadIdx.x / 16;
   const double* A = dom_mat(s, d);
-  double e[NB][NB];
+  T e[NB][NB];
   double dmax = 0;
 #pragma unroll
   for (int a = 0; a < NB; a++)
@@ -428,7 +448,7 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
         v = i == k ? A[(size_t)i * dw + i] : 0.5 * (A[(size_t)k * dw + i] + A[(size_t)i * dw + k]);
         if (i == k) dmax = jsm::jmax(dmax, jsm::fabs_(v));
       }
-      e[a][b] = v;
+      e[a][b] = (T)v;
     }
   red[threadIdx.x] = dmax;
   __syncthreads();
@@ -440,20 +460,45 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
     ok = 1;
   }
   __syncthreads();
+  const double tiny_v = tiny;
   bool alive = true;
 #pragma unroll
   for (int bt = 0; bt < NB; bt++) {
     for (uint32_t ct = 0; ct < 16 && alive; ct++) {
       const uint32_t t = 16 * bt + ct;
       if (t >= dw) break;
-      double* cb = colbuf[t & 1];
-      if (tr == ct && tc == ct) {
-        const double sum = e[bt][bt];
-        if (!(sum > tiny)) {
-          ok = 0;
+      T* cb = colbuf[t & 1];
+      if (tc == ct) {
+        // lanes of column t; the pivot's owner is lane (17 ct) % 64
+        T sum;
+        if constexpr (kF32) sum = readlane_f32(e[bt][bt], (17 * ct) % kWave);
+        else sum = readlane_f64(e[bt][bt], (17 * ct) % kWave);
+        if (!((double)sum > tiny_v)) {
+          if (tr == ct) ok = 0;
         } else {
-          e[bt][bt] = jsm::sqrt_(sum);
-          cb[t] = e[bt][bt];
+          if constexpr (kF32) {
+            const float rp = 1.0f / sqrtf(sum);
+            if (tr == ct) e[bt][bt] = rp;
+#pragma unroll
+            for (int a = bt; a < NB; a++) {
+              const uint32_t i = 16 * a + tr;
+              if (i > t && i < dw) {
+                e[a][bt] = e[a][bt] * rp;
+                cb[i] = e[a][bt];
+              }
+            }
+          } else {
+            const double lt = jsm::sqrt_(sum);
+            if (tr == ct) e[bt][bt] = lt;
+#pragma unroll
+            for (int a = bt; a < NB; a++) {
+              const uint32_t i = 16 * a + tr;
+              if (i > t && i < dw) {
+                e[a][bt] = e[a][bt] / lt;
+                cb[i] = e[a][bt];
+              }
+            }
+          }
         }
       }
       __syncthreads();
@@ -461,40 +506,20 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
         alive = false;
         break;
       }
-      const double ltt = cb[t];
-      if (tc == ct) {
+      T lr[NB];
 #pragma unroll
-        for (int a = bt; a < NB; a++) {
-          const uint32_t i = 16 * a + tr;
-          if (i > t && i < dw) {
-            e[a][bt] = e[a][bt] / ltt;
-            cb[i] = e[a][bt];
-          }
-        }
-      }
-      __syncthreads();
-      double lr[NB];
-#pragma unroll
-      for (int a = bt; a < NB; a++) {
-        const uint32_t i = 16 * a + tr;
-        lr[a] = (i > t && i < dw) ? cb[i] : 0.0;
-      }
+      for (int a = bt; a < NB; a++) lr[a] = cb[16 * a + tr];
 #pragma unroll
       for (int b = bt; b < NB; b++) {
-        const uint32_t k = 16 * b + tc;
-        if (k <= t || k >= dw) continue;
-        const double lk = cb[k];
+        if (16 * b + tc <= t) continue;
+        const T lk = cb[16 * b + tc];
 #pragma unroll
-        for (int a = b; a < NB; a++) {
-          const uint32_t i = 16 * a + tr;
-          if (i >= k && i < dw) e[a][b] -= lr[a] * lk;
-        }
+        for (int a = b; a < NB; a++) e[a][b] -= lr[a] * lk;
       }
     }
   }
   __syncthreads();
-  double* Lg = s.L + (size_t)d * tri_size(s.DW);
-  float* L32 = s.L32 + (size_t)d * tri_size(s.DW);
+  const size_t base = (size_t)d * tri_size(s.DW);
   if (ok) {
 #pragma unroll
     for (int a = 0; a < NB; a++)
@@ -502,8 +527,13 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
       for (int b = 0; b <= a; b++) {
         const uint32_t i = 16 * a + tr, k = 16 * b + tc;
         if (i < dw && k <= i) {
-          Lg[col_off(dw, k) + (i - k)] = e[a][b];
-          if (want32) store32(s, d, dw, i, k, e[a][b]);
+          if constexpr (kF32) {
+            s.L32[base + col_off(dw, k) + (i - k)] = e[a][b];
+            s.L32T[base + row_off(i) + k] = e[a][b];
+          } else {
+            s.L[base + col_off(dw, k) + (i - k)] = e[a][b];
+            if (want32) store32(s, d, dw, i, k, e[a][b]);
+          }
         }
       }
   } else {
@@ -512,10 +542,10 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
       const uint32_t v = s.members[(size_t)d * s.dmax + i / 3];
       const double dd = diag[(size_t)v * 3 + i % 3];
       const double inv = dd > 1e-12 ? 1 / dd : 0;
-      Lg[col_off(dw, i)] = inv;
-      if (want32) {
-        L32[col_off(dw, i)] = (float)inv;
-        s.L32T[(size_t)d * tri_size(s.DW) + row_off(i) + i] = (float)inv;
+      if (!kF32) s.L[base + col_off(dw, i)] = inv;
+      if (kF32 || want32) {
+        s.L32[base + col_off(dw, i)] = (float)inv;
+        s.L32T[base + row_off(i) + i] = (float)inv;
       }
     }
   }
@@ -527,15 +557,6 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
 // factor contiguously, in blocks of K columns: the loads of the next block
 // are issued before the current one is used, which hides the memory latency
 // behind K columns of work.
-__device__ __forceinline__ double readlane_f64(double x, uint32_t l) {
-  const long long u = __builtin_bit_cast(long long, x);
-  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
-  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float readlane_f32(float x, uint32_t l) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
-}
 template <class T>
 __device__ __forceinline__ T bcast(T x, uint32_t l) {
   if constexpr (sizeof(T) == 8) return readlane_f64(x, l);
@@ -858,21 +879,34 @@ static SwArgs sw_args(SchwarzImpl& S) {
   return a;
 }
 
-void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool want32) {
+void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool fp32) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
-  if (want32 && !S.L32.ptr) {
+  if (fp32 && !S.L32.ptr) {
     S.L32.alloc((size_t)S.D * tri_size(S.DW));
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
   }
   const SwArgs a = sw_args(S);
   hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwBlock), 0, S.st, a, me, mat, x, diag);
-  const int w32 = want32 ? 1 : 0;
   const uint32_t nb = (S.DW + 15) / 16;
+  if (fp32 && nb <= 12) {
+#define SW_RL32(C, NB)                                                                                       \
+  case C:                                                                                                    \
+    hipLaunchKernelGGL((k_sw_factor_rl<NB, true>), dim3(S.D), dim3(256), 0, S.st, a, diag, 0);                        \
+    break;
+    switch (nb) {
+      SW_RL32(1, 1) SW_RL32(2, 2) SW_RL32(3, 3) SW_RL32(4, 4) SW_RL32(5, 5) SW_RL32(6, 6) SW_RL32(7, 7)
+      SW_RL32(8, 9) SW_RL32(9, 9) SW_RL32(10, 10) SW_RL32(11, 11) default: SW_RL32(12, 12)
+    }
+#undef SW_RL32
+    CS_LAUNCH_CHECK();
+    return;
+  }
+  const int w32 = fp32 ? 1 : 0;
   const size_t need = tri_size(S.DW) * sizeof(double);
 #define SW_RL(C, NB)                                                                                         \
   case C:                                                                                                    \
-    hipLaunchKernelGGL(k_sw_factor_rl<NB>, dim3(S.D), dim3(256), 0, S.st, a, diag, w32);                     \
+    hipLaunchKernelGGL((k_sw_factor_rl<NB, false>), dim3(S.D), dim3(256), 0, S.st, a, diag, w32);                     \
     break;
   // (the 8-block instance would keep its tile in scratch memory; 9 blocks
   // cover the same domains from registers)
@@ -888,6 +922,10 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
   }
 #undef SW_RL
   CS_LAUNCH_CHECK();
+}
+
+void Schwarz::flags(uint8_t* out) const {
+  if (im_->D) CS_HIP(hipMemcpy(out, im_->flag.ptr, im_->D, hipMemcpyDeviceToHost));
 }
 
 void Schwarz::apply64(const double* r, double* z, const int32_t* done) {

@@ -821,6 +821,40 @@ int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, i
   });
 }
 
+int cs_probe_schwarz(cs_solver* s, const double* x, const double* diag, const double* r, double* z64, double* z32,
+                     uint8_t* flags64, uint8_t* flags32, uint32_t* domains) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.schwarz) return fail(CS_E_INVALID, "solver was not created with the Schwarz preconditioner");
+  if (!x || !diag || !r) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.diag.ptr, diag, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.r.ptr, r, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    if (domains) *domains = S.schwarz->domain_count();
+    S.schwarz->factor(S.dmesh(), S.dmat(), S.x.ptr, S.diag.ptr, false);
+    S.schwarz->apply64(S.r.ptr, S.z.ptr, nullptr);
+    CS_HIP(hipStreamSynchronize(S.st));
+    if (flags64) S.schwarz->flags(flags64);
+    if (z64) CS_HIP(hipMemcpy(z64, S.z.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> h32(r, r + n3);
+    cloth::DevBuf<float> r32, o32;
+    r32.upload(h32.data(), n3, S.st);
+    o32.alloc(n3);
+    S.schwarz->factor(S.dmesh(), S.dmat(), S.x.ptr, S.diag.ptr, true);
+    S.schwarz->apply32(r32.ptr, o32.ptr, nullptr);
+    CS_HIP(hipStreamSynchronize(S.st));
+    if (flags32) S.schwarz->flags(flags32);
+    if (z32) {
+      CS_HIP(hipMemcpy(h32.data(), o32.ptr, n3 * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n3; i++) z32[i] = h32[i];
+    }
+    return CS_OK;
+  });
+}
+
 int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t* ids6, double* vals7,
                         uint64_t cap, uint64_t* count, uint64_t* n_candidates) {
   if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");

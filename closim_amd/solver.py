@@ -58,6 +58,8 @@ def load_library(path=LIB_PATH):
         "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
+                                       C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cs_probe_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, _dp]),
@@ -246,6 +248,22 @@ class HipSolver:
         if k > cap:
             return self.probe_active_set(x0, x, cap=int(k))
         return ids[:k].copy(), vals[:k].copy(), nc.value
+
+    def probe_schwarz(self, x, diag, r):
+        """buildSchwarzFactors + applySchwarz with this solver's domains:
+        (z64, z32, flags64, flags32)."""
+        self._need()
+        x, diag, r = (np.ascontiguousarray(a, np.float64) for a in (x, diag, r))
+        z64 = np.zeros_like(r)
+        z32 = np.zeros_like(r)
+        nd = C.c_uint32()
+        f64 = np.zeros(len(r) // 3 + 1, np.uint8)
+        f32 = np.zeros(len(r) // 3 + 1, np.uint8)
+        _check(self._lib.cs_probe_schwarz(self._h, x.ctypes.data_as(_dp), diag.ctypes.data_as(_dp),
+                                          r.ctypes.data_as(_dp), z64.ctypes.data_as(_dp), z32.ctypes.data_as(_dp),
+                                          f64.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          f32.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(nd)))
+        return z64, z32, f64[:nd.value].copy(), f32[:nd.value].copy()
 
     def probe_check_trial(self, x0, xt):
         self._need()

@@ -230,6 +230,15 @@ int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
 int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
                          uint32_t* local_of, uint32_t* domains);
 
+/* buildSchwarzFactors + applySchwarz (src/solver/schwarz.ts:154-309) with the
+ * solver's domains (created with CS_PRECOND_SCHWARZ) at positions x and
+ * Jacobi diagonal diag: z64 = the FP64 apply in the reference's order, z32 =
+ * the mixed solve's FP32 factors and apply (either may be NULL); flags64 /
+ * flags32 receive one byte per domain (1 Cholesky, 0 Jacobi fallback) and
+ * *domains the domain count. */
+int cs_probe_schwarz(cs_solver* s, const double* x, const double* diag, const double* r, double* z64, double* z32,
+                     uint8_t* flags64, uint8_t* flags32, uint32_t* domains);
+
 /* Begin a step at x0 and build the frozen active set at x (updateActiveSet,
  * contact-assembly.ts:164-220).  Writes min(count, cap) records in active-list
  * order: ids6 = (kind, p, a, b, c, d), vals7 = (w0, w1, w2, w3, nx, ny, nz). */

@@ -85,3 +85,42 @@ def test_small_strip_fp64_schwarz_large_domains_matches_oracle():
     kw = dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64), hvp_mode=_abi.CS_HVP_ANALYTIC)
     errs, g, o = run_pair(S.small_strip, 10, kw)
     assert max(errs) <= REL_TOL
+
+
+def _schwarz_case(sizes, seed, stretch=1.02):
+    sc = S.small_strip()
+    rng = np.random.default_rng(seed)
+    x = sc.positions * stretch + 1e-4 * rng.standard_normal(sc.positions.shape)
+    masses = np.repeat(np.asarray(sc.mesh.masses, np.float64), 3)
+    diag = masses * 3600.0 + 50.0 * (1 + rng.random(len(x)))
+    r = rng.standard_normal(len(x))
+    g = HipSolver("fp64", precond=_abi.CS_PRECOND_SCHWARZ, schwarz=sizes)
+    g.initialize(S.small_strip())
+    o = O.OracleSolver(S.small_strip())
+    return g, o, x, diag, r
+
+
+@pytest.mark.parametrize("sizes", [(16, 8, 32), (64, 1, 64), (1, 0, 1), (91, 1, 91)])
+def test_schwarz_apply_component_parity(sizes):
+    # buildSchwarzFactors + applySchwarz: the FP64 apply is the oracle's bit for
+    # bit; the mixed solve's FP32 factors and apply agree to FP32 accuracy
+    g, o, x, diag, r = _schwarz_case(sizes, 11)
+    z64, z32, f64, f32 = g.probe_schwarz(x, diag, r)
+    zo, fo = o.schwarz_apply(x, diag, r, *sizes)
+    assert np.array_equal(f64, fo) and np.all(f64 == 1)
+    assert np.array_equal(z64, zo)
+    assert np.array_equal(f32, fo)
+    err = np.max(np.abs(z32 - z64)) / np.max(np.abs(z64))
+    print(f"domains {len(f64)}, FP32 apply rel err {err:.2e}")
+    assert err < 1e-3
+
+
+def test_schwarz_apply_jacobi_fallback_parity():
+    g, o, x, diag, r = _schwarz_case((16, 8, 32), 12)
+    diag = -np.abs(diag)
+    diag[::7] = 0.0
+    z64, z32, f64, f32 = g.probe_schwarz(x, diag, r)
+    zo, fo = o.schwarz_apply(x, diag, r, 16, 8, 32)
+    assert np.all(fo == 0) and np.array_equal(f64, fo) and np.array_equal(f32, fo)
+    assert np.array_equal(z64, zo)
+    assert np.allclose(z32, zo, rtol=1e-6, atol=0)
