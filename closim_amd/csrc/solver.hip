@@ -30,6 +30,7 @@ struct NewtonScalars {
   double gtdx;
   double maxdx;
   double max_strain;
+  double adj[4];  // adjoint PCG: r.z, p.Ap, r.r, a.a
 };
 
 struct NonFinite {};
@@ -69,6 +70,56 @@ __global__ void k_init_diag(const float* mass, double inv_h2, double beta, doubl
 __global__ void k_velocity(const double* x, const double* x0, double h, double damp, double* v, size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n3) v[i] = ((x[i] - x0[i]) / h) * damp;  // newton.ts:222-226
+}
+
+// ---- adjoint kernels (step_adjoint) ----------------------------------------
+
+__global__ void k_adj_fd(const double* gp, const double* gm, double scale, double* out, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = (gp[i] - gm[i]) * scale;
+}
+
+__global__ void k_adj_shift(const double* x, const double* p, double s, double* out, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = x[i] + s * p[i];
+}
+
+__global__ void k_adj_pre(const double* r, const double* diag, const uint8_t* pinmask, double* z, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) z[i] = pinmask[i / 3] ? 0.0 : r[i] / diag[i];
+}
+
+__global__ void k_adj_axpy(double* y, double a, const double* x, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) y[i] += a * x[i];
+}
+
+__global__ void k_adj_xpby(double* p, const double* z, double beta, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) p[i] = z[i] + beta * p[i];
+}
+
+// right-hand side: dL/dx_{n+1} plus the velocity's share, free DOFs only
+__global__ void k_adj_rhs(const double* gx, const double* gv, double c, const uint8_t* pinmask, double* a,
+                          size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) a[i] = pinmask[i / 3] ? 0.0 : gx[i] + c * gv[i];
+}
+
+// -g on the free DOFs (the Newton right-hand side of the root polish)
+__global__ void k_neg_free(const double* g, const uint8_t* pinmask, double* out, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = pinmask[i / 3] ? 0.0 : -g[i];
+}
+
+// dL/dx_n = lambda M / h^2 - (damp / h) dL/dv_{n+1};  dL/dv_n = lambda M / h
+__global__ void k_adj_out(const double* lam, const double* gv, const float* mass, double h, double damp,
+                          double* dxp, double* dvp, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  const double m = mass[i / 3];
+  dxp[i] = lam[i] * m / (h * h) - (damp / h) * gv[i];
+  dvp[i] = lam[i] * m / h;
 }
 
 struct Solver {
@@ -291,12 +342,194 @@ struct Solver {
     return e;
   }
 
+  // ---- adjoint of one step ---------------------------------------------------
+  //
+  // The step returns the root x1 of g(x; x0, v0, theta) = M (x - yhat) / h^2 +
+  // grad E_int(x; theta) (+ contact) on the free DOFs, yhat = x0 + h v0 +
+  // h^2 gravity, and v1 = damp (x1 - x0) / h.  For a loss L(x1, v1) the
+  // implicit function theorem gives, with H = dg/dx at x1 and
+  // a = dL/dx1 + (damp / h) dL/dv1 on the free DOFs,
+  //   H lambda = a,   dL/dtheta_j = -lambda . dg/dtheta_j,
+  //   dL/dx0 = lambda M / h^2 - (damp / h) dL/dv1,   dL/dv0 = lambda M / h.
+  // H p is the central difference of the whole Newton gradient along p (so it
+  // includes the FD bending gradient the step uses, which the reference's
+  // membrane-only HVP leaves out), solved by Jacobi PCG (full_pcg) to `tol`
+  // relative residual.  The membrane energy is linear in C00, C11, C01, G and the
+  // hinge energy in kb = (bendWarp + bendWeft) / 2, so dg/dtheta_j is the
+  // internal gradient of the material with theta_j = 1 and the others 0.
+  // With contact, the active set is frozen at x1 (lagged friction as the
+  // solver holds it); the velocity filter is not differentiated.
+  DevBuf<double> adj_x1, adj_lam, adj_r, adj_z, adj_p, adj_ap, adj_gp, adj_gm, adj_xs, adj_gx, adj_gv, adj_a;
+
+  void adj_gradient(const double* xx, double* out) {
+    const double* cg = nullptr;
+    if (contact) {
+      contact->energy_grad(xx, ceg.ptr);
+      cg = ceg.ptr;
+    }
+    launch_gradient(st, dmesh(), dmat(), xx, yhat.ptr, 1.0 / (hadj * hadj), cg, out, partials.ptr,
+                    &ds()->grad_sumsq);
+  }
+  double hadj = 0;
+
+  void adj_alloc() {
+    const size_t n3 = (size_t)n * 3;
+    for (DevBuf<double>* b : {&adj_x1, &adj_lam, &adj_r, &adj_z, &adj_p, &adj_ap, &adj_gp, &adj_gm, &adj_xs,
+                              &adj_gx, &adj_gv, &adj_a})
+      b->alloc(n3);
+  }
+
+  // H p at `base` by the central difference of the Newton gradient along p
+  // (1e-6 m along p / |p|_inf); H includes every term of the gradient.
+  void full_hvp(const double* base, const double* p, double* out) {
+    const size_t n3 = (size_t)n * 3;
+    launch_max_abs(st, p, n3, mpartials.ptr, &ds()->maxdx);
+    const double pmax = read(&ds()->maxdx);
+    if (!(pmax > 0)) {
+      CS_HIP(hipMemsetAsync(out, 0, n3 * sizeof(double), st));
+      return;
+    }
+    const double eps = 1e-6;  // metres along p / |p|_inf
+    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, eps / pmax, adj_xs.ptr, n3);
+    adj_gradient(adj_xs.ptr, adj_gp.ptr);
+    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, -eps / pmax, adj_xs.ptr, n3);
+    adj_gradient(adj_xs.ptr, adj_gm.ptr);
+    hipLaunchKernelGGL(k_adj_fd, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr,
+                       pmax / (2 * eps), out, n3);
+    CS_LAUNCH_CHECK();
+  }
+
+  double adj_dot(const double* a, const double* b, double* slot) {
+    launch_dot(st, a, b, (size_t)n * 3, partials.ptr, slot);
+    return read(slot);
+  }
+
+  // Jacobi PCG (the `diag` buffer) on H(base) sol = rhs over the free DOFs,
+  // from sol = 0, to `tol` relative residual; returns the relative residual.
+  // `rhs` must be zero on pinned DOFs.  Uses adj_r/z/p/ap/xs/gp/gm.
+  double full_pcg(const double* base, const double* rhs, double* sol, double tol, int max_iters, int* iters) {
+    const size_t n3 = (size_t)n * 3;
+    CS_HIP(hipMemsetAsync(sol, 0, n3 * sizeof(double), st));
+    CS_HIP(hipMemcpyAsync(adj_r.ptr, rhs, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const double bnorm2 = adj_dot(rhs, rhs, &ds()->adj[3]);
+    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, diag.ptr, pinmask.ptr,
+                       adj_z.ptr, n3);
+    CS_HIP(hipMemcpyAsync(adj_p.ptr, adj_z.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double rz = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
+    int it = 0;
+    double rel = bnorm2 > 0 ? 1.0 : 0.0;
+    while (bnorm2 > 0 && it < max_iters) {
+      full_hvp(base, adj_p.ptr, adj_ap.ptr);
+      const double pap = adj_dot(adj_p.ptr, adj_ap.ptr, &ds()->adj[1]);
+      if (!(pap > 0)) break;  // not positive definite along p
+      const double alpha = rz / pap;
+      hipLaunchKernelGGL(k_adj_axpy, dim3(grid_for(n3)), dim3(kBlock), 0, st, sol, alpha, adj_p.ptr, n3);
+      hipLaunchKernelGGL(k_adj_axpy, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, -alpha, adj_ap.ptr, n3);
+      it++;
+      rel = std::sqrt(adj_dot(adj_r.ptr, adj_r.ptr, &ds()->adj[2]) / bnorm2);
+      if (rel <= tol) break;
+      hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, diag.ptr, pinmask.ptr,
+                         adj_z.ptr, n3);
+      const double rz_new = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
+      hipLaunchKernelGGL(k_adj_xpby, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_p.ptr, adj_z.ptr, rz_new / rz, n3);
+      rz = rz_new;
+    }
+    if (iters) *iters = it;
+    return rel;
+  }
+
+  void step_adjoint(double h, const double* hx0, const double* hv0, const double* hx1, const double* hgx,
+                    const double* hgv, double tol, int max_iters, double* hdx0, double* hdv0, double* hdtheta,
+                    int* iters_out, double* resid_out) {
+    upload_pins();
+    const size_t n3 = (size_t)n * 3;
+    adj_alloc();
+    hadj = h;
+    CS_HIP(hipMemcpyAsync(x0.ptr, hx0, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(adj_gv.ptr, hv0, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_predictor, dim3(grid_for(n)), dim3(kBlock), 0, st, x0.ptr, adj_gv.ptr, h, gravity[0],
+                       gravity[1], gravity[2], yhat.ptr, xt.ptr, n);
+    CS_HIP(hipMemcpyAsync(adj_x1.ptr, hx1, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(adj_gx.ptr, hgx, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(adj_gv.ptr, hgv, n3 * sizeof(double), hipMemcpyHostToDevice, st));
+    if (contact) {
+      contact->begin_step(x0.ptr);
+      contact->update_active_set(adj_x1.ptr);
+    }
+    const double damp = 1 - jsm::jmin(jsm::jmax(mat.damping, 0), 0.1);
+    hipLaunchKernelGGL(k_adj_rhs, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gx.ptr, adj_gv.ptr, damp / h,
+                       pinmask.ptr, adj_a.ptr, n3);
+    // Jacobi preconditioner: the step's diagonal M / h^2 + beta (+ contact)
+    const double invH2 = 1 / (h * h);
+    const double beta =
+        jsm::jmax(jsm::jmax(mat.stretch_warp, mat.stretch_weft), mat.shear) * mat.thickness * 0.1 + 1e-6;
+    hipLaunchKernelGGL(k_init_diag, dim3(grid_for(n)), dim3(kBlock), 0, st, mass.ptr, invH2, beta, diag.ptr, n);
+    if (contact) contact->add_diag_estimate(diag.ptr, adj_x1.ptr);
+    CS_LAUNCH_CHECK();
+    int it = 0;
+    const double rel = full_pcg(adj_x1.ptr, adj_a.ptr, adj_lam.ptr, tol, max_iters, &it);
+    if (iters_out) *iters_out = it;
+    if (resid_out) *resid_out = rel;
+    // parameter sensitivities: dg/dtheta_j = internal gradient with theta_j = 1
+    const DMat base = dmat();
+    const DMat unit[5] = {{base.thick, 1, 0, 0, 0, 0}, {base.thick, 0, 1, 0, 0, 0}, {base.thick, 0, 0, 1, 0, 0},
+                          {base.thick, 0, 0, 0, 1, 0}, {base.thick, 0, 0, 0, 0, 1}};
+    for (int j = 0; j < 5; j++) {
+      launch_gradient(st, dmesh(), unit[j], adj_x1.ptr, yhat.ptr, 0.0, nullptr, adj_gp.ptr, partials.ptr,
+                      &ds()->grad_sumsq);
+      hdtheta[j] = -adj_dot(adj_lam.ptr, adj_gp.ptr, &ds()->adj[0]);
+    }
+    hipLaunchKernelGGL(k_adj_out, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_lam.ptr, adj_gv.ptr, mass.ptr, h,
+                       damp, adj_gp.ptr, adj_gm.ptr, n3);
+    CS_LAUNCH_CHECK();
+    CS_HIP(hipMemcpyAsync(hdx0, adj_gp.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipMemcpyAsync(hdv0, adj_gm.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+  }
+
+  // ---- root polish (CS_FLAG_POLISH_ROOT; not in the reference) ----------------
+  //
+  // Near the root the Armijo test compares energies whose difference is below
+  // their round-off and rejects every trial, which ends Newton at a gradient
+  // norm set by the energy's magnitude rather than by grad_tol.  With the flag,
+  // an iteration whose every Armijo trial failed takes instead a full Newton
+  // step of the complete Hessian (full_pcg: the central difference of the
+  // whole gradient, bending included, which the step's own HVP leaves out),
+  // kept when it passes the contact check and lowers the Newton gradient
+  // norm; Newton then goes on until grad_tol, newton_iters or a step that
+  // lowers nothing.  Differentiable rollouts need this: the implicit-function
+  // gradient is the derivative of the exact root.
+  bool polish_step(double h, double gradNorm, double& energy) {
+    const size_t n3 = (size_t)n * 3;
+    adj_alloc();
+    hadj = h;
+    // Newton direction of the complete Hessian: H(x) d = -g(x)
+    hipLaunchKernelGGL(k_neg_free, dim3(grid_for(n3)), dim3(kBlock), 0, st, g.ptr, pinmask.ptr, adj_a.ptr, n3);
+    CS_LAUNCH_CHECK();
+    (void)full_pcg(x.ptr, adj_a.ptr, adj_lam.ptr, 1e-8, std::max(opt.pcg_max_iters, 1000), nullptr);
+    launch_trial(st, x.ptr, adj_lam.ptr, 1.0, xt.ptr, n3);
+    enforce_pins(xt.ptr);
+    if (contact) {
+      contact->set_segment(x.ptr, xt.ptr);
+      if (!contact->check_trial(xt.ptr)) return false;
+    }
+    adj_gradient(xt.ptr, adj_gp.ptr);
+    sync_scalars();
+    const double gNew = std::sqrt(hscal->grad_sumsq);
+    if (!(gNew < gradNorm)) return false;
+    std::swap(x.ptr, xt.ptr);
+    energy = total_energy(x.ptr, h);
+    if (contact) contact->commit(x.ptr);
+    return true;
+  }
+
   // ---- the step --------------------------------------------------------------
   cs_step_stats step(double h) {
     upload_pins();
     const size_t n3 = (size_t)n * 3;
     const int newtonIters = opt.newton_iters, pcgMaxIters = opt.pcg_max_iters, lsIters = opt.line_search_iters;
     const double gradTol = opt.grad_tol;
+    const bool polish = (opt.flags & CS_FLAG_POLISH_ROOT) != 0;
     CS_HIP(hipMemcpyAsync(x0.ptr, pos.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_predictor, dim3(grid_for(n)), dim3(kBlock), 0, st, x0.ptr, vel.ptr, h, gravity[0],
                        gravity[1], gravity[2], yhat.ptr, x.ptr, n);
@@ -413,6 +646,7 @@ struct Solver {
         if (contact) contact->add_rejection();
         alpha *= 0.5;
       }
+      if (!accepted && polish && polish_step(h, gradNorm, energy)) continue;
       if (!accepted) break;
       if (alpha < 1e-6) break;
     }
@@ -679,6 +913,22 @@ int cs_step(cs_solver* s, double dt) {
     CS_HIP(hipSetDevice(s->s.device));
     s->s.last = s->s.step(dt);
     s->s.has_last = true;
+    return CS_OK;
+  });
+}
+
+int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double* v_prev, const double* x_next,
+                    const double* dL_dx_next, const double* dL_dv_next, double tol, int max_iters,
+                    double* dL_dx_prev, double* dL_dv_prev, double* dL_dtheta, int* iters, double* rel_residual) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CS_E_INVALID, "dt must be positive and finite");
+  if (!x_prev || !v_prev || !x_next || !dL_dx_next || !dL_dv_next || !dL_dx_prev || !dL_dv_prev || !dL_dtheta)
+    return fail(CS_E_INVALID, "null argument");
+  if (!(tol > 0) || max_iters < 1) return fail(CS_E_INVALID, "tol must be positive and max_iters >= 1");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(s->s.device));
+    s->s.step_adjoint(dt, x_prev, v_prev, x_next, dL_dx_next, dL_dv_next, tol, max_iters, dL_dx_prev, dL_dv_prev,
+                      dL_dtheta, iters, rel_residual);
     return CS_OK;
   });
 }

@@ -58,6 +58,8 @@ def load_library(path=LIB_PATH):
         "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
+                                      _dp, _dp, C.POINTER(C.c_int), _dp]),
         "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
@@ -207,6 +209,22 @@ class HipSolver:
         _check(self._lib.cs_set_state(self._h, x.ctypes.data_as(_dp), v.ctypes.data_as(_dp), len(x)))
         self.scene.positions[:] = x
         self.scene.velocities[:] = v
+
+    def step_adjoint(self, dt, x_prev, v_prev, x_next, dL_dx_next, dL_dv_next, tol=1e-10, max_iters=4000):
+        """Adjoint of one step (cs_step_adjoint): returns dL/dx_prev, dL/dv_prev,
+        dL/d(stretchWarp, stretchWeft, stretchCoupling, shear, kb), and the
+        adjoint solve's iteration count and relative residual."""
+        self._need()
+        a = [np.ascontiguousarray(q, np.float64) for q in (x_prev, v_prev, x_next, dL_dx_next, dL_dv_next)]
+        dx = np.zeros_like(a[0])
+        dv = np.zeros_like(a[0])
+        dth = np.zeros(5)
+        it = C.c_int()
+        rr = C.c_double()
+        _check(self._lib.cs_step_adjoint(self._h, dt, *[q.ctypes.data_as(_dp) for q in a], tol, int(max_iters),
+                                         dx.ctypes.data_as(_dp), dv.ctypes.data_as(_dp), dth.ctypes.data_as(_dp),
+                                         C.byref(it), C.byref(rr)))
+        return dx, dv, dth, it.value, rr.value
 
     def kernel_timing(self):
         self._need()

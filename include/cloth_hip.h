@@ -56,6 +56,12 @@ enum { CS_HVP_FD = 0, CS_HVP_ANALYTIC = 1 };
 enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 2 };
 
 #define CS_FLAG_TIMING 1u   /* record per-kernel HIP event timings (cs_kernel_timing) */
+/* Root polish (not in the reference): when every Armijo trial of a Newton
+ * iteration fails, take the full (trust-scaled) step anyway if it passes the
+ * contact check and lowers the Newton gradient norm, and keep iterating.  The
+ * energy test stops resolving decreases near the root; rollouts that are
+ * differentiated with cs_step_adjoint need the root to grad_tol. */
+#define CS_FLAG_POLISH_ROOT 2u
 
 /* ClothMaterial (src/physics/types.ts:1-14). */
 typedef struct cs_material {
@@ -223,6 +229,23 @@ int cs_reset_timing(cs_solver* s);
  * resident, bracketed by HIP events on the solver stream; `bytes` receives
  * the algorithmic bytes of one launch.  Requires a previous cs_step. */
 int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
+
+/* Adjoint of one step (no reference counterpart: the reference has no
+ * differentiation).  Given the state before the step (x_prev, v_prev), the
+ * state the step produced (x_next; the velocity follows from it), the same
+ * pins and material as that step, and a loss's gradients dL/dx_next,
+ * dL/dv_next (3n each), solves H lambda = dL/dx_next + (damp/dt) dL/dv_next on
+ * the free DOFs (H = d(Newton gradient)/dx at x_next, applied as a central
+ * difference of the step's own gradient; Jacobi PCG to relative residual
+ * tol, at most max_iters iterations) and writes dL/dx_prev, dL/dv_prev (3n)
+ * and dL_dtheta[5] = dL/d(stretchWarp, stretchWeft, stretchCoupling, shear,
+ * kb) with kb = (bendWarp + bendWeft) / 2, the hinge stiffness of
+ * src/physics/fem.ts:44.  *iters / *rel_residual report the solve.  Contact,
+ * when the scene has it, enters with the active set frozen at x_next; the
+ * velocity filter is not differentiated. */
+int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double* v_prev, const double* x_next,
+                    const double* dL_dx_next, const double* dL_dv_next, double tol, int max_iters,
+                    double* dL_dx_prev, double* dL_dv_prev, double* dL_dtheta, int* iters, double* rel_residual);
 
 /* ---- component probes for parity tests (no reference counterpart) ------- */
 /* buildSchwarzDomains (src/solver/schwarz.ts:44-140) on the host, without a
