@@ -202,3 +202,22 @@ def converged_solver(scene, precision="fp64", grad_tol=1e-13, newton_iters=200, 
                   pcg_max_iters=pcg_max_iters, flags=_abi.CS_FLAG_POLISH_ROOT, **kw)
     g.initialize(scene)
     return g
+
+
+def material_problem(nx=127, side=1.0, seed=0, guess_spread=0.2):
+    """The material-parameter workload: an (nx+1)^2 sheet of side `side`
+    pinned at two corners; true parameters drawn log-uniformly (C00, C11 in
+    [1e5, 1e7], G in [5e4, 5e6], kb in [1e-6, 1e-4], C01 = 0) from a seeded
+    generator, and an initial guess that scales each drawn parameter by
+    1 + U[-spread, spread].  Returns (scene, theta_true, theta_guess)."""
+    from .scenes import STRIP_MATERIAL
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi):
+        return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+    theta_true = np.array([logu(1e5, 1e7), logu(1e5, 1e7), 0.0, logu(5e4, 5e6), logu(1e-6, 1e-4)])
+    theta_guess = theta_true * (1 + rng.uniform(-guess_spread, guess_spread, 5))
+    scene = corner_pinned_sheet(nx, side, material=material_with(STRIP_MATERIAL, theta_true))
+    return scene, theta_true, theta_guess
+

@@ -65,6 +65,8 @@ inline unsigned grid_for(size_t n, int block = kBlock) {
 // partial layout (and thus the reduction tree) deterministic.
 constexpr unsigned kMaxRedBlocks = 1024;  // partial slots per reduction
 constexpr unsigned kMaxHvpBlocks = 4096;  // HVP kernels: more blocks to hide gather latency
+constexpr unsigned kXcds = 8;             // MI355X: 8 XCDs, work-groups dispatched round-robin
+static_assert(kMaxHvpBlocks % kXcds == 0, "HVP grids are whole multiples of the XCD count");
 inline unsigned red_grid(size_t n, int block = kBlock) {
   size_t g = (n + block - 1) / block;
   if (g > kMaxRedBlocks) g = kMaxRedBlocks;

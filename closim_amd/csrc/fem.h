@@ -35,6 +35,12 @@ struct DMesh {
   const uint32_t* vh_off;
   const uint32_t* vh_rec;
   const uint8_t* pinmask;  // 1 = vertex is pinned (makePinFilter, scene.ts:46-55)
+  // vt_corner regrouped into 64-lane chunks that each hold whole vertices
+  // (padding 0xffffffff, a vertex without triangles one 0xfffffffe lane);
+  // null when some vertex has more than 64 corners
+  const uint4* ck_corner;
+  uint32_t ck_count;   // chunks
+  uint32_t ck_maxdeg;  // largest lane count of one vertex
 };
 
 struct DMat {
@@ -52,6 +58,8 @@ struct DMesh32 {
   const uint32_t* vt_rec;
   const uint4* vt_corner;
   const uint8_t* pinmask;
+  const uint4* ck_corner;
+  uint32_t ck_count, ck_maxdeg;
 };
 
 struct Scalars;  // device scalar block (solver.hip)

@@ -91,7 +91,8 @@ __device__ __forceinline__ void ld3f(const float* x, uint32_t v, float* o) {
 }
 
 // Analytic membrane HVP corner contribution (membrane-hvp.ts:18-75) in FP32.
-__device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const float* p, float* out) {
+__device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const float* p, float* out,
+                                             float* pown = nullptr) {
   const uint32_t i0 = cr.x, i1 = cr.y, i2 = cr.z, t = cr.w >> 2;
   const int corner = cr.w & 3;
   float x0[3], x1[3], x2[3], v0[3], v1[3], v2[3];
@@ -127,6 +128,9 @@ __device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const fl
     const float g2 = sc * (dP0 * ic + dP1 * id);
     out[k] = corner == 0 ? -(g1 + g2) : corner == 1 ? g1 : g2;
   }
+  if (pown)
+#pragma unroll
+    for (int k = 0; k < 3; k++) pown[k] = corner == 0 ? v0[k] : corner == 1 ? v1[k] : v2[k];
 }
 
 // Frozen barrier HVP term of one incidence entry: coeff (J.p) times the
@@ -349,6 +353,99 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
   block_partial_f(pap, mpart(a, MX_PAP));
 }
 
+// The same product over the chunked corner records: one lane per corner,
+// whole vertices per wave, so no lane waits on the vertex offsets before its
+// record load.  A segmented inclusive scan over the lanes of each vertex
+// (fixed order) leaves the vertex's sum in its last lane, which adds the
+// contact sums and the mass term and writes Ap.
+struct CkLane {
+  uint32_t vid;
+  bool pad, self, last;
+  float mh;
+  bool pinned;
+  uint32_t cb, ce;
+};
+
+__device__ __forceinline__ CkLane ck_lane(const MxArgs& a, uint4 cr, int lane) {
+  CkLane L;
+  L.pad = cr.w == 0xffffffffu;
+  L.self = cr.w == 0xfffffffeu;
+  const int corner = cr.w & 3;
+  L.vid = L.pad ? 0xffffffffu : (L.self || corner == 0) ? cr.x : corner == 1 ? cr.y : cr.z;
+  const uint32_t next = __shfl_down(L.vid, 1, kWave);
+  L.last = !L.pad && (lane == kWave - 1 || next != L.vid);
+  L.mh = 0.f;
+  L.pinned = false;
+  L.cb = L.ce = 0;
+  // the last lane's per-vertex operands, issued before the corner gathers
+  if (L.last) {
+    L.mh = a.me.mass[L.vid] * a.inv_h2;
+    L.pinned = a.me.pinmask[L.vid] != 0;
+    if (a.scan) {
+      L.cb = a.v_off[L.vid];
+      L.ce = a.v_off[L.vid + 1];
+    }
+  }
+  return L;
+}
+
+__device__ __forceinline__ float ck_finish(const MxArgs& a, const CkLane& L, int lane, float* t, const float* pv) {
+  float cs[3] = {0.f, 0.f, 0.f};
+  if (L.cb < L.ce)
+    for (uint32_t k = L.cb / kWave; k <= (L.ce - 1) / kWave; k++) {
+      const size_t e = (size_t)min(L.ce, (k + 1) * (uint32_t)kWave) - 1;
+      cs[0] += a.scan[e * 3];
+      cs[1] += a.scan[e * 3 + 1];
+      cs[2] += a.scan[e * 3 + 2];
+    }
+  for (uint32_t d = 1; d < a.me.ck_maxdeg; d <<= 1) {
+    const uint32_t uu = __shfl_up(L.vid, d, kWave);
+    float y[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) y[i] = __shfl_up(t[i], d, kWave);
+    if (lane >= (int)d && uu == L.vid) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) t[i] += y[i];
+    }
+  }
+  float pap = 0.f;
+  if (L.last) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const float ap = L.pinned ? 0.f : L.mh * pv[i] + (t[i] + cs[i]);
+      a.Ap[(size_t)L.vid * 3 + i] = ap;
+      pap += pv[i] * ap;
+    }
+  }
+  return pap;
+}
+
+// Work-groups are dispatched to the 8 XCDs round-robin (block b on XCD
+// b % 8), each XCD with its own L2.  Block b takes the contiguous chunk range
+// of logical block (b % 8) * (G / 8) + b / 8, so each XCD walks one eighth of
+// the vertex order and its x/p gathers stay within that eighth (and its
+// halo) in its own L2 (measured on the 512x512 fold: L2 hits 50k -> 211k,
+// misses 498k -> 234k per launch).  G is a multiple of 8.
+__global__ void __launch_bounds__(kBlock) k_mx_hvp_ck(MxArgs a) {
+  if (block_latched(&a.s->done)) return;
+  const int lane = threadIdx.x % kWave;
+  const uint32_t wpb = kBlock / kWave;
+  const uint32_t G = gridDim.x;
+  const uint32_t lb = (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds;
+  const uint32_t cpb = (a.me.ck_count + G - 1) / G;
+  const uint32_t c_end = min(a.me.ck_count, (lb + 1) * cpb);
+  float pap = 0;
+  for (uint32_t c = lb * cpb + threadIdx.x / kWave; c < c_end; c += wpb) {
+    const uint4 cr = a.me.ck_corner[(size_t)c * kWave + lane];
+    const CkLane L = ck_lane(a, cr, lane);
+    float t[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f};
+    if (L.self) ld3f(a.p, L.vid, pv);
+    else if (!L.pad) hvp_corner32(a, cr, a.p, t, pv);
+    pap += ck_finish(a, L, lane, t, pv);
+  }
+  block_partial_f(pap, mpart(a, MX_PAP));
+}
+
 __global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned pap_count) {
   MxScalars* s = a.s;
   if (block_latched(&s->done)) return;
@@ -435,8 +532,19 @@ struct MixedImpl {
 };
 
 static unsigned mx_grid(uint32_t n) { return red_grid(n); }
-static unsigned mx_hvp_grid(uint32_t n) {
-  return (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
+static unsigned mx_hvp_grid(const DMesh32& me) {
+  if (me.ck_corner) {
+    const size_t per_block = kBlock / kWave;
+    const size_t g = std::min<size_t>(kMaxHvpBlocks, (me.ck_count + per_block - 1) / per_block);
+    return (unsigned)((g + kXcds - 1) / kXcds * kXcds);  // a multiple of the XCD count
+  }
+  return (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)me.n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
+}
+
+static void launch_mx_hvp(const MxArgs& a, hipStream_t st) {
+  const dim3 gh(mx_hvp_grid(a.me)), b(kBlock);
+  if (a.me.ck_corner) hipLaunchKernelGGL(k_mx_hvp_ck, gh, b, 0, st, a);
+  else hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, 0);
 }
 
 MixedSolve::MixedSolve(const cs_scene_desc& d, const cs_options&, hipStream_t st) : im_(new MixedImpl) {
@@ -468,8 +576,9 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
                      in.me.pinmask, M.x32.ptr, M.r.ptr, M.d32.ptr, in.binv ? M.binv32.ptr : nullptr, M.sol.ptr);
   CS_LAUNCH_CHECK();
   MxArgs a;
-  a.me = DMesh32{in.me.n, in.me.m, in.me.idx, in.me.inv_dm, in.me.area, in.me.mass, in.me.vt_off, in.me.vt_rec,
-                 in.me.vt_corner, in.me.pinmask};
+  a.me = DMesh32{in.me.n,         in.me.m,       in.me.idx,       in.me.inv_dm,   in.me.area,
+                 in.me.mass,      in.me.vt_off,  in.me.vt_rec,    in.me.vt_corner, in.me.pinmask,
+                 in.me.ck_corner, in.me.ck_count, in.me.ck_maxdeg};
   a.c00 = (float)in.mat.c00; a.c11 = (float)in.mat.c11; a.c01 = (float)in.mat.c01; a.g = (float)in.mat.g;
   a.thick = (float)in.mat.thick;
   a.inv_h2 = (float)in.inv_h2;
@@ -488,7 +597,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.v_ent = in.contact.v_ent;
   M.last = a;
   M.has_last = true;
-  const dim3 g(mx_grid(n)), gh(mx_hvp_grid(n)), b(kBlock);
+  const dim3 g(mx_grid(n)), b(kBlock);
   hipLaunchKernelGGL(k_mx_init, g, b, 0, st, a);
   if (in.sw) {
     in.sw->apply32(a.r, a.z, nullptr);
@@ -500,9 +609,9 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   for (int k = 0; k < in.max_iters; k++) {
     if (E) hipLaunchKernelGGL(k_mx_cscan, dim3(grid_for(E)), b, 0, st, a);
     if (timer) timer(ctx, k, true);
-    hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, k);
+    launch_mx_hvp(a, st);
     if (timer) timer(ctx, k, false);
-    hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k, mx_hvp_grid(n));
+    hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k, mx_hvp_grid(a.me));
     if (in.sw) {
       in.sw->apply32(a.r, a.z, &M.s.ptr->done);
       hipLaunchKernelGGL(k_mx_ext_z, g, b, 0, st, a, 0);
@@ -534,10 +643,9 @@ double MixedSolve::time_hvp(int reps) {
   hipEvent_t e0, e1;
   CS_HIP(hipEventCreate(&e0));
   CS_HIP(hipEventCreate(&e1));
-  const dim3 gh(mx_hvp_grid(M.n)), b(kBlock);
-  hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, M.last, 0);
+  launch_mx_hvp(M.last, st);
   CS_HIP(hipEventRecord(e0, st));
-  for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, M.last, 0);
+  for (int i = 0; i < reps; i++) launch_mx_hvp(M.last, st);
   CS_HIP(hipEventRecord(e1, st));
   CS_LAUNCH_CHECK();
   CS_HIP(hipEventSynchronize(e1));

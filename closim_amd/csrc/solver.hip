@@ -135,7 +135,8 @@ struct Solver {
 
   // mesh
   DevBuf<uint32_t> idx, vt_off, vt_rec, hv, vh_off, vh_rec;
-  DevBuf<uint4> vt_corner;
+  DevBuf<uint4> vt_corner, ck_corner;
+  uint32_t ck_count = 0, ck_maxdeg = 0;
   DevBuf<float> inv_dm, area, mass;
   DevBuf<double> hrest, hlen, hasum;
   // state
@@ -183,6 +184,9 @@ struct Solver {
     d.hv = hv.ptr; d.hrest = hrest.ptr; d.hlen = hlen.ptr; d.hasum = hasum.ptr;
     d.vh_off = vh_off.ptr; d.vh_rec = vh_rec.ptr;
     d.pinmask = pinmask.ptr;
+    d.ck_corner = ck_count ? ck_corner.ptr : nullptr;
+    d.ck_count = ck_count;
+    d.ck_maxdeg = ck_maxdeg;
     return d;
   }
   DMat dmat() const {
@@ -274,6 +278,33 @@ struct Solver {
         cor[q] = make_uint4(d->indices[t * 3], d->indices[t * 3 + 1], d->indices[t * 3 + 2], rec[q]);
       }
       vt_corner.upload(cor.data(), cor.size(), st);
+      // the same records in 64-lane chunks of whole vertices (the mixed HVP)
+      std::vector<uint4> ck;
+      ck.reserve(cor.size() + cor.size() / 8 + 64);
+      uint32_t fill = 0, maxdeg = 1;
+      bool ok = true;
+      for (uint32_t v = 0; v < n && ok; v++) {
+        const uint32_t deg = off[v + 1] - off[v], need = std::max<uint32_t>(deg, 1);
+        if (need > kWave) {
+          ok = false;
+          break;
+        }
+        if (fill + need > kWave) {
+          for (; fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+          fill = 0;
+        }
+        if (deg == 0) ck.push_back(make_uint4(v, v, v, 0xfffffffeu));
+        else
+          for (uint32_t q = off[v]; q < off[v + 1]; q++) ck.push_back(cor[q]);
+        fill += need;
+        maxdeg = std::max(maxdeg, need);
+      }
+      if (ok && n) {
+        for (; fill > 0 && fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+        ck_corner.upload(ck.data(), ck.size(), st);
+        ck_count = (uint32_t)(ck.size() / kWave);
+        ck_maxdeg = maxdeg;
+      }
     }
     if (nh) {
       hv.upload(d->hinge_verts, (size_t)nh * 4, st);
@@ -495,9 +526,9 @@ struct Solver {
   // an iteration whose every Armijo trial failed takes instead a full Newton
   // step of the complete Hessian (full_pcg: the central difference of the
   // whole gradient, bending included, which the step's own HVP leaves out),
-  // kept when it passes the contact check and lowers the Newton gradient
-  // norm; Newton then goes on until grad_tol, newton_iters or a step that
-  // lowers nothing.  Differentiable rollouts need this: the implicit-function
+  // kept when it passes the contact check and at least halves the Newton
+  // gradient norm; Newton then goes on until grad_tol, newton_iters or a step
+  // that does not (the root's round-off floor).  Differentiable rollouts need this: the implicit-function
   // gradient is the derivative of the exact root.
   bool polish_step(double h, double gradNorm, double& energy) {
     const size_t n3 = (size_t)n * 3;
@@ -515,8 +546,9 @@ struct Solver {
     }
     adj_gradient(xt.ptr, adj_gp.ptr);
     sync_scalars();
+    // a step that does not halve |grad| has reached the root's round-off
     const double gNew = std::sqrt(hscal->grad_sumsq);
-    if (!(gNew < gradNorm)) return false;
+    if (!(gNew < 0.5 * gradNorm)) return false;
     std::swap(x.ptr, xt.ptr);
     energy = total_energy(x.ptr, h);
     if (contact) contact->commit(x.ptr);

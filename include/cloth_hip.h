@@ -57,8 +57,10 @@ enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 
 
 #define CS_FLAG_TIMING 1u   /* record per-kernel HIP event timings (cs_kernel_timing) */
 /* Root polish (not in the reference): when every Armijo trial of a Newton
- * iteration fails, take the full (trust-scaled) step anyway if it passes the
- * contact check and lowers the Newton gradient norm, and keep iterating.  The
+ * iteration fails, take a full Newton step of the complete Hessian instead
+ * (central differences of the whole gradient, bending included) if it passes
+ * the contact check and at least halves the Newton gradient norm, and keep
+ * iterating.  The
  * energy test stops resolving decreases near the root; rollouts that are
  * differentiated with cs_step_adjoint need the root to grad_tol. */
 #define CS_FLAG_POLISH_ROOT 2u
