@@ -131,6 +131,55 @@ def fold_section(steps, h):
                               "traffic": profile_traffic("fold")[0], "traffic_source": profile_traffic("fold")[1]})
 
 
+def batch_section(steps, h, threads=8, count=64):
+    """The batched uneven drape (`scenes.batch_drapes`: 64 sheets of 1e3..1e5
+    vertices, each on its own sphere and floor): all instances stepped
+    together in mixed precision (cs_batch_step, `threads` host threads, one
+    HIP stream per instance) against one instance at a time in FP64, both
+    from the same initial states after one warmup step, block-Jacobi."""
+    from closim_amd import _abi, scenes as S
+    from closim_amd.solver import HipBatch
+    popts = dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI)
+    t0 = time.perf_counter()
+    scenes_m, scenes_d = S.batch_drapes(count), S.batch_drapes(count)
+    verts = int(sum(sc.mesh.count for sc in scenes_m))
+    gm = [solver_for(sc, "mixed", popts, timing=False) for sc in scenes_m]
+    gd = [solver_for(sc, "fp64", popts, timing=False) for sc in scenes_d]
+    print(f"[bench] batch: {count} instances, {verts} vertices, set up in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    B = HipBatch(gm, threads=threads)
+    B.step(h)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        B.step(h)
+    hip_sync()
+    t_b = time.perf_counter() - t0
+    print(f"[bench] batch mixed: {steps} steps of {count} instances in {t_b:.3f} s", file=sys.stderr, flush=True)
+    for g in gd:
+        raw_step(g, h)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for g in gd:
+            raw_step(g, h)
+    hip_sync()
+    t_s = time.perf_counter() - t0
+    print(f"[bench] batch fp64 one at a time: {t_s:.3f} s", file=sys.stderr, flush=True)
+    for g in gd:
+        g._copy_out()
+    err = max(float(np.max(np.abs(a.positions - b.positions)) / np.max(np.abs(b.positions)))
+              for a, b in zip(scenes_m, scenes_d))
+    for g in gm + gd:
+        g.close()
+    return dict(workload="batched_uneven_drape_64", instances=count, vertices=verts, threads=threads,
+                preconditioner="block-jacobi", warmup=1, steps=steps,
+                mixed_batched=dict(batch_steps_per_s=steps / t_b, instance_steps_per_s=steps * count / t_b,
+                                   vertex_steps_per_s=steps * verts / t_b),
+                fp64_one_at_a_time=dict(batch_steps_per_s=steps / t_s, instance_steps_per_s=steps * count / t_s),
+                speedup=t_s / t_b, pos_rel_err_mixed_vs_fp64_max=err)
+
+
 def solver_for(scene, precision, popts, timing):
     from closim_amd import _abi
     from closim_amd.solver import HipSolver
@@ -225,6 +274,9 @@ def main():
                     help="steps of the converged small-scene comparison (default 20, 0 = skip)")
     ap.add_argument("--fold-steps", type=int, default=3,
                     help="timed steps of the 512x512 self-contact fold section on the sphere workload (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=2,
+                    help="timed steps of the 64-instance batched drape section (0 = skip)")
+    ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (for counter calibration runs)")
     args = ap.parse_args()
@@ -345,6 +397,8 @@ def main():
         gm.close()
         gd.close()
         out["fold"] = fold_section(args.fold_steps, h)
+    if args.workload == "sphere" and args.batch_steps > 0 and gd:
+        out["batch"] = batch_section(args.batch_steps, h, threads=args.batch_threads)
     if args.copy_calibration:
         copy_calibration()
     if not args.no_cpu:

@@ -21,6 +21,7 @@ struct MixedInputs {
   int max_iters;
   double tol;
   Schwarz* sw = nullptr;  // additive Schwarz with FP32 factors (else binv / diag)
+  bool graph = false;     // replay the iterations as a captured HIP graph (no per-kernel timing)
 };
 
 struct MixedImpl;

@@ -524,6 +524,7 @@ __global__ void k_mx_out(const float* sol, double* dx, size_t n3) {
 struct MixedImpl {
   hipStream_t st;
   uint32_t n;
+  hipGraphExec_t exec = nullptr;  // the last captured chunk of kGraphIters iterations
   DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, scan;
   DevBuf<Frozen32> act32;
   DevBuf<MxScalars> s;
@@ -532,6 +533,7 @@ struct MixedImpl {
 };
 
 static unsigned mx_grid(uint32_t n) { return red_grid(n); }
+constexpr int kGraphIters = 8;  // PCG iterations per captured chunk (even)
 static unsigned mx_hvp_grid(const DMesh32& me) {
   if (me.ck_corner) {
     const size_t per_block = kBlock / kWave;
@@ -557,7 +559,10 @@ MixedSolve::MixedSolve(const cs_scene_desc& d, const cs_options&, hipStream_t st
   CS_HIP(hipMemsetAsync(im_->s.ptr, 0, sizeof(MxScalars), st));
 }
 
-MixedSolve::~MixedSolve() { delete im_; }
+MixedSolve::~MixedSolve() {
+  if (im_->exec) (void)hipGraphExecDestroy(im_->exec);
+  delete im_;
+}
 
 int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, bool), void* ctx) {
   MixedImpl& M = *im_;
@@ -605,12 +610,13 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   }
   hipLaunchKernelGGL(k_mx_decide, dim3(1), b, 0, st, a, mx_grid(n));
   CS_LAUNCH_CHECK();
-  const int chunk = 8;
-  for (int k = 0; k < in.max_iters; k++) {
+  // one PCG iteration; k only selects the r.z slot (k & 1) and is the
+  // iteration count the finishing kernel records (k + 1)
+  auto iteration = [&](int k, bool timed) {
     if (E) hipLaunchKernelGGL(k_mx_cscan, dim3(grid_for(E)), b, 0, st, a);
-    if (timer) timer(ctx, k, true);
+    if (timed) timer(ctx, k, true);
     launch_mx_hvp(a, st);
-    if (timer) timer(ctx, k, false);
+    if (timed) timer(ctx, k, false);
     hipLaunchKernelGGL(k_mx_update, g, b, 0, st, a, k, mx_hvp_grid(a.me));
     if (in.sw) {
       in.sw->apply32(a.r, a.z, &M.s.ptr->done);
@@ -618,19 +624,60 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
     }
     hipLaunchKernelGGL(k_mx_dir, g, b, 0, st, a, k);
     CS_LAUNCH_CHECK();
-    if ((k + 1) % chunk == 0 || k + 1 == in.max_iters) {
+  };
+  auto finished = [&](MxScalars& hs) {
+    CS_HIP(hipMemcpyAsync(&hs, M.s.ptr, sizeof(hs), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    return hs.done != 0;
+  };
+  const int chunk = kGraphIters;  // the host checks convergence every chunk
+  int k0 = 0, iters = -1;
+  if (in.graph && in.max_iters >= chunk) {
+    // Capture kGraphIters iterations (k = 0 .. kGraphIters-1; kGraphIters is
+    // even, so the r.z slots line up in every replay) and replay the chunk;
+    // the executable graph of the previous solve is updated in place when the
+    // launch sequence has the same shape.  Nothing in a chunk is a memset.
+    hipGraph_t graph = nullptr;
+    CS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; k++) iteration(k, false);
+    CS_HIP(hipStreamEndCapture(st, &graph));
+    bool updated = false;
+    if (M.exec) {
+      hipGraphNode_t err_node = nullptr;
+      hipGraphExecUpdateResult res;
+      updated = hipGraphExecUpdate(M.exec, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+      if (!updated) {
+        (void)hipGetLastError();
+        CS_HIP(hipGraphExecDestroy(M.exec));
+        M.exec = nullptr;
+      }
+    }
+    if (!M.exec) CS_HIP(hipGraphInstantiate(&M.exec, graph, nullptr, nullptr, 0));
+    CS_HIP(hipGraphDestroy(graph));
+    for (; k0 + chunk <= in.max_iters; k0 += chunk) {
+      CS_HIP(hipGraphLaunch(M.exec, st));
       MxScalars hs;
-      CS_HIP(hipMemcpyAsync(&hs, M.s.ptr, sizeof(hs), hipMemcpyDeviceToHost, st));
-      CS_HIP(hipStreamSynchronize(st));
-      if (hs.done) break;
+      if (finished(hs)) {
+        iters = k0 + hs.iters;  // the chunk recorded its local count
+        break;
+      }
+    }
+  }
+  if (iters < 0) {
+    for (int k = k0; k < in.max_iters; k++) {
+      iteration(k, timer != nullptr);
+      MxScalars hs;
+      if (((k + 1) % chunk == 0 || k + 1 == in.max_iters) && finished(hs)) {
+        iters = hs.iters;
+        break;
+      }
     }
   }
   hipLaunchKernelGGL(k_mx_out, dim3(grid_for((size_t)n * 3)), dim3(kBlock), 0, st, M.sol.ptr, in.dx, (size_t)n * 3);
   CS_LAUNCH_CHECK();
+  if (iters >= 0) return iters;
   MxScalars hs;
-  CS_HIP(hipMemcpyAsync(&hs, M.s.ptr, sizeof(hs), hipMemcpyDeviceToHost, st));
-  CS_HIP(hipStreamSynchronize(st));
-  return hs.done ? hs.iters : in.max_iters;
+  return finished(hs) ? hs.iters : in.max_iters;
 }
 
 // Back-to-back HVP launches on the operands of the last solve (inputs stay

@@ -5,10 +5,12 @@
 // stays resident in HBM and only scalars cross PCIe (gradient norm, PCG
 // completion, g.dx, max |dx|, trial energies, contact counts/validity).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cloth_hip.h"
@@ -760,6 +762,7 @@ struct Solver {
     in.max_iters = maxIters;
     in.tol = 1e-3;
     int mark = -1;
+    in.graph = (opt.flags & (CS_FLAG_TIMING | CS_FLAG_EAGER)) == 0;
     return mixed->solve(in, [&](int, bool begin) {
       if (begin) mark = ev_mark();
       else ev_span(CS_KERNEL_HVP, mark, ev_mark());
@@ -947,6 +950,49 @@ int cs_step(cs_solver* s, double dt) {
     s->s.has_last = true;
     return CS_OK;
   });
+}
+
+int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int threads, int32_t* status) {
+  if (!solvers && count) return fail(CS_E_INVALID, "null solver array");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CS_E_INVALID, "dt must be positive and finite");
+  for (uint32_t i = 0; i < count; i++)
+    if (!solvers[i]) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  {
+    std::vector<cs_solver*> seen(solvers, solvers + count);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+      return fail(CS_E_INVALID, "a solver appears twice in the batch");
+  }
+  // largest instances first, so the tail of the queue is short work
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return solvers[a]->s.n > solvers[b]->s.n; });
+  std::vector<int32_t> st(count, CS_OK);
+  std::vector<std::string> msg(count);
+  std::atomic<uint32_t> next{0};
+  auto worker = [&]() {
+    for (uint32_t k = next.fetch_add(1); k < count; k = next.fetch_add(1)) {
+      const uint32_t i = order[k];
+      st[i] = cs_step(solvers[i], dt);
+      if (st[i] != CS_OK) msg[i] = g_last_error;
+    }
+  };
+  const uint32_t nt = (uint32_t)std::max(1, std::min<int>(threads < 1 ? 1 : threads, (int)std::max<uint32_t>(count, 1)));
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (uint32_t t = 1; t < nt; t++) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  int first = CS_OK;
+  for (uint32_t i = 0; i < count; i++) {
+    if (status) status[i] = st[i];
+    if (st[i] != CS_OK && first == CS_OK) {
+      first = st[i];
+      g_last_error = "instance " + std::to_string(i) + ": " + msg[i];
+    }
+  }
+  return first;
 }
 
 int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double* v_prev, const double* x_next,

@@ -197,3 +197,44 @@ def small_strip():
     scene = create_scene(mesh, DEFAULT_MATERIAL, (0.0, -9.81, 0.0))
     pin_column(scene, lambda x, y, z, i: x < 1e-9)
     return scene
+
+
+def batch_drapes(count=64, seed=0xB47C4, d_hat=1e-3, mu=0.3):
+    """The batched uneven drape: `count` independent sheets, each with
+    (nx+1)(ny+1) ~ logU[1e3, 1e5] vertices and aspect nx/ny ~ U[0.5, 2] over
+    1 m^2 (side sqrt(aspect) x 1/sqrt(aspect)) at y = 0.5 with seeded +-1e-4 m
+    jitter, material moduli drawn log-uniformly (C00, C11 in [1e5, 1e7], G in
+    [5e4, 5e6], k in [1e-6, 1e-4], C01 = 0; density, thickness and damping of
+    the strip material), each over its own static sphere (r ~ U[0.1, 0.4],
+    top 5 cm below the sheet, 32x16 segments) and a floor at y = 0."""
+    from dataclasses import replace
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+        aspect = float(rng.uniform(0.5, 2.0))
+        ny = max(2, int(round(np.sqrt(n_target / aspect))) - 1)
+        nx = max(2, int(round(aspect * (ny + 1))) - 1)
+        w, l = float(np.sqrt(aspect)), float(1.0 / np.sqrt(aspect))
+        c00, c11 = (float(np.exp(rng.uniform(np.log(1e5), np.log(1e7)))) for _ in range(2))
+        g = float(np.exp(rng.uniform(np.log(5e4), np.log(5e6))))
+        k = float(np.exp(rng.uniform(np.log(1e-6), np.log(1e-4))))
+        r = float(rng.uniform(0.1, 0.4))
+        mat = replace(STRIP_MATERIAL, stretchWarp=c00, stretchWeft=c11, stretchCoupling=0.0, shear=g,
+                      bendWarp=k, bendWeft=k)
+        pos, uv, idx = build_grid(nx, ny, w, l)
+        P = pos.reshape(-1, 3).copy()
+        P[:, 1] = 0.5
+        mesh = preprocess(P.reshape(-1), uv, idx, mat.arealDensityKgM2)
+        scene = create_scene(mesh, mat, (0.0, -9.81, 0.0))
+        jit = mulberry32_array((seed + 7919 * i) & 0xFFFFFFFF, mesh.count * 3)
+        scene.positions += (jit * 2 - 1) * 1e-4
+        c = ContactSystem(ContactParams(dHatM=d_hat, kappaJ=50.0, dMinM=1e-4, frictionMu=mu, frictionEpsM=1e-4))
+        c.setFloor(0.0)
+        sp, si = sphere_mesh(r, 32, 16, center=(0.5 * w, 0.45 - r, 0.5 * l))
+        c.setStaticMesh(sp, si)
+        scene.contact = c
+        scene.meta = dict(name=f"batch_drape_{i}_{nx + 1}x{ny + 1}", radius=r)
+        out.append(scene)
+    return out
+

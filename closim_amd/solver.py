@@ -58,6 +58,7 @@ def load_library(path=LIB_PATH):
         "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cs_batch_step": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_int32)]),
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
         "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
@@ -313,3 +314,28 @@ class HipSolver:
             self.close()
         except Exception:
             pass
+
+
+class HipBatch:
+    """Many independent HipSolvers stepped together (cs_batch_step): host
+    threads take the instances largest first and step each on its own HIP
+    stream, so small instances' kernels overlap on the GPU."""
+
+    def __init__(self, solvers, threads=8):
+        self.solvers = list(solvers)
+        if len({id(s) for s in self.solvers}) != len(self.solvers):
+            raise ValueError("a solver appears twice in the batch")
+        self.threads = int(threads)
+        self._lib = load_library()
+
+    def step(self, dt):
+        for s in self.solvers:
+            s._need()
+        n = len(self.solvers)
+        hs = (C.c_void_p * n)(*[s._h.value for s in self.solvers])
+        status = (C.c_int32 * n)()
+        _check(self._lib.cs_batch_step(hs, n, float(dt), self.threads, status))
+        for s in self.solvers:
+            s._copy_out()
+        return list(status)
+

@@ -64,6 +64,11 @@ enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 
  * energy test stops resolving decreases near the root; rollouts that are
  * differentiated with cs_step_adjoint need the root to grad_tol. */
 #define CS_FLAG_POLISH_ROOT 2u
+/* Launch the mixed-precision PCG iterations one kernel at a time.  By default
+ * they run as a captured HIP graph of 8 iterations replayed until the solve
+ * converges (same kernels, same order, same results); CS_FLAG_TIMING also
+ * selects the one-by-one launches. */
+#define CS_FLAG_EAGER 4u
 
 /* ClothMaterial (src/physics/types.ts:1-14). */
 typedef struct cs_material {
@@ -231,6 +236,15 @@ int cs_reset_timing(cs_solver* s);
  * resident, bracketed by HIP events on the solver stream; `bytes` receives
  * the algorithmic bytes of one launch.  Requires a previous cs_step. */
 int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
+
+/* One step of each of `count` independent solvers (no reference counterpart:
+ * the reference steps one scene).  Host threads (`threads`, at least 1) take
+ * the instances from a queue, largest first, and each steps its instance on
+ * that instance's own HIP stream, so the instances' kernels overlap on the
+ * GPU.  `status` (optional, `count` entries) receives each cs_step result; the
+ * return value is the first failure (cs_last_error names the instance) or
+ * CS_OK.  A solver may appear only once in `solvers`. */
+int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int threads, int32_t* status);
 
 /* Adjoint of one step (no reference counterpart: the reference has no
  * differentiation).  Given the state before the step (x_prev, v_prev), the

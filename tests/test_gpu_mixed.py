@@ -14,6 +14,7 @@ import pytest
 
 import oracle_ffi as O
 from closim_amd import _abi, scenes as S
+from contact_scenes import crumpled_sheet
 from closim_amd.solver import HipSolver
 
 pytestmark = pytest.mark.gpu
@@ -104,3 +105,36 @@ def test_mixed_default_budget_tracks_oracle(make, steps, precond):
         assert e_fp64 <= 1e-9
     print(f"worst rel err (default budget): mixed {worst:.2e}, fp64-analytic {worst_a:.2e}")
     assert worst <= DRIFT_FACTOR * worst_a + 1e-6, (worst, worst_a)
+
+
+GRAPH_SCENES = [
+    (S.small_strip, 12, _abi.CS_PRECOND_JACOBI),
+    (S.floor_patch, 12, _abi.CS_PRECOND_BLOCK_JACOBI),
+    (lambda: S.floor_patch(mu=0.0), 12, _abi.CS_PRECOND_SCHWARZ),
+    (crumpled_sheet, 4, _abi.CS_PRECOND_SCHWARZ),
+]
+
+
+@pytest.mark.parametrize("make,steps,precond", GRAPH_SCENES)
+def test_graph_replay_matches_eager_launches(make, steps, precond):
+    # The mixed PCG runs as a captured HIP graph of 8 iterations replayed
+    # until convergence (the graph of the previous solve updated in place);
+    # every step replays it many times with the Newton loop's own kernels and
+    # host synchronisations in between.  It must give the eager launches'
+    # results bit for bit, step after step.
+    sg, se = make(), make()
+    kw = dict(precond=precond, pcg_max_iters=200)
+    if precond == _abi.CS_PRECOND_SCHWARZ:
+        kw["schwarz"] = (64, 1, 64)
+    g = HipSolver("mixed", **kw)
+    e = HipSolver("mixed", flags=_abi.CS_FLAG_EAGER, **kw)
+    g.initialize(sg)
+    e.initialize(se)
+    for t in range(steps):
+        g.step(H)
+        e.step(H)
+        assert g.lastStats["pcg_iters"] == e.lastStats["pcg_iters"], t
+        assert np.array_equal(sg.positions, se.positions), t
+        assert np.array_equal(sg.velocities, se.velocities), t
+    g.close()
+    e.close()
