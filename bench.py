@@ -14,6 +14,12 @@ Prints one JSON line:
                timed steps left resident (HIP events on the solver's
                stream), vs the 8 TB/s HBM peak
   cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample
+  fold         the 512x512 self-contact fold (additive Schwarz): mixed and
+               FP64 steps/s and its own HVP roofline
+  batch        64 uneven drapes stepped together in mixed precision
+               (cs_batch_step) vs one instance at a time in FP64
+  adjoint      the 128x128 material adjoint: loss gradient by checkpointed
+               step adjoints (FP64 and mixed forwards), FD check in shear, kb
 """
 import argparse
 import ctypes as C
@@ -180,6 +186,58 @@ def batch_section(steps, h, threads=8, count=64):
                 speedup=t_s / t_b, pos_rel_err_mixed_vs_fp64_max=err)
 
 
+def adjoint_section(steps, h, fd_eps=1e-3):
+    """The material-parameter adjoint on the 128x128 sheet pinned at two
+    corners (`adjoint.material_problem`, seed 0): the target trajectory from
+    the drawn parameters, then the loss and its gradient at the perturbed
+    guess (forward, checkpoints every 5 steps, backward through
+    cs_step_adjoint) with FP64 and with mixed forwards, and the central
+    differences of the loss in shear and kb (relative step `fd_eps`) beside
+    the FP64 gradient.  Forwards converge to the root (root polish, up to 400
+    Newton iterations, Jacobi PCG capped at 100 iterations per Newton
+    iteration); the largest final gradient norm of the steps is reported."""
+    from closim_amd import adjoint as A
+    out = dict(workload="material_adjoint_128x128", steps=steps, checkpoint_every=5, side_m=1.0)
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        sc, th_true, th_guess = A.material_problem(127)
+        g = A.converged_solver(sc, prec, newton_iters=400, pcg_max_iters=100)
+        r = A.MaterialRollout(g, steps=steps, h=h, checkpoint_every=5)
+        t0 = time.perf_counter()
+        targets = r.run(th_true)
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        L, grad = A.loss_and_grad(r, th_guess, targets)
+        t_fb = time.perf_counter() - t0
+        its = [it for it, _ in r.adjoint_iters]
+        res = dict(loss=L, gradient=dict(zip(A.PARAMS, map(float, grad))), max_final_grad_norm=r.max_grad_norm,
+                   forward_s_per_step=t_fwd / steps,
+                   forward_backward_s=t_fb, backward_s_per_step=(t_fb - t_fwd) / steps,
+                   adjoint_pcg_iters_mean=float(np.mean(its)),
+                   adjoint_rel_residual_max=float(max(rr for _, rr in r.adjoint_iters)))
+        if prec == "fp64":
+            out["theta_true"] = dict(zip(A.PARAMS, map(float, th_true)))
+            out["theta_guess"] = dict(zip(A.PARAMS, map(float, th_guess)))
+            res["fd"] = {}
+            for j in (3, 4):  # shear (the largest relative signal) and kb
+                e = A.fd_step(th_guess, j, fd_eps)
+                tp, tm = th_guess.copy(), th_guess.copy()
+                tp[j] += e
+                tm[j] -= e
+                fd = (A.trajectory_loss(r.run(tp), targets) - A.trajectory_loss(r.run(tm), targets)) / (2 * e)
+                res["fd"][A.PARAMS[j]] = dict(rel_step=fd_eps, fd=fd, adjoint=float(grad[j]),
+                                              rel_err=abs(fd - grad[j]) / abs(fd))
+            res["max_final_grad_norm_fd_rollouts"] = r.max_grad_norm
+        grads[prec] = grad
+        out[prec] = res
+        print(f"[bench] adjoint {prec}: {steps} steps forward {t_fwd:.1f} s, forward+backward {t_fb:.1f} s",
+              file=sys.stderr, flush=True)
+        g.close()
+    out["gradient_rel_diff_mixed_vs_fp64"] = float(np.max(np.abs(grads["mixed"] - grads["fp64"]) /
+                                                        np.maximum(np.abs(grads["fp64"]), 1e-300)))
+    return out
+
+
 def solver_for(scene, precision, popts, timing):
     from closim_amd import _abi
     from closim_amd.solver import HipSolver
@@ -277,6 +335,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed steps of the 64-instance batched drape section (0 = skip)")
     ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--adjoint-steps", type=int, default=5,
+                    help="rollout steps of the 128x128 material-adjoint section (0 = skip; the workload has 30)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run device copies of known size (for counter calibration runs)")
     args = ap.parse_args()
@@ -399,6 +459,8 @@ def main():
         out["fold"] = fold_section(args.fold_steps, h)
     if args.workload == "sphere" and args.batch_steps > 0 and gd:
         out["batch"] = batch_section(args.batch_steps, h, threads=args.batch_threads)
+    if args.workload == "sphere" and args.adjoint_steps > 0 and gd:
+        out["adjoint"] = adjoint_section(args.adjoint_steps, h)
     if args.copy_calibration:
         copy_calibration()
     if not args.no_cpu:

@@ -61,9 +61,11 @@ class MaterialRollout:
         self.base = replace(sc.material)
         self.checkpoints = {}
         self.adjoint_iters = []
+        self.max_grad_norm = 0.0  # largest final Newton gradient norm of the steps run so far
 
     def _step(self):
         self.solver.step(self.h)
+        self.max_grad_norm = max(self.max_grad_norm, self.solver.lastStats["grad_norm"])
         sc = self.solver.scene
         return sc.positions.copy(), sc.velocities.copy()
 

@@ -77,6 +77,10 @@ double pcg_time_hvp(hipStream_t st, const PcgArgs& a, int reps);
 // blocks at x and the Jacobi diagonal.
 void launch_block_factors(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* diag,
                           double* binv, uint8_t* flag);
+// out = M p / h^2 + H_membrane(x) p with the analytic membrane Hessian, pins
+// zeroed (no contact): the exact part of the step adjoint's Hessian product.
+void launch_hvp64_analytic(hipStream_t st, const DMesh& me, const DMat& mat, double inv_h2, const double* x,
+                           const double* p, double* out);
 
 // Vector helpers used by the Newton driver.
 void launch_dot(hipStream_t st, const double* a, const double* b, size_t n, jsm::dd* partials, double* out);

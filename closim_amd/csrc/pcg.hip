@@ -306,6 +306,39 @@ __global__ void __launch_bounds__(kBlock) k_pcg_hvp(PcgArgs a, int k, unsigned p
   block_partial_dd<kBlock>(pap, part(a, PART_PAP));
 }
 
+// out = M p / h^2 + H_membrane(x) p (analytic), pins zeroed: the exact
+// membrane-and-inertia part of the Hessian for the step adjoint.
+__global__ void __launch_bounds__(kBlock) k_hvp64_analytic(PcgArgs a, const double* __restrict__ p,
+                                                           double* __restrict__ out) {
+  const int sub = threadIdx.x % kLanesPerVertex;
+  const uint32_t stride = gridDim.x * kVertsPerBlock;
+  for (uint32_t v0 = blockIdx.x * kVertsPerBlock; v0 < a.me.n; v0 += stride) {
+    const uint32_t v = v0 + threadIdx.x / kLanesPerVertex;
+    const bool valid = v < a.me.n;
+    double o[3];
+    hvp_vertex8(a, v, valid, p, 0.0, o);
+    if (valid && sub == 0) {
+      const bool pinned = a.me.pinmask[v] != 0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) out[(size_t)v * 3 + i] = pinned ? 0.0 : o[i];
+    }
+  }
+}
+
+void launch_hvp64_analytic(hipStream_t st, const DMesh& me, const DMat& mat, double inv_h2, const double* x,
+                           const double* p, double* out) {
+  PcgArgs a{};
+  a.me = me;
+  a.mat = mat;
+  a.inv_h2 = inv_h2;
+  a.x = x;
+  a.hvp_mode = HVP_ANALYTIC;
+  a.contact.act = nullptr;
+  const unsigned g = (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)me.n + kVertsPerBlock - 1) / kVertsPerBlock);
+  hipLaunchKernelGGL(k_hvp64_analytic, dim3(std::max(1u, g)), dim3(kBlock), 0, st, a, p, out);
+  CS_LAUNCH_CHECK();
+}
+
 __global__ void __launch_bounds__(kBlock) k_pcg_update(PcgArgs a, int k, unsigned count) {
   PcgScalars* s = a.s;
   if (block_latched(&s->done)) return;

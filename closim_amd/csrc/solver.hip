@@ -76,19 +76,26 @@ __global__ void k_velocity(const double* x, const double* x0, double h, double d
 
 // ---- adjoint kernels (step_adjoint) ----------------------------------------
 
-__global__ void k_adj_fd(const double* gp, const double* gm, double scale, double* out, size_t n3) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n3) out[i] = (gp[i] - gm[i]) * scale;
-}
-
 __global__ void k_adj_shift(const double* x, const double* p, double s, double* out, size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n3) out[i] = x[i] + s * p[i];
 }
 
-__global__ void k_adj_pre(const double* r, const double* diag, const uint8_t* pinmask, double* z, size_t n3) {
+__global__ void k_adj_fd_add(const double* gp, const double* gm, double scale, double* out, size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n3) z[i] = pinmask[i / 3] ? 0.0 : r[i] / diag[i];
+  if (i < n3) out[i] += (gp[i] - gm[i]) * scale;
+}
+
+// z = B^-1 r per vertex with the block-Jacobi inverses (pins zeroed)
+__global__ void k_adj_pre_block(const double* r, const double* binv, const uint8_t* pinmask, double* z, uint32_t n) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const double* f = binv + (size_t)v * 9;
+  const double r0 = r[(size_t)v * 3], r1 = r[(size_t)v * 3 + 1], r2 = r[(size_t)v * 3 + 2];
+  const bool pinned = pinmask[v] != 0;
+  z[(size_t)v * 3] = pinned ? 0.0 : f[0] * r0 + f[1] * r1 + f[2] * r2;
+  z[(size_t)v * 3 + 1] = pinned ? 0.0 : f[3] * r0 + f[4] * r1 + f[5] * r2;
+  z[(size_t)v * 3 + 2] = pinned ? 0.0 : f[6] * r0 + f[7] * r1 + f[8] * r2;
 }
 
 __global__ void k_adj_axpy(double* y, double a, const double* x, size_t n3) {
@@ -384,14 +391,16 @@ struct Solver {
   // a = dL/dx1 + (damp / h) dL/dv1 on the free DOFs,
   //   H lambda = a,   dL/dtheta_j = -lambda . dg/dtheta_j,
   //   dL/dx0 = lambda M / h^2 - (damp / h) dL/dv1,   dL/dv0 = lambda M / h.
-  // H p is the central difference of the whole Newton gradient along p (so it
+  // H p is the analytic inertia + membrane product plus the central
+  // difference of the bending and contact gradients along p (full_hvp), so H
   // includes the FD bending gradient the step uses, which the reference's
-  // membrane-only HVP leaves out), solved by Jacobi PCG (full_pcg) to `tol`
+  // membrane-only HVP leaves out; block-Jacobi PCG (full_pcg) solves to `tol`
   // relative residual.  The membrane energy is linear in C00, C11, C01, G and the
   // hinge energy in kb = (bendWarp + bendWeft) / 2, so dg/dtheta_j is the
   // internal gradient of the material with theta_j = 1 and the others 0.
   // With contact, the active set is frozen at x1 (lagged friction as the
   // solver holds it); the velocity filter is not differentiated.
+  DevBuf<double> adj_binv;
   DevBuf<double> adj_x1, adj_lam, adj_r, adj_z, adj_p, adj_ap, adj_gp, adj_gm, adj_xs, adj_gx, adj_gv, adj_a;
 
   void adj_gradient(const double* xx, double* out) {
@@ -412,24 +421,38 @@ struct Solver {
       b->alloc(n3);
   }
 
-  // H p at `base` by the central difference of the Newton gradient along p
-  // (1e-6 m along p / |p|_inf); H includes every term of the gradient.
+  // H p at `base`: the inertia and membrane terms with the analytic membrane
+  // Hessian (exact), plus the central difference (1e-6 m along p / |p|_inf)
+  // of the remaining gradient terms, the FD bending gradient and the contact
+  // gradient; H is the derivative of the whole Newton gradient.
   void full_hvp(const double* base, const double* p, double* out) {
     const size_t n3 = (size_t)n * 3;
+    launch_hvp64_analytic(st, dmesh(), dmat(), 1.0 / (hadj * hadj), base, p, out);
+    const double kb = 0.5 * (mat.bend_warp + mat.bend_weft);
+    if (kb == 0 && !contact) return;
     launch_max_abs(st, p, n3, mpartials.ptr, &ds()->maxdx);
     const double pmax = read(&ds()->maxdx);
-    if (!(pmax > 0)) {
-      CS_HIP(hipMemsetAsync(out, 0, n3 * sizeof(double), st));
-      return;
-    }
+    if (!(pmax > 0)) return;
     const double eps = 1e-6;  // metres along p / |p|_inf
     hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, eps / pmax, adj_xs.ptr, n3);
-    adj_gradient(adj_xs.ptr, adj_gp.ptr);
+    rest_gradient(adj_xs.ptr, adj_gp.ptr);
     hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, -eps / pmax, adj_xs.ptr, n3);
-    adj_gradient(adj_xs.ptr, adj_gm.ptr);
-    hipLaunchKernelGGL(k_adj_fd, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr,
+    rest_gradient(adj_xs.ptr, adj_gm.ptr);
+    hipLaunchKernelGGL(k_adj_fd_add, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr,
                        pmax / (2 * eps), out, n3);
     CS_LAUNCH_CHECK();
+  }
+
+  // the gradient terms full_hvp differentiates numerically: bending (the
+  // material with its membrane moduli zeroed) and contact, no inertia
+  void rest_gradient(const double* xx, double* out) {
+    const double* cg = nullptr;
+    if (contact) {
+      contact->energy_grad(xx, ceg.ptr);
+      cg = ceg.ptr;
+    }
+    const DMat bend{mat.thickness, 0, 0, 0, 0, 0.5 * (mat.bend_warp + mat.bend_weft)};
+    launch_gradient(st, dmesh(), bend, xx, yhat.ptr, 0.0, cg, out, partials.ptr, &ds()->grad_sumsq);
   }
 
   double adj_dot(const double* a, const double* b, double* slot) {
@@ -437,7 +460,7 @@ struct Solver {
     return read(slot);
   }
 
-  // Jacobi PCG (the `diag` buffer) on H(base) sol = rhs over the free DOFs,
+  // Block-Jacobi PCG on H(base) sol = rhs over the free DOFs,
   // from sol = 0, to `tol` relative residual; returns the relative residual.
   // `rhs` must be zero on pinned DOFs.  Uses adj_r/z/p/ap/xs/gp/gm.
   double full_pcg(const double* base, const double* rhs, double* sol, double tol, int max_iters, int* iters) {
@@ -445,8 +468,15 @@ struct Solver {
     CS_HIP(hipMemsetAsync(sol, 0, n3 * sizeof(double), st));
     CS_HIP(hipMemcpyAsync(adj_r.ptr, rhs, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     const double bnorm2 = adj_dot(rhs, rhs, &ds()->adj[3]);
-    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, diag.ptr, pinmask.ptr,
-                       adj_z.ptr, n3);
+    // block-Jacobi: the 3x3 membrane vertex blocks at `base` plus the step's
+    // diagonal, inverted (Jacobi where a block is not positive definite)
+    adj_binv.alloc((size_t)n * 9);
+    launch_block_factors(st, dmesh(), dmat(), base, diag.ptr, adj_binv.ptr, nullptr);
+    auto precondition = [&]() {
+      hipLaunchKernelGGL(k_adj_pre_block, dim3(grid_for(n)), dim3(kBlock), 0, st, adj_r.ptr, adj_binv.ptr,
+                         pinmask.ptr, adj_z.ptr, n);
+    };
+    precondition();
     CS_HIP(hipMemcpyAsync(adj_p.ptr, adj_z.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     double rz = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
     int it = 0;
@@ -461,8 +491,7 @@ struct Solver {
       it++;
       rel = std::sqrt(adj_dot(adj_r.ptr, adj_r.ptr, &ds()->adj[2]) / bnorm2);
       if (rel <= tol) break;
-      hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, diag.ptr, pinmask.ptr,
-                         adj_z.ptr, n3);
+      precondition();
       const double rz_new = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
       hipLaunchKernelGGL(k_adj_xpby, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_p.ptr, adj_z.ptr, rz_new / rz, n3);
       rz = rz_new;
@@ -526,8 +555,8 @@ struct Solver {
   // their round-off and rejects every trial, which ends Newton at a gradient
   // norm set by the energy's magnitude rather than by grad_tol.  With the flag,
   // an iteration whose every Armijo trial failed takes instead a full Newton
-  // step of the complete Hessian (full_pcg: the central difference of the
-  // whole gradient, bending included, which the step's own HVP leaves out),
+  // step of the complete Hessian (full_pcg: analytic membrane, central
+  // differences of bending and contact, which the step's own HVP leaves out),
   // kept when it passes the contact check and at least halves the Newton
   // gradient norm; Newton then goes on until grad_tol, newton_iters or a step
   // that does not (the root's round-off floor).  Differentiable rollouts need this: the implicit-function

@@ -58,7 +58,7 @@ enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 
 #define CS_FLAG_TIMING 1u   /* record per-kernel HIP event timings (cs_kernel_timing) */
 /* Root polish (not in the reference): when every Armijo trial of a Newton
  * iteration fails, take a full Newton step of the complete Hessian instead
- * (central differences of the whole gradient, bending included) if it passes
+ * (analytic membrane, central differences of bending and contact) if it passes
  * the contact check and at least halves the Newton gradient norm, and keep
  * iterating.  The
  * energy test stops resolving decreases near the root; rollouts that are
