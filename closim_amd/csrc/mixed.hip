@@ -30,6 +30,7 @@ struct MxArgs {
   DMesh32 me;
   float c00, c11, c01, g, thick, inv_h2;
   const float* x;
+  const float4* tri;  // k_mx_tri_prep output (chunked HVP), 3 per triangle
   const float* diag;
   const float* binv;  // null => Jacobi
   float* sol;
@@ -131,6 +132,69 @@ __device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const fl
   if (pown)
 #pragma unroll
     for (int k = 0; k < 3; k++) pown[k] = corner == 0 ? v0[k] : corner == 1 ? v1[k] : v2[k];
+}
+
+// Per-triangle quantities of the membrane HVP that depend only on the
+// linearisation point x, computed once per solve: the deformation gradient
+// columns F0, F1, the second Piola stress S and thickness * area, packed in
+// three float4 per triangle (F0x F0y F0z F1x | F1y F1z S00 S11 | S01 sc - -).
+__global__ void __launch_bounds__(kBlock) k_mx_tri_prep(MxArgs a, float4* __restrict__ tri) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.me.m) return;
+  const uint32_t i0 = a.me.idx[(size_t)t * 3], i1 = a.me.idx[(size_t)t * 3 + 1], i2 = a.me.idx[(size_t)t * 3 + 2];
+  float x0[3], x1[3], x2[3];
+  ld3f(a.x, i0, x0); ld3f(a.x, i1, x1); ld3f(a.x, i2, x2);
+  const float4 inv = *reinterpret_cast<const float4*>(a.me.inv_dm + (size_t)t * 4);
+  float F0[3], F1[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float e1 = x1[k] - x0[k], e2 = x2[k] - x0[k];
+    F0[k] = inv.x * e1 + inv.z * e2;
+    F1[k] = inv.y * e1 + inv.w * e2;
+  }
+  const float E00 = 0.5f * (F0[0] * F0[0] + F0[1] * F0[1] + F0[2] * F0[2] - 1.f);
+  const float E11 = 0.5f * (F1[0] * F1[0] + F1[1] * F1[1] + F1[2] * F1[2] - 1.f);
+  const float E01 = 0.5f * (F0[0] * F1[0] + F0[1] * F1[1] + F0[2] * F1[2]);
+  const float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+  tri[(size_t)t * 3] = make_float4(F0[0], F0[1], F0[2], F1[0]);
+  tri[(size_t)t * 3 + 1] = make_float4(F1[1], F1[2], S00, S11);
+  tri[(size_t)t * 3 + 2] = make_float4(S01, a.thick * a.me.area[t], 0.f, 0.f);
+}
+
+// hvp_corner32 from the prepared triangle quantities (same arithmetic on the
+// p-dependent terms).
+__device__ __forceinline__ void hvp_corner32_prep(const MxArgs& a, uint4 cr, const float* p, float* out,
+                                                  float* pown) {
+  const uint32_t i0 = cr.x, i1 = cr.y, i2 = cr.z, t = cr.w >> 2;
+  const int corner = cr.w & 3;
+  const float4 q0 = a.tri[(size_t)t * 3], q1 = a.tri[(size_t)t * 3 + 1], q2 = a.tri[(size_t)t * 3 + 2];
+  const float4 inv = *reinterpret_cast<const float4*>(a.me.inv_dm + (size_t)t * 4);
+  float v0[3], v1[3], v2[3];
+  ld3f(p, i0, v0); ld3f(p, i1, v1); ld3f(p, i2, v2);
+  const float ia = inv.x, ib = inv.y, ic = inv.z, id = inv.w;
+  const float F0[3] = {q0.x, q0.y, q0.z}, F1[3] = {q0.w, q1.x, q1.y};
+  const float S00 = q1.z, S11 = q1.w, S01 = q2.x, sc = q2.y;
+  float dF0[3], dF1[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float w1 = v1[k] - v0[k], w2 = v2[k] - v0[k];
+    dF0[k] = ia * w1 + ic * w2;
+    dF1[k] = ib * w1 + id * w2;
+  }
+  const float dE00 = F0[0] * dF0[0] + F0[1] * dF0[1] + F0[2] * dF0[2];
+  const float dE11 = F1[0] * dF1[0] + F1[1] * dF1[1] + F1[2] * dF1[2];
+  const float dE01 = 0.5f * (dF0[0] * F1[0] + F0[0] * dF1[0] + dF0[1] * F1[1] + F0[1] * dF1[1] + dF0[2] * F1[2] +
+                             F0[2] * dF1[2]);
+  const float dS00 = a.c00 * dE00 + a.c01 * dE11, dS11 = a.c11 * dE11 + a.c01 * dE00, dS01 = 2.f * a.g * dE01;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float dP0 = dF0[k] * S00 + F0[k] * dS00 + dF1[k] * S01 + F1[k] * dS01;
+    const float dP1 = dF0[k] * S01 + F0[k] * dS01 + dF1[k] * S11 + F1[k] * dS11;
+    const float g1 = sc * (dP0 * ia + dP1 * ib);
+    const float g2 = sc * (dP0 * ic + dP1 * id);
+    out[k] = corner == 0 ? -(g1 + g2) : corner == 1 ? g1 : g2;
+    pown[k] = corner == 0 ? v0[k] : corner == 1 ? v1[k] : v2[k];
+  }
 }
 
 // Frozen barrier HVP term of one incidence entry: coeff (J.p) times the
@@ -440,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp_ck(MxArgs a) {
     const CkLane L = ck_lane(a, cr, lane);
     float t[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f};
     if (L.self) ld3f(a.p, L.vid, pv);
-    else if (!L.pad) hvp_corner32(a, cr, a.p, t, pv);
+    else if (!L.pad) hvp_corner32_prep(a, cr, a.p, t, pv);
     pap += ck_finish(a, L, lane, t, pv);
   }
   block_partial_f(pap, mpart(a, MX_PAP));
@@ -526,6 +590,7 @@ struct MixedImpl {
   uint32_t n;
   hipGraphExec_t exec = nullptr;  // the last captured chunk of kGraphIters iterations
   DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, scan;
+  DevBuf<float4> tri;
   DevBuf<Frozen32> act32;
   DevBuf<MxScalars> s;
   MxArgs last{};
@@ -588,6 +653,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.thick = (float)in.mat.thick;
   a.inv_h2 = (float)in.inv_h2;
   a.x = M.x32.ptr; a.diag = M.d32.ptr; a.binv = in.binv ? M.binv32.ptr : nullptr;
+  a.tri = nullptr;
   a.sol = M.sol.ptr; a.r = M.r.ptr; a.z = M.z.ptr; a.p = M.p.ptr; a.Ap = M.Ap.ptr;
   a.tol = (float)in.tol;
   a.ext = in.sw ? 1 : 0;
@@ -600,6 +666,13 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.act = C ? M.act32.ptr : nullptr;
   a.v_off = in.contact.v_off;
   a.v_ent = in.contact.v_ent;
+  if (a.me.ck_corner) {
+    M.tri.alloc((size_t)std::max<uint32_t>(a.me.m, 1) * 3);
+    a.tri = M.tri.ptr;
+    hipLaunchKernelGGL(k_mx_tri_prep, dim3(grid_for(std::max<uint32_t>(a.me.m, 1))), dim3(kBlock), 0, st, a,
+                       M.tri.ptr);
+    CS_LAUNCH_CHECK();
+  }
   M.last = a;
   M.has_last = true;
   const dim3 g(mx_grid(n)), b(kBlock);

@@ -810,6 +810,10 @@ struct Solver {
     const bool mixed_p = opt.precision == CS_PREC_MIXED;
     const double fs = mixed_p ? 4.0 : 8.0;
     double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (16 + 4 + 3 * 16);
+    if (mixed_p && ck_count)
+      // chunked kernel: per vertex p, Ap, mass, pin flag; per triangle Dm^-1,
+      // the prepared F/S/area record (48 B) and three corner records
+      b = (double)n * (6 * fs + 4 + 1) + (double)m * (16 + 48 + 3 * 16);
     if (contact && contact->active_count())
       b += mixed_p ? 4.0 * n + 3 * fs * (double)contact->incidence_vertices() : 3 * fs * n;
     return b;
