@@ -9,10 +9,14 @@ Prints one JSON line:
   value        mixed-precision HipSolver steps/s (state resident in HBM;
                host copy-out of positions is excluded and reported apart)
   fp64         the FP64 HipSolver (reference operator) on the same scene
-  roofline     HVP kernel (k_mx_hvp_ck): algorithmic bytes per launch / mean
-               launch time of back-to-back launches on the operands the
-               timed steps left resident (HIP events on the solver's
-               stream), vs the 8 TB/s HBM peak
+  roofline     HVP kernel (k_mx_hvp_ck): algorithmic (compulsory) bytes per
+               launch, 40n + 32m + 64C for n vertices, m triangles and C
+               active contacts, / mean launch time of back-to-back launches on
+               the operands the timed steps left resident (HIP events on the
+               solver's stream), vs the 8 TB/s HBM peak; the bytes the kernel
+               actually moves (its index maps and prepared triangle records
+               included) are kernel_bytes_per_launch, the counter-measured
+               HBM traffic is traffic
   cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample
   fold         the 512x512 self-contact fold (additive Schwarz): mixed and
                FP64 steps/s and its own HVP roofline
@@ -119,9 +123,11 @@ def fold_section(steps, h):
         t = timed_steps(g, h, steps)
         st = g.lastStats
         ms, nbytes = g.time_hvp(200)
+        _, rbytes = g.hvp_bytes()
         g._copy_out()
         res[prec] = dict(steps_per_s=steps / t, ms_per_step=1000 * t / steps, newton_used_last_step=st["newton_used"],
                          pcg_iters_last_step=st["pcg_iters"], hvp_mean_us=1000 * ms,
+                         hvp_algorithmic_bytes=nbytes, hvp_kernel_bytes=rbytes,
                          hvp_gbs=nbytes / (ms / 1000) / 1e9 if ms > 0 else 0.0,
                          positions=g.getPositions().copy())
         print(f"[bench] fold {prec}: {steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
@@ -134,6 +140,8 @@ def fold_section(steps, h):
                 pos_rel_err_mixed_vs_fp64=float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd))),
                 roofline_hvp={"bound": "hbm", "achieved": m["hvp_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": m["hvp_gbs"] / HBM_PEAK_GBS, "kernel": "k_mx_hvp_ck",
+                              "bytes_per_launch": m["hvp_algorithmic_bytes"],
+                              "kernel_bytes_per_launch": m["hvp_kernel_bytes"],
                               "traffic": profile_traffic("fold")[0], "traffic_source": profile_traffic("fold")[1]})
 
 
@@ -413,6 +421,7 @@ def main():
         res["accuracy_converged"] = converged_comparison(acc_steps, h)
     mean_s = hvp_ms / 1000.0
     achieved = hvp_bytes / mean_s / 1e9 if mean_s > 0 else 0.0
+    _, hvp_kernel_bytes = gm.hvp_bytes()
     traffic = args.traffic
     traffic_src = "--traffic" if traffic is not None else None
     if traffic is None:
@@ -443,7 +452,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_mx_hvp_ck (FP32 analytic membrane HVP + mass + frozen-contact HVP)",
-                     "bytes_per_launch": hvp_bytes, "mean_launch_us": mean_s * 1e6, "launches": args.hvp_reps,
+                     "bytes_per_launch": hvp_bytes, "kernel_bytes_per_launch": hvp_kernel_bytes,
+                     "bytes_model": "40n + 32m + 64C (compulsory: x, p, y, mass per vertex; indices, Dm^-1, area "
+                                    "per triangle; one record per active contact)",
+                     "mean_launch_us": mean_s * 1e6, "launches": args.hvp_reps,
                      "traffic_source": traffic_src},
         "newton_used_last_step": st_m["newton_used"],
         "pcg_iters_last_step": st_m["pcg_iters"],

@@ -801,19 +801,28 @@ struct Solver {
   PcgArgs last_pcg{};
   bool has_last_pcg = false;
 
-  // Algorithmic HBM bytes of one HVP launch: every input element read once
-  // and Ap written once (per vertex: x, p, Ap, mass, pin flag, CSR offset;
-  // per triangle: Dm^-1, area and three CSR corner records of 16 bytes; with
-  // contacts: FP64 reads the per-vertex contact sums, mixed precision the
-  // incidence offsets and one chunk running sum per contact vertex).
+  // Algorithmic (compulsory) HBM bytes of one HVP y = (M/h^2 + H_mem +
+  // H_bar) p: FP32 40n + 32m + 64C (x, p, y 12 B each and the mass per
+  // vertex; three indices, Dm^-1 and the area per triangle; a 64 B record per
+  // active contact), FP64 80n + 52m + 96C.  Index maps and per-element
+  // scratch are not counted.
   double hvp_bytes() const {
+    const double C = contact ? (double)contact->active_count() : 0.0;
+    if (opt.precision == CS_PREC_MIXED) return 40.0 * n + 32.0 * m + 64.0 * C;
+    return 80.0 * n + 52.0 * m + 96.0 * C;
+  }
+
+  // Bytes the HVP kernel of this build reads and writes per launch, its
+  // index maps and prepared records included (per vertex p, Ap, mass, pin
+  // flag [, x and the CSR offset]; per triangle Dm^-1 and three 16 B corner
+  // records [+ the 48 B F/S record of the chunked kernel, or x gathers and
+  // the area]; with contacts the incidence offsets and one running sum per
+  // contact vertex (FP32) or the per-vertex contact sums (FP64)).
+  double hvp_read_bytes() const {
     const bool mixed_p = opt.precision == CS_PREC_MIXED;
     const double fs = mixed_p ? 4.0 : 8.0;
     double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (16 + 4 + 3 * 16);
-    if (mixed_p && ck_count)
-      // chunked kernel: per vertex p, Ap, mass, pin flag; per triangle Dm^-1,
-      // the prepared F/S/area record (48 B) and three corner records
-      b = (double)n * (6 * fs + 4 + 1) + (double)m * (16 + 48 + 3 * 16);
+    if (mixed_p && ck_count) b = (double)n * (6 * fs + 4 + 1) + (double)m * (16 + 48 + 3 * 16);
     if (contact && contact->active_count())
       b += mixed_p ? 4.0 * n + 3 * fs * (double)contact->incidence_vertices() : 3 * fs * n;
     return b;
@@ -983,6 +992,13 @@ int cs_step(cs_solver* s, double dt) {
     s->s.has_last = true;
     return CS_OK;
   });
+}
+
+int cs_hvp_bytes(const cs_solver* s, double* algorithmic, double* read) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (algorithmic) *algorithmic = s->s.hvp_bytes();
+  if (read) *read = s->s.hvp_read_bytes();
+  return CS_OK;
 }
 
 int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int threads, int32_t* status) {

@@ -58,6 +58,7 @@ def load_library(path=LIB_PATH):
         "cs_time_hvp": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
         "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cs_hvp_bytes": (C.c_int, [C.c_void_p, _dp, _dp]),
         "cs_batch_step": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_int32)]),
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
@@ -250,6 +251,15 @@ class HipSolver:
         b = C.c_double()
         _check(self._lib.cs_time_hvp(self._h, int(reps), C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+    def hvp_bytes(self):
+        """(compulsory bytes, bytes this build's kernel moves) of one HVP
+        launch on the current operands (cs_hvp_bytes)."""
+        self._need()
+        a = C.c_double()
+        r = C.c_double()
+        _check(self._lib.cs_hvp_bytes(self._h, C.byref(a), C.byref(r)))
+        return a.value, r.value
 
     # ---- component probes (parity tests) ------------------------------------
     def probe_active_set(self, x0, x, cap=1 << 20):

@@ -236,6 +236,12 @@ int cs_reset_timing(cs_solver* s);
  * resident, bracketed by HIP events on the solver stream; `bytes` receives
  * the algorithmic bytes of one launch.  Requires a previous cs_step. */
 int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
+/* (no reference counterpart: measurement hook) Bytes of one HVP launch on the
+ * current operands: `algorithmic` the compulsory traffic (FP32 40n + 32m +
+ * 64C, FP64 80n + 52m + 96C for n vertices, m triangles, C active contacts),
+ * `read` what this build's kernel reads and writes, index maps and prepared
+ * per-triangle records included.  cs_time_hvp's `bytes` is `algorithmic`. */
+int cs_hvp_bytes(const cs_solver* s, double* algorithmic, double* read);
 
 /* One step of each of `count` independent solvers (no reference counterpart:
  * the reference steps one scene).  Host threads (`threads`, at least 1) take
