@@ -344,8 +344,11 @@ class HipBatch:
         n = len(self.solvers)
         hs = (C.c_void_p * n)(*[s._h.value for s in self.solvers])
         status = (C.c_int32 * n)()
-        _check(self._lib.cs_batch_step(hs, n, float(dt), self.threads, status))
-        for s in self.solvers:
-            s._copy_out()
+        rc = self._lib.cs_batch_step(hs, n, float(dt), self.threads, status)
+        # the instances that stepped are copied out even when another failed
+        for s, st in zip(self.solvers, status):
+            if st == _abi.CS_OK:
+                s._copy_out()
+        _check(rc)
         return list(status)
 

@@ -98,3 +98,34 @@ def test_schwarz_partition_matches_oracle(make, sizes):
     assert np.array_equal(loc, ol)
     sizes_seen = np.bincount(dom)
     assert sizes_seen.max() < sizes[0] + sizes[1]  # a merged remainder has fewer than min vertices
+
+
+def test_batch_arguments_are_rejected_without_a_device():
+    """cs_batch_step / HipBatch: null or repeated solvers and a bad dt fail
+    before any instance steps (no device needed)."""
+    from closim_amd.solver import HipBatch, load_library
+    L = load_library()
+    assert L.cs_batch_step(None, 0, 1 / 60, 1, None) == _abi.CS_OK  # an empty batch steps nothing
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.cs_batch_step(arr, 2, 1 / 60, 2, None) == _abi.CS_E_NOT_INIT
+    assert L.cs_batch_step(arr, 2, -1.0, 2, None) == _abi.CS_E_INVALID
+    assert L.cs_batch_step(None, 2, 1 / 60, 2, None) == _abi.CS_E_INVALID
+    s = object()
+    with pytest.raises(ValueError):
+        HipBatch([s, s])
+
+
+def test_batch_drapes_workload_is_seeded():
+    """The batched drape workload: 64 instances with the stated size and
+    parameter ranges, identical for the same seed."""
+    from closim_amd import scenes as S
+    a, b = S.batch_drapes(64), S.batch_drapes(64)
+    assert len(a) == 64
+    n = np.array([sc.mesh.count for sc in a])
+    assert n.min() >= 900 and n.max() <= 110000
+    for x, y in zip(a, b):
+        assert np.array_equal(x.positions, y.positions)
+        m = x.material
+        assert 1e5 <= m.stretchWarp <= 1e7 and 1e5 <= m.stretchWeft <= 1e7 and 5e4 <= m.shear <= 5e6
+        assert 1e-6 <= m.bendWarp <= 1e-4 and m.bendWarp == m.bendWeft and m.stretchCoupling == 0.0
+        assert x.contact.floorY == 0.0 and x.contact.params.frictionMu == 0.3
