@@ -124,10 +124,11 @@ def fold_section(steps, h):
         st = g.lastStats
         ms, nbytes = g.time_hvp(200)
         _, rbytes = g.hvp_bytes()
+        tuned = g.tuned_launch()
         g._copy_out()
         res[prec] = dict(steps_per_s=steps / t, ms_per_step=1000 * t / steps, newton_used_last_step=st["newton_used"],
                          pcg_iters_last_step=st["pcg_iters"], hvp_mean_us=1000 * ms,
-                         hvp_algorithmic_bytes=nbytes, hvp_kernel_bytes=rbytes,
+                         hvp_algorithmic_bytes=nbytes, hvp_kernel_bytes=rbytes, tuned_launch=tuned,
                          hvp_gbs=nbytes / (ms / 1000) / 1e9 if ms > 0 else 0.0,
                          positions=g.getPositions().copy())
         print(f"[bench] fold {prec}: {steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
@@ -463,6 +464,7 @@ def main():
         "candidate_prims_last_step": st_m["contact"]["candidate_pairs"] if st_m["contact"] else 0,
         "velocity_filter_rounds_last_step": st_m["contact"]["filter_rounds"] if st_m["contact"] else 0,
         "pcie_inclusive_steps_per_s": pcie_rate,
+        "tuned_launch": gm.tuned_launch(),
     }
     out.update(res)
     if args.workload == "sphere" and args.fold_steps > 0 and gd:

@@ -38,6 +38,8 @@ class ContactSystem {
   // Copy the active list (kind,p,a,b,c,d + w0..w3,nx,ny,nz) to host arrays.
   uint64_t copy_active(int32_t* ids6, double* vals7, uint64_t cap);
   uint64_t candidate_count() const;
+  // grid caps the launch autotuner chose ("tuning" until it has measured)
+  std::string tuned() const;
   uint64_t incidence_count() const;
   uint64_t incidence_vertices() const;
   ContactImpl* impl() { return im_; }

@@ -1654,7 +1654,68 @@ static size_t kill_lds_limit() {
   return lim > 64 ? lim - 64 : 0;
 }
 
+// Grid-size autotuner for a grid-stride kernel whose results do not depend on
+// its grid (every reduction is an atomic min or count, every emission is
+// sorted or consumed by an order-free pass).  The first calls run the
+// candidate grid caps in turn, timed with HIP events on the stream; once each
+// has `kSamples` timings the cap with the smallest median is kept for the
+// life of the solver.  The timings are read after the caller's own
+// synchronisation, so tuning adds no host waits.
+struct GridTuner {
+  static constexpr int kSamples = 3;
+  const char* name;
+  std::vector<unsigned> caps{512, 1024, 2048, 4096, 8192};
+  std::vector<std::vector<float>> ms;
+  int chosen = -1, cur = -1, turn = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  explicit GridTuner(const char* n) : name(n), ms(caps.size()) {}
+  ~GridTuner() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  // grid for `work` items of one thread each; brackets the launch when tuning
+  unsigned grid(size_t work, hipStream_t st) {
+    const size_t need = std::max<size_t>(1, (work + kBlock - 1) / kBlock);
+    if (chosen >= 0) return (unsigned)std::min<size_t>(caps[chosen], need);
+    if (!e0) {
+      CS_HIP(hipEventCreate(&e0));
+      CS_HIP(hipEventCreate(&e1));
+    }
+    cur = turn;
+    turn = (turn + 1) % (int)caps.size();
+    CS_HIP(hipEventRecord(e0, st));
+    return (unsigned)std::min<size_t>(caps[cur], need);
+  }
+  void after_launch(hipStream_t st) {
+    if (cur >= 0) CS_HIP(hipEventRecord(e1, st));
+  }
+  // after the stream has been synchronised
+  void collect() {
+    if (cur < 0) return;
+    float t = 0;
+    CS_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms[cur].push_back(t);
+    cur = -1;
+    bool done = true;
+    for (auto& v : ms) done = done && (int)v.size() >= kSamples;
+    if (!done) return;
+    float best = 0;
+    for (size_t c = 0; c < caps.size(); c++) {
+      std::vector<float> v = ms[c];
+      std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+      if (chosen < 0 || v[v.size() / 2] < best) {
+        chosen = (int)c;
+        best = v[v.size() / 2];
+      }
+    }
+  }
+  std::string describe() const {
+    return std::string(name) + "=" + (chosen >= 0 ? std::to_string(caps[chosen]) : std::string("tuning"));
+  }
+};
+
 struct ContactImpl {
+  GridTuner tune_trial{"k_trial_pairs"}, tune_active{"k_active_pairs"};
   hipStream_t st;
   cs_contact_params prm;
   uint32_t n, m, ns, ms;
@@ -2051,20 +2112,21 @@ struct ContactImpl {
     for (int attempt = 0; attempt < 3; attempt++) {
       CS_HIP(hipMemsetAsync(&cs.ptr->pair_count, 0, 3 * sizeof(uint32_t), st));  // pair, spair, active counts
       if (total) {
+        const dim3 ag(tune_active.grid(total, st));
         if (src.filter)
-          hipLaunchKernelGGL(k_active_pairs<true>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, x,
-                             prm.d_hat, src.p, src.np, src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr,
-                             act_cap, cs.ptr);
+          hipLaunchKernelGGL(k_active_pairs<true>, ag, dim3(kBlock), 0, st, G, xstep, x, prm.d_hat, src.p, src.np,
+                             src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr, act_cap, cs.ptr);
         else
-          hipLaunchKernelGGL(k_active_pairs<false>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, x,
-                             prm.d_hat, src.p, src.np, src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr,
-                             act_cap, cs.ptr);
+          hipLaunchKernelGGL(k_active_pairs<false>, ag, dim3(kBlock), 0, st, G, xstep, x, prm.d_hat, src.p, src.np,
+                             src.s, src.nsp, sbox.ptr, rec.ptr, akeys.ptr, anke_raw.ptr, act_cap, cs.ptr);
+        tune_active.after_launch(st);
       }
       if (has_floor)
         hipLaunchKernelGGL(k_floor_active, dim3(red_grid(n)), dim3(kBlock), 0, st, G, x, rec.ptr, akeys.ptr,
                            anke_raw.ptr, act_cap, dpart.ptr + 1024, cs.ptr);
       CS_LAUNCH_CHECK();
       sync_cs();
+      if (total) tune_active.collect();
       if (hcs->active_count <= act_cap) break;
       grow_active((uint32_t)(hcs->active_count * 1.25) + 1024);
       if (attempt == 2) throw OverflowError("active contact buffer could not be grown");
@@ -2264,12 +2326,14 @@ struct ContactImpl {
         tail_cap = (size_t)total * 15;
         tailq.alloc(tail_cap);
       }
+      const dim3 tg(tune_trial.grid(total, st));
       if (src.filter)
-        hipLaunchKernelGGL(k_trial_pairs<true>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, xt, prm.d_min,
-                           src.p, src.np, src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
+        hipLaunchKernelGGL(k_trial_pairs<true>, tg, dim3(kBlock), 0, st, G, xstep, xt, prm.d_min, src.p, src.np,
+                           src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
       else
-        hipLaunchKernelGGL(k_trial_pairs<false>, dim3(red_grid(total)), dim3(kBlock), 0, st, G, xstep, xt,
-                           prm.d_min, src.p, src.np, src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
+        hipLaunchKernelGGL(k_trial_pairs<false>, tg, dim3(kBlock), 0, st, G, xstep, xt, prm.d_min, src.p, src.np,
+                           src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
+      tune_trial.after_launch(st);
       hipLaunchKernelGGL(k_trial_ccd, dim3(256), dim3(kBlock), 0, st, G, xstep, xt, src.p, src.np, src.s, tailq.ptr,
                          cs.ptr);
     }
@@ -2277,6 +2341,7 @@ struct ContactImpl {
       hipLaunchKernelGGL(k_floor_trial, dim3(red_grid(n)), dim3(kBlock), 0, st, G, xstep, xt, dpart.ptr + 2048, cs.ptr);
     CS_LAUNCH_CHECK();
     sync_cs();
+    if (total) tune_trial.collect();
     candidate_counts(src);
     double minDist = mink_value(5);
     double minTOI = mink_value(6);
@@ -2327,6 +2392,7 @@ cs_contact_diag ContactSystem::diagnostics() const { return im_->diag; }
 uint64_t ContactSystem::active_count() const { return im_->C; }
 
 uint64_t ContactSystem::candidate_count() const { return im_->diag.candidate_pairs; }
+std::string ContactSystem::tuned() const { return im_->tune_trial.describe() + " " + im_->tune_active.describe(); }
 uint64_t ContactSystem::incidence_count() const { return im_->E; }
 uint64_t ContactSystem::incidence_vertices() const { return im_->nv; }
 

@@ -994,6 +994,13 @@ int cs_step(cs_solver* s, double dt) {
   });
 }
 
+int cs_tuned_launch(const cs_solver* s, char* buf, size_t cap) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  const std::string t = s->s.contact ? s->s.contact->tuned() : std::string();
+  if (buf && cap) std::snprintf(buf, cap, "%s", t.c_str());
+  return CS_OK;
+}
+
 int cs_hvp_bytes(const cs_solver* s, double* algorithmic, double* read) {
   if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
   if (algorithmic) *algorithmic = s->s.hvp_bytes();

@@ -59,6 +59,7 @@ def load_library(path=LIB_PATH):
         "cs_schwarz_partition": (C.c_int, [C.POINTER(_abi.CsSceneDesc), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "cs_hvp_bytes": (C.c_int, [C.c_void_p, _dp, _dp]),
+        "cs_tuned_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
         "cs_batch_step": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_int32)]),
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
@@ -251,6 +252,13 @@ class HipSolver:
         b = C.c_double()
         _check(self._lib.cs_time_hvp(self._h, int(reps), C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+    def tuned_launch(self):
+        """Grid caps the launch autotuner chose for the contact kernels."""
+        self._need()
+        buf = C.create_string_buffer(256)
+        _check(self._lib.cs_tuned_launch(self._h, buf, 256))
+        return buf.value.decode()
 
     def hvp_bytes(self):
         """(compulsory bytes, bytes this build's kernel moves) of one HVP

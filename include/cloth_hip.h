@@ -242,6 +242,12 @@ int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
  * `read` what this build's kernel reads and writes, index maps and prepared
  * per-triangle records included.  cs_time_hvp's `bytes` is `algorithmic`. */
 int cs_hvp_bytes(const cs_solver* s, double* algorithmic, double* read);
+/* (no reference counterpart: measurement hook) The grid caps the launch
+ * autotuner chose for the contact kernels whose results do not depend on
+ * their grid (k_trial_pairs, k_active_pairs), as text such as
+ * "k_trial_pairs=2048 k_active_pairs=1024" ("tuning" while it still measures
+ * the candidates on the first calls; empty without contact). */
+int cs_tuned_launch(const cs_solver* s, char* buf, size_t cap);
 
 /* One step of each of `count` independent solvers (no reference counterpart:
  * the reference steps one scene).  Host threads (`threads`, at least 1) take
