@@ -38,6 +38,9 @@ class MixedSolve {
   // Mean duration (ms) of `reps` back-to-back HVP launches on the operands of
   // the last solve, bracketed by HIP events on the solver stream.
   double time_hvp(int reps);
+  // One launch of the solve's HVP kernel (no contact) at (x, p) into out
+  // (FP64 device arrays); for component parity tests.
+  void probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p, double* out);
   void set_material(const cs_material&) {}
 
  private:

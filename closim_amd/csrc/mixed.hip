@@ -753,6 +753,45 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   return finished(hs) ? hs.iters : in.max_iters;
 }
 
+__global__ void k_mx_probe_in(const double* x, const double* p, float* x32, float* p32, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n3) {
+    x32[i] = (float)x[i];
+    p32[i] = (float)p[i];
+  }
+}
+
+// One HVP launch (the kernel the solve uses, no contact) at (x, p), for
+// component parity tests; overwrites the solve's x, p and Ap buffers.
+void MixedSolve::probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p,
+                           double* out) {
+  MixedImpl& M = *im_;
+  hipStream_t st = M.st;
+  const size_t n3 = (size_t)me.n * 3;
+  hipLaunchKernelGGL(k_mx_probe_in, dim3(grid_for(n3)), dim3(kBlock), 0, st, x, p, M.x32.ptr, M.p.ptr, n3);
+  MxArgs a{};
+  a.me = DMesh32{me.n,         me.m,        me.idx,       me.inv_dm,   me.area,
+                 me.mass,      me.vt_off,   me.vt_rec,    me.vt_corner, me.pinmask,
+                 me.ck_corner, me.ck_count, me.ck_maxdeg};
+  a.c00 = (float)mat.c00; a.c11 = (float)mat.c11; a.c01 = (float)mat.c01; a.g = (float)mat.g;
+  a.thick = (float)mat.thick;
+  a.inv_h2 = (float)inv_h2;
+  a.x = M.x32.ptr; a.p = M.p.ptr; a.Ap = M.Ap.ptr;
+  a.s = M.s.ptr;
+  a.partials = M.partials.ptr;
+  CS_HIP(hipMemsetAsync(&M.s.ptr->done, 0, sizeof(int32_t), st));
+  if (a.me.ck_corner) {
+    M.tri.alloc((size_t)std::max<uint32_t>(a.me.m, 1) * 3);
+    a.tri = M.tri.ptr;
+    hipLaunchKernelGGL(k_mx_tri_prep, dim3(grid_for(std::max<uint32_t>(a.me.m, 1))), dim3(kBlock), 0, st, a,
+                       M.tri.ptr);
+  }
+  launch_mx_hvp(a, st);
+  hipLaunchKernelGGL(k_mx_out, dim3(grid_for(n3)), dim3(kBlock), 0, st, M.Ap.ptr, out, n3);
+  CS_LAUNCH_CHECK();
+  CS_HIP(hipStreamSynchronize(st));
+}
+
 // Back-to-back HVP launches on the operands of the last solve (inputs stay
 // resident; Ap and the p.Ap partials are overwritten).
 double MixedSolve::time_hvp(int reps) {

@@ -1205,6 +1205,23 @@ int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, i
   });
 }
 
+int cs_probe_hvp(cs_solver* s, const double* x, const double* p, double inv_h2, double* out) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.mixed) return fail(CS_E_INVALID, "solver was not created in mixed precision");
+  if (!x || !p || !out) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    S.upload_pins();
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.p.ptr, p, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.mixed->probe_hvp(S.dmesh(), S.dmat(), inv_h2, S.x.ptr, S.p.ptr, S.Ap.ptr);
+    CS_HIP(hipMemcpy(out, S.Ap.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return CS_OK;
+  });
+}
+
 int cs_probe_schwarz(cs_solver* s, const double* x, const double* diag, const double* r, double* z64, double* z32,
                      uint8_t* flags64, uint8_t* flags32, uint32_t* domains) {
   if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");

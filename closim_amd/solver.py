@@ -63,6 +63,7 @@ def load_library(path=LIB_PATH):
         "cs_batch_step": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_int32)]),
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
+        "cs_probe_hvp": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp]),
         "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
@@ -270,6 +271,17 @@ class HipSolver:
         return a.value, r.value
 
     # ---- component probes (parity tests) ------------------------------------
+    def probe_hvp(self, x, p, inv_h2=0.0):
+        """One launch of the mixed solve's FP32 HVP kernel at (x, p), no
+        contact, pinned rows zeroed (cs_probe_hvp)."""
+        self._need()
+        x = np.ascontiguousarray(x, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        out = np.zeros_like(x)
+        _check(self._lib.cs_probe_hvp(self._h, x.ctypes.data_as(_dp), p.ctypes.data_as(_dp), float(inv_h2),
+                                      out.ctypes.data_as(_dp)))
+        return out
+
     def probe_active_set(self, x0, x, cap=1 << 20):
         self._need()
         x0 = np.ascontiguousarray(x0, np.float64)

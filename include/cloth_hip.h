@@ -281,6 +281,13 @@ int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double*
 int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
                          uint32_t* local_of, uint32_t* domains);
 
+/* Component probe (membraneHvp with the analytic Hessian,
+ * membrane-hvp.ts:18-108, plus the inertia term): one launch of the mixed
+ * solve's FP32 HVP kernel at (x, p), no contact, pinned rows zeroed:
+ * out = M p inv_h2 + H_membrane(x) p in FP32, returned as FP64.  Mixed
+ * precision solvers only. */
+int cs_probe_hvp(cs_solver* s, const double* x, const double* p, double inv_h2, double* out);
+
 /* buildSchwarzFactors + applySchwarz (src/solver/schwarz.ts:154-309) with the
  * solver's domains (created with CS_PRECOND_SCHWARZ) at positions x and
  * Jacobi diagonal diag: z64 = the FP64 apply in the reference's order, z32 =
