@@ -211,10 +211,12 @@ def adjoint_section(steps, h, fd_eps=1e-3):
     for prec in ("fp64", "mixed"):
         sc, th_true, th_guess = A.material_problem(127)
         g = A.converged_solver(sc, prec, newton_iters=400, pcg_max_iters=100)
-        r = A.MaterialRollout(g, steps=steps, h=h, checkpoint_every=5)
+        r = A.MaterialRollout(g, steps=steps, h=h, checkpoint_every=5, strict=False,
+                              progress=lambda m: print(f"[bench] adjoint {prec}: {m}", file=sys.stderr, flush=True))
         t0 = time.perf_counter()
         targets = r.run(th_true)
         t_fwd = time.perf_counter() - t0
+        print(f"[bench] adjoint {prec}: target rollout {t_fwd:.1f} s", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         L, grad = A.loss_and_grad(r, th_guess, targets)
         t_fb = time.perf_counter() - t0
@@ -233,7 +235,11 @@ def adjoint_section(steps, h, fd_eps=1e-3):
                 tp, tm = th_guess.copy(), th_guess.copy()
                 tp[j] += e
                 tm[j] -= e
-                fd = (A.trajectory_loss(r.run(tp), targets) - A.trajectory_loss(r.run(tm), targets)) / (2 * e)
+                lp = A.trajectory_loss(r.run(tp), targets)
+                print(f"[bench] adjoint fd {A.PARAMS[j]} +", file=sys.stderr, flush=True)
+                lm = A.trajectory_loss(r.run(tm), targets)
+                print(f"[bench] adjoint fd {A.PARAMS[j]} -", file=sys.stderr, flush=True)
+                fd = (lp - lm) / (2 * e)
                 res["fd"][A.PARAMS[j]] = dict(rel_step=fd_eps, fd=fd, adjoint=float(grad[j]),
                                               rel_err=abs(fd - grad[j]) / abs(fd))
             res["max_final_grad_norm_fd_rollouts"] = r.max_grad_norm

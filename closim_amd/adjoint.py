@@ -48,8 +48,13 @@ class MaterialRollout:
     scene's current state, differentiable in the material parameters."""
 
     def __init__(self, solver: HipSolver, steps: int, h: float, checkpoint_every: int = 5,
-                 adjoint_tol: float = 1e-10, adjoint_iters: int = 4000):
+                 adjoint_tol: float = 1e-10, adjoint_iters: int = 4000, progress=None, strict=True):
         self.solver = solver
+        # strict: raise when an adjoint solve ends above 100x its tolerance (a
+        # Hessian that is not positive definite along the search, typically
+        # at a forward step that did not reach its root)
+        self.strict = strict
+        self.progress = progress  # optional callable(text) after every checkpoint segment
         self.steps = int(steps)
         self.h = float(h)
         self.k = max(1, int(checkpoint_every))
@@ -83,6 +88,8 @@ class MaterialRollout:
                 self.checkpoints[t] = (x.copy(), v.copy())
             x, v = self._step()
             out[t] = x
+            if self.progress and (t + 1) % self.k == 0:
+                self.progress(f"forward step {t + 1}/{self.steps}")
         return out
 
     def backward(self, dL_dxs, dL_dvs=None):
@@ -112,8 +119,13 @@ class MaterialRollout:
                 dx, dv, dth, it, rr = self.solver.step_adjoint(self.h, xs[j], vs[j], xs[j + 1], gx, gv, self.tol,
                                                                self.iters)
                 self.adjoint_iters.append((it, rr))
+                if self.strict and not rr <= 100 * self.tol:
+                    raise RuntimeError(f"adjoint solve of step {t} ended at relative residual {rr:.2e} "
+                                       f"after {it} iterations (forward |grad| up to {self.max_grad_norm:.2e})")
                 gtheta += dth
                 gx, gv = dx, dv
+            if self.progress:
+                self.progress(f"backward segment from step {s}")
         self.dL_dx0, self.dL_dv0 = gx, gv
         return gtheta
 
