@@ -24,6 +24,7 @@ CS_PRECOND_SCHWARZ = 2
 CS_FLAG_TIMING = 1
 CS_FLAG_POLISH_ROOT = 2
 CS_FLAG_EAGER = 4
+CS_FLAG_PSD_NEWTON = 8
 
 CS_KERNEL_HVP = 0
 CS_KERNEL_GRADIENT = 1

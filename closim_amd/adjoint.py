@@ -210,10 +210,12 @@ def corner_pinned_sheet(nx=127, side=1.0, material=None):
 
 def converged_solver(scene, precision="fp64", grad_tol=1e-13, newton_iters=200, pcg_max_iters=600, **kw):
     """A HipSolver whose steps converge to the root: a small grad_tol, enough
-    Newton and PCG iterations, and root polish (CS_FLAG_POLISH_ROOT) past the
-    point where the Armijo test stops resolving energy decreases."""
+    Newton and PCG iterations, a positive semidefinite Newton operator
+    (CS_FLAG_PSD_NEWTON) so that Newton descends through buckling, and root
+    polish (CS_FLAG_POLISH_ROOT) past the point where the Armijo test stops
+    resolving energy decreases."""
     g = HipSolver(precision, hvp_mode=_abi.CS_HVP_ANALYTIC, grad_tol=grad_tol, newton_iters=newton_iters,
-                  pcg_max_iters=pcg_max_iters, flags=_abi.CS_FLAG_POLISH_ROOT, **kw)
+                  pcg_max_iters=pcg_max_iters, flags=_abi.CS_FLAG_POLISH_ROOT | _abi.CS_FLAG_PSD_NEWTON, **kw)
     g.initialize(scene)
     return g
 

@@ -61,9 +61,30 @@ __device__ __forceinline__ void tri_grad_corner(const TriState& s, double area, 
 }
 
 // Analytic HVP contribution of `corner` (membrane-hvp.ts:18-75).
+// The 2x2 symmetric stress [[s00, s01], [s01, s11]] with its negative
+// eigenvalues set to zero (tension only): the geometric term dF S of the
+// membrane Hessian becomes positive semidefinite.  Not in the reference; used
+// for the Newton direction under CS_FLAG_PSD_NEWTON.
+template <class T>
+__device__ __forceinline__ void stress_psd(T& s00, T& s11, T& s01) {
+  const T mean = T(0.5) * (s00 + s11), half = T(0.5) * (s00 - s11);
+  const T rad = sqrt(half * half + s01 * s01);
+  const T l1 = mean + rad, l2 = mean - rad;
+  if (l2 >= T(0)) return;
+  if (l1 <= T(0)) {
+    s00 = s11 = s01 = T(0);
+    return;
+  }
+  // S+ = l1 v1 v1^T = l1 / (l1 - l2) (S - l2 I)
+  const T k = l1 / (l1 - l2);
+  s00 = k * (s00 - l2);
+  s11 = k * (s11 - l2);
+  s01 = k * s01;
+}
+
 __device__ __forceinline__ void tri_hvp_corner(const TriState& s, const double* v0, const double* v1,
                                                const double* v2, double area, const DMat& mat, int corner,
-                                               double* out) {
+                                               double* out, bool psd = false) {
   const double w1x = v1[0] - v0[0], w1y = v1[1] - v0[1], w1z = v1[2] - v0[2];
   const double w2x = v2[0] - v0[0], w2y = v2[1] - v0[1], w2z = v2[2] - v0[2];
   const double a = s.a, b = s.b, c = s.c, d = s.d;
@@ -78,9 +99,10 @@ __device__ __forceinline__ void tri_hvp_corner(const TriState& s, const double* 
   const double dC11 = 2 * (F01 * dF01 + F11 * dF11 + F21 * dF21);
   const double dC01 = dF00 * F01 + F00 * dF01 + dF10 * F11 + F10 * dF11 + dF20 * F21 + F20 * dF21;
   const double dE00 = 0.5 * dC00, dE11 = 0.5 * dC11, dE01 = 0.5 * dC01;
-  const double S00 = mat.c00 * E00 + mat.c01 * E11;
-  const double S11 = mat.c11 * E11 + mat.c01 * E00;
-  const double S01 = 2 * mat.g * E01;
+  double S00 = mat.c00 * E00 + mat.c01 * E11;
+  double S11 = mat.c11 * E11 + mat.c01 * E00;
+  double S01 = 2 * mat.g * E01;
+  if (psd) stress_psd(S00, S11, S01);
   const double dS00 = mat.c00 * dE00 + mat.c01 * dE11;
   const double dS11 = mat.c11 * dE11 + mat.c01 * dE00;
   const double dS01 = 2 * mat.g * dE01;

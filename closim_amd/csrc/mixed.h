@@ -22,6 +22,7 @@ struct MixedInputs {
   double tol;
   Schwarz* sw = nullptr;  // additive Schwarz with FP32 factors (else binv / diag)
   bool graph = false;     // replay the iterations as a captured HIP graph (no per-kernel timing)
+  bool psd = false;       // stress projected to PSD in the HVP (CS_FLAG_PSD_NEWTON)
 };
 
 struct MixedImpl;

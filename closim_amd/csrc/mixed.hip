@@ -5,6 +5,7 @@
 // (membrane-hvp.ts:18-108) evaluated in FP32 and FP32 reductions.  The
 // per-vertex gather over the element-ordered CSR keeps the result
 // deterministic run to run.
+#include "elements.h"
 #include "fem.h"
 #include "mixed.h"
 #include "schwarz.h"
@@ -31,6 +32,7 @@ struct MxArgs {
   float c00, c11, c01, g, thick, inv_h2;
   const float* x;
   const float4* tri;  // k_mx_tri_prep output (chunked HVP), 3 per triangle
+  int psd;            // stress projected to PSD (stress_psd)
   const float* diag;
   const float* binv;  // null => Jacobi
   float* sol;
@@ -118,7 +120,8 @@ __device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const fl
   const float dE11 = F1[0] * dF1[0] + F1[1] * dF1[1] + F1[2] * dF1[2];
   const float dE01 = 0.5f * (dF0[0] * F1[0] + F0[0] * dF1[0] + dF0[1] * F1[1] + F0[1] * dF1[1] + dF0[2] * F1[2] +
                              F0[2] * dF1[2]);
-  const float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+  float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+  if (a.psd) stress_psd(S00, S11, S01);
   const float dS00 = a.c00 * dE00 + a.c01 * dE11, dS11 = a.c11 * dE11 + a.c01 * dE00, dS01 = 2.f * a.g * dE01;
   const float sc = a.thick * a.me.area[t];
 #pragma unroll
@@ -155,7 +158,8 @@ __global__ void __launch_bounds__(kBlock) k_mx_tri_prep(MxArgs a, float4* __rest
   const float E00 = 0.5f * (F0[0] * F0[0] + F0[1] * F0[1] + F0[2] * F0[2] - 1.f);
   const float E11 = 0.5f * (F1[0] * F1[0] + F1[1] * F1[1] + F1[2] * F1[2] - 1.f);
   const float E01 = 0.5f * (F0[0] * F1[0] + F0[1] * F1[1] + F0[2] * F1[2]);
-  const float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+  float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+  if (a.psd) stress_psd(S00, S11, S01);
   tri[(size_t)t * 3] = make_float4(F0[0], F0[1], F0[2], F1[0]);
   tri[(size_t)t * 3 + 1] = make_float4(F1[1], F1[2], S00, S11);
   tri[(size_t)t * 3 + 2] = make_float4(S01, a.thick * a.me.area[t], 0.f, 0.f);
@@ -654,6 +658,7 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.inv_h2 = (float)in.inv_h2;
   a.x = M.x32.ptr; a.diag = M.d32.ptr; a.binv = in.binv ? M.binv32.ptr : nullptr;
   a.tri = nullptr;
+  a.psd = in.psd ? 1 : 0;
   a.sol = M.sol.ptr; a.r = M.r.ptr; a.z = M.z.ptr; a.p = M.p.ptr; a.Ap = M.Ap.ptr;
   a.tol = (float)in.tol;
   a.ext = in.sw ? 1 : 0;

@@ -56,6 +56,7 @@ struct PcgArgs {
   double* p;
   double* Ap;
   int hvp_mode;        // HVP_FD / HVP_ANALYTIC
+  int psd;             // HVP_ANALYTIC with the stress projected to PSD (stress_psd)
   int precond;         // PrecondKind
   double tol;
   PcgScalars* s;

@@ -83,7 +83,7 @@ __device__ __forceinline__ void hvp_corner(const PcgArgs& a, uint4 cr, const dou
     tri_grad_corner(sm, area, a.mat, corner, g1);
   } else {
     const TriState s = tri_state(x0, x1, x2, inv);
-    tri_hvp_corner(s, p0, p1, p2, area, a.mat, corner, g0);
+    tri_hvp_corner(s, p0, p1, p2, area, a.mat, corner, g0, a.psd != 0);
     g1[0] = g1[1] = g1[2] = 0;
   }
 }

@@ -755,6 +755,7 @@ struct Solver {
     a.sol = dx.ptr;
     a.r = r.ptr; a.z = z.ptr; a.p = p.ptr; a.Ap = Ap.ptr;
     a.hvp_mode = opt.hvp_mode == CS_HVP_ANALYTIC ? HVP_ANALYTIC : HVP_FD;
+    a.psd = (opt.flags & CS_FLAG_PSD_NEWTON) && a.hvp_mode == HVP_ANALYTIC ? 1 : 0;
     a.precond = opt.precond == CS_PRECOND_BLOCK_JACOBI ? PRE_BLOCK
                 : opt.precond == CS_PRECOND_SCHWARZ       ? PRE_SCHWARZ
                                                           : PRE_JACOBI;
@@ -792,6 +793,7 @@ struct Solver {
     in.tol = 1e-3;
     int mark = -1;
     in.graph = (opt.flags & (CS_FLAG_TIMING | CS_FLAG_EAGER)) == 0;
+    in.psd = (opt.flags & CS_FLAG_PSD_NEWTON) != 0;
     return mixed->solve(in, [&](int, bool begin) {
       if (begin) mark = ev_mark();
       else ev_span(CS_KERNEL_HVP, mark, ev_mark());

@@ -69,6 +69,13 @@ enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 
  * converges (same kernels, same order, same results); CS_FLAG_TIMING also
  * selects the one-by-one launches. */
 #define CS_FLAG_EAGER 4u
+/* Newton direction with the analytic membrane Hessian whose stress term is
+ * projected to positive semidefinite per triangle (tension only; not in the
+ * reference): PCG then solves a positive definite system and Newton descends
+ * through compressed, buckling states where the unprojected operator stalls.
+ * Takes effect with CS_HVP_ANALYTIC; the gradient, energy, line search and
+ * the adjoint's Hessian are unchanged. */
+#define CS_FLAG_PSD_NEWTON 8u
 
 /* ClothMaterial (src/physics/types.ts:1-14). */
 typedef struct cs_material {
