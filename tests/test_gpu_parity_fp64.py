@@ -124,3 +124,28 @@ def test_schwarz_apply_jacobi_fallback_parity():
     assert np.all(fo == 0) and np.array_equal(f64, fo) and np.array_equal(f32, fo)
     assert np.array_equal(z64, zo)
     assert np.allclose(z32, zo, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("newton,ls", [(0, 10), (1, 10), (3, 1), (10, 0)])
+def test_iteration_caps_match_oracle(newton, ls):
+    # the reference's failure-injection corners (maxNewton = 0 / 1, a single
+    # line-search trial, none at all): the step must still be the oracle's,
+    # on a contact scene where the line search rejects trials
+    run_pair(S.floor_patch, 8, dict(newton_iters=newton, line_search_iters=ls))
+
+
+def test_nonfinite_state_is_reported():
+    # newton.ts:129: a non-finite gradient throws "non-finite gradient"
+    from closim_amd.solver import ClothError
+    sc = S.small_strip()
+    g = HipSolver("fp64")
+    g.initialize(sc)
+    x = sc.positions.copy()
+    free = next(v for v in range(sc.mesh.count) if v not in sc.pinned)
+    x[free * 3 + 1] = np.inf
+    g.set_state(x, sc.velocities)
+    with pytest.raises(ClothError) as e:
+        g.step(H)
+    assert e.value.code == _abi.CS_E_NONFINITE
+    assert "non-finite gradient" in str(e.value)
+    g.close()
