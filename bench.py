@@ -349,7 +349,7 @@ def main():
                     help="timed steps of the 512x512 self-contact fold section on the sphere workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed steps of the 64-instance batched drape section (0 = skip)")
-    ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--adjoint-steps", type=int, default=5,
                     help="rollout steps of the 128x128 material-adjoint section (0 = skip; the workload has 30)")
     ap.add_argument("--copy-calibration", action="store_true",
