@@ -1669,6 +1669,7 @@ struct GridTuner {
   int chosen = -1, cur = -1, turn = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   explicit GridTuner(const char* n) : name(n), ms(caps.size()) {}
+  GridTuner(const char* n, std::vector<unsigned> c) : name(n), caps(std::move(c)), ms(caps.size()) {}
   ~GridTuner() {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1716,6 +1717,8 @@ struct GridTuner {
 
 struct ContactImpl {
   GridTuner tune_trial{"k_trial_pairs"}, tune_active{"k_active_pairs"};
+  // the CCD tail's length is only known on the device: fixed grids
+  GridTuner tune_ccd{"k_trial_ccd", {128, 256, 512, 1024, 2048}};
   hipStream_t st;
   cs_contact_params prm;
   uint32_t n, m, ns, ms;
@@ -2334,14 +2337,19 @@ struct ContactImpl {
         hipLaunchKernelGGL(k_trial_pairs<false>, tg, dim3(kBlock), 0, st, G, xstep, xt, prm.d_min, src.p, src.np,
                            src.s, src.nsp, sbox.ptr, tailq.ptr, cs.ptr);
       tune_trial.after_launch(st);
-      hipLaunchKernelGGL(k_trial_ccd, dim3(256), dim3(kBlock), 0, st, G, xstep, xt, src.p, src.np, src.s, tailq.ptr,
+      const dim3 cg(tune_ccd.grid((size_t)1 << 40, st));
+      hipLaunchKernelGGL(k_trial_ccd, cg, dim3(kBlock), 0, st, G, xstep, xt, src.p, src.np, src.s, tailq.ptr,
                          cs.ptr);
+      tune_ccd.after_launch(st);
     }
     if (has_floor)
       hipLaunchKernelGGL(k_floor_trial, dim3(red_grid(n)), dim3(kBlock), 0, st, G, xstep, xt, dpart.ptr + 2048, cs.ptr);
     CS_LAUNCH_CHECK();
     sync_cs();
-    if (total) tune_trial.collect();
+    if (total) {
+      tune_trial.collect();
+      tune_ccd.collect();
+    }
     candidate_counts(src);
     double minDist = mink_value(5);
     double minTOI = mink_value(6);
@@ -2392,7 +2400,9 @@ cs_contact_diag ContactSystem::diagnostics() const { return im_->diag; }
 uint64_t ContactSystem::active_count() const { return im_->C; }
 
 uint64_t ContactSystem::candidate_count() const { return im_->diag.candidate_pairs; }
-std::string ContactSystem::tuned() const { return im_->tune_trial.describe() + " " + im_->tune_active.describe(); }
+std::string ContactSystem::tuned() const {
+  return im_->tune_trial.describe() + " " + im_->tune_ccd.describe() + " " + im_->tune_active.describe();
+}
 uint64_t ContactSystem::incidence_count() const { return im_->E; }
 uint64_t ContactSystem::incidence_vertices() const { return im_->nv; }
 

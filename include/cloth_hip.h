@@ -251,8 +251,8 @@ int cs_time_hvp(cs_solver* s, int reps, double* mean_ms, double* bytes);
 int cs_hvp_bytes(const cs_solver* s, double* algorithmic, double* read);
 /* (no reference counterpart: measurement hook) The grid caps the launch
  * autotuner chose for the contact kernels whose results do not depend on
- * their grid (k_trial_pairs, k_active_pairs), as text such as
- * "k_trial_pairs=2048 k_active_pairs=1024" ("tuning" while it still measures
+ * their grid (k_trial_pairs, k_trial_ccd, k_active_pairs), as text such as
+ * "k_trial_pairs=2048 k_trial_ccd=256 k_active_pairs=1024" ("tuning" while it still measures
  * the candidates on the first calls; empty without contact). */
 int cs_tuned_launch(const cs_solver* s, char* buf, size_t cap);
 
