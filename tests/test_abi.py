@@ -129,3 +129,15 @@ def test_batch_drapes_workload_is_seeded():
         assert 1e5 <= m.stretchWarp <= 1e7 and 1e5 <= m.stretchWeft <= 1e7 and 5e4 <= m.shear <= 5e6
         assert 1e-6 <= m.bendWarp <= 1e-4 and m.bendWarp == m.bendWeft and m.stretchCoupling == 0.0
         assert x.contact.floorY == 0.0 and x.contact.params.frictionMu == 0.3
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus 2 exits with a message before touching
+    a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
