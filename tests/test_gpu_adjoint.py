@@ -130,3 +130,49 @@ def test_torch_autograd_path_matches_rollout_backward():
     L2, g2 = A.loss_and_grad(r, THETA, targets.numpy())
     assert abs(float(loss.detach()) - L2) <= 1e-14 * abs(L2)
     assert np.array_equal(th.grad.numpy(), g2)
+
+
+def test_one_step_gradients_with_floor_contact_match_fd():
+    # A frictionless patch settled on the floor (every vertex inside the
+    # barrier zone), then stretched and sheared 2% in its plane: the membrane pulls it
+    # back during the step while the barrier holds it up.  The active set
+    # stays the same within the differences' step, so the loss is smooth
+    # there; the adjoint differentiates the barrier with that set frozen.
+    from closim_amd import scenes as S
+    sc = S.floor_patch(mu=0.0)
+    theta = A.theta_of(sc.material)
+    theta[2] = 0.3 * theta[1]
+    g = A.converged_solver(sc, "fp64")
+    g.setMaterial(A.material_with(sc.material, theta))
+    for _ in range(40):
+        g.step(H)
+    P = sc.positions.reshape(-1, 3).copy()
+    c = P.mean(0)
+    P[:, 0] = c[0] + 1.02 * (P[:, 0] - c[0]) + 0.02 * (P[:, 2] - c[2])  # stretch and shear
+    P[:, 2] = c[2] + 1.02 * (P[:, 2] - c[2])
+    x0, v0 = P.reshape(-1), np.zeros(len(sc.velocities))
+    rng = np.random.default_rng(7)
+    wx = rng.standard_normal(len(x0))
+
+    def loss(th):
+        g.setMaterial(A.material_with(sc.material, th))
+        g.set_state(x0, v0)
+        g.step(H)
+        return float(wx @ sc.positions), g.lastStats["contact"]["active_pairs"]
+
+    L0, act0 = loss(theta)
+    assert act0 > 0
+    x1 = sc.positions.copy()
+    dx, dv, dth, it, rr = g.step_adjoint(H, x0, v0, x1, wx, np.zeros_like(wx))
+    print(f"active {act0}, |g| {g.lastStats['grad_norm']:.1e}, adjoint PCG {it} iterations, rel residual {rr:.1e}")
+    assert rr <= 1e-10
+    for j in range(4):  # the membrane moduli (the patch is flat: kb does not act)
+        e = A.fd_step(theta, j, REL_EPS)
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += e
+        tm[j] -= e
+        (lp, ap), (lm, am) = loss(tp), loss(tm)
+        fd = (lp - lm) / (2 * e)
+        print(f"{A.PARAMS[j]}: adjoint {dth[j]:.9e} fd {fd:.9e} rel {rel_err(dth[j], fd):.1e} active {ap}/{am}")
+        assert ap == act0 and am == act0
+        assert rel_err(dth[j], fd) < REL, j
