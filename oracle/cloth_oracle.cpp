@@ -18,6 +18,11 @@
 // Parity status of each part is recorded in tests/test_oracle_*.py, which pin
 // it against the reference's own tests (known answers, FD checks, invariants)
 // and against brute force / numpy where those tests leave it open.
+// Parity unpinned (no reference value or closed form fixes the bits): the FD
+// bending gradient and the FD membrane HVP at round-off level, PCG iteration
+// counts (emergent), and killApproachVelocity beyond its invariants (no
+// approaching normal velocity left, untouched vertices unchanged); for these
+// the claim is "the HIP path is identical to this restatement".
 
 #include "cloth_oracle.h"
 #include "jsmath.h"
