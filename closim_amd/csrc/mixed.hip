@@ -495,7 +495,9 @@ __device__ __forceinline__ float ck_finish(const MxArgs& a, const CkLane& L, int
 // halo) in its own L2 (measured on the 512x512 fold: L2 hits 50k -> 211k,
 // misses 498k -> 234k per launch).  G is a multiple of 8.
 __global__ void __launch_bounds__(kBlock) k_mx_hvp_ck(MxArgs a) {
-  if (block_latched(&a.s->done)) return;
+  // the PCG latch, read by every wave without a barrier: an earlier kernel
+  // set it, so every wave of the grid sees the same value
+  if (__hip_atomic_load(&a.s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
   const int lane = threadIdx.x % kWave;
   const uint32_t wpb = kBlock / kWave;
   const uint32_t G = gridDim.x;
