@@ -131,6 +131,39 @@ __global__ void k_adj_out(const double* lam, const double* gv, const float* mass
   dvp[i] = lam[i] * m / h;
 }
 
+// The vertex-sorted corner records regrouped into 64-lane chunks that each
+// hold whole vertices: a vertex's records go to the current chunk when they
+// fit and start a new one (the rest of the current padded with 0xffffffff)
+// otherwise; a vertex without triangles takes one 0xfffffffe lane; the last
+// chunk is padded.  False when some vertex has more than 64 corners (the
+// chunked HVP is then not used).  `maxdeg` is the largest lane count of one
+// vertex.
+static bool pack_chunks(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint4>& cor,
+                        std::vector<uint4>& ck, uint32_t& maxdeg) {
+  ck.clear();
+  ck.reserve(cor.size() + cor.size() / 8 + kWave);
+  uint32_t fill = 0;
+  maxdeg = 1;
+  for (uint32_t v = 0; v < n; v++) {
+    const uint32_t deg = off[v + 1] - off[v], need = std::max<uint32_t>(deg, 1);
+    if (need > kWave) {
+      ck.clear();
+      return false;
+    }
+    if (fill + need > kWave) {
+      for (; fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+      fill = 0;
+    }
+    if (deg == 0) ck.push_back(make_uint4(v, v, v, 0xfffffffeu));
+    else
+      for (uint32_t q = off[v]; q < off[v + 1]; q++) ck.push_back(cor[q]);
+    fill += need;
+    maxdeg = std::max(maxdeg, need);
+  }
+  for (; fill > 0 && fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+  return true;
+}
+
 struct Solver {
   int device = 0;
   hipStream_t st = nullptr;
@@ -289,27 +322,8 @@ struct Solver {
       vt_corner.upload(cor.data(), cor.size(), st);
       // the same records in 64-lane chunks of whole vertices (the mixed HVP)
       std::vector<uint4> ck;
-      ck.reserve(cor.size() + cor.size() / 8 + 64);
-      uint32_t fill = 0, maxdeg = 1;
-      bool ok = true;
-      for (uint32_t v = 0; v < n && ok; v++) {
-        const uint32_t deg = off[v + 1] - off[v], need = std::max<uint32_t>(deg, 1);
-        if (need > kWave) {
-          ok = false;
-          break;
-        }
-        if (fill + need > kWave) {
-          for (; fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
-          fill = 0;
-        }
-        if (deg == 0) ck.push_back(make_uint4(v, v, v, 0xfffffffeu));
-        else
-          for (uint32_t q = off[v]; q < off[v + 1]; q++) ck.push_back(cor[q]);
-        fill += need;
-        maxdeg = std::max(maxdeg, need);
-      }
-      if (ok && n) {
-        for (; fill > 0 && fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+      uint32_t maxdeg = 0;
+      if (pack_chunks(n, off, cor, ck, maxdeg) && n) {
         ck_corner.upload(ck.data(), ck.size(), st);
         ck_count = (uint32_t)(ck.size() / kWave);
         ck_maxdeg = maxdeg;
@@ -1203,6 +1217,45 @@ int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, i
   return guarded([&]() -> int {
     const int D = cloth::schwarz_partition_host(*scene, target, min_size, domain_of, local_of);
     if (domains) *domains = (uint32_t)D;
+    return CS_OK;
+  });
+}
+
+int cs_hvp_chunks(const cs_scene_desc* scene, uint32_t* records, uint64_t cap, uint64_t* lanes,
+                  uint32_t* maxdeg) {
+  if (!scene || !scene->indices || !lanes || !maxdeg) return fail(CS_E_INVALID, "null argument");
+  for (size_t k = 0; k < (size_t)scene->n_tris * 3; k++)
+    if (scene->indices[k] >= scene->n_verts) return fail(CS_E_INVALID, "triangle index out of range");
+  return guarded([&]() -> int {
+    const uint32_t n = scene->n_verts, m = scene->n_tris;
+    std::vector<uint32_t> off(n + 1, 0), rec((size_t)m * 3);
+    for (uint32_t e = 0; e < m; e++)
+      for (int c = 0; c < 3; c++) off[scene->indices[(size_t)e * 3 + c] + 1]++;
+    for (uint32_t v = 0; v < n; v++) off[v + 1] += off[v];
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t e = 0; e < m; e++)
+      for (int c = 0; c < 3; c++) rec[cur[scene->indices[(size_t)e * 3 + c]]++] = (e << 2) | (uint32_t)c;
+    std::vector<uint4> cor(rec.size());
+    for (size_t q = 0; q < rec.size(); q++) {
+      const size_t t = rec[q] >> 2;
+      cor[q] = make_uint4(scene->indices[t * 3], scene->indices[t * 3 + 1], scene->indices[t * 3 + 2], rec[q]);
+    }
+    std::vector<uint4> ck;
+    uint32_t md = 0;
+    if (!cloth::pack_chunks(n, off, cor, ck, md)) {
+      *lanes = 0;
+      *maxdeg = 0;
+      return CS_OK;
+    }
+    *lanes = ck.size();
+    *maxdeg = md;
+    if (records)
+      for (size_t k = 0; k < ck.size() && k < cap; k++) {
+        records[k * 4] = ck[k].x;
+        records[k * 4 + 1] = ck[k].y;
+        records[k * 4 + 2] = ck[k].z;
+        records[k * 4 + 3] = ck[k].w;
+      }
     return CS_OK;
   });
 }

@@ -288,6 +288,16 @@ int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double*
 int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
                          uint32_t* local_of, uint32_t* domains);
 
+/* (no reference counterpart; no device needed) The mixed HVP's chunk layout
+ * for a mesh: the vertex->corner records of incidence.ts:41-93 (ascending
+ * element, each as i0, i1, i2, tri<<2|corner) regrouped into 64-lane chunks
+ * of whole vertices, padding lanes 0xffffffff in the last word, a vertex
+ * without triangles one lane (v, v, v, 0xfffffffe).  `lanes` receives the
+ * lane count (a multiple of 64; 0 when a vertex has more than 64 corners and
+ * the chunked kernel is not used), `maxdeg` the largest lane count of one
+ * vertex; up to `cap` records (4 words each) go to `records` when non-null. */
+int cs_hvp_chunks(const cs_scene_desc* scene, uint32_t* records, uint64_t cap, uint64_t* lanes, uint32_t* maxdeg);
+
 /* Component probe (membraneHvp with the analytic Hessian,
  * membrane-hvp.ts:18-108, plus the inertia term): one launch of the mixed
  * solve's FP32 HVP kernel at (x, p), no contact, pinned rows zeroed:
