@@ -215,10 +215,11 @@ int cs_set_state(cs_solver* s, const double* x, const double* v, size_t n3);
 /* CpuSolver.lastStats — cpu-solver.ts:9. */
 int cs_last_stats(const cs_solver* s, cs_step_stats* out);
 
-/* Thread-local message of the last failing call on this thread. */
+/* Thread-local message of the last failing call on this thread: the text of
+ * the Error the reference throws (e.g. newton.ts:129, cpu-solver.ts:16). */
 const char* cs_last_error(void);
 
-/* Device name / ABI version (diagnostics only). */
+/* Device name / ABI version (diagnostics only; no reference counterpart). */
 int cs_device_info(int device, char* name, size_t cap);
 int cs_abi_version(void);
 
@@ -325,8 +326,9 @@ int cs_probe_check_trial(cs_solver* s, const double* x0, const double* xt, int32
                          double* min_toi);
 /* Batched primitives on device `device`: kinds[i] 0 = VT (p,a,b,c), 1 = EE
  * (a,b,c,d); ids4 index interleaved xyz arrays x0/x1 of n_verts vertices.
- * cs_probe_ccd -> vtCCD/eeCCD time of impact; cs_probe_closest -> (s, t, dist)
- * of the closest point at x0. */
+ * cs_probe_ccd -> vtCCD/eeCCD time of impact (ccd-vt.ts:20-115,
+ * ccd-ee.ts:24-101); cs_probe_closest -> (s, t, dist) of the closest point at
+ * x0 (closest-point.ts:14-112). */
 int cs_probe_ccd(int device, const double* x0, const double* x1, uint32_t n_verts, const int32_t* kinds,
                  const int32_t* ids4, uint64_t count, double thickness, double* toi);
 int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32_t* kinds, const int32_t* ids4,
