@@ -353,11 +353,13 @@ class HipBatch:
     threads take the instances largest first and step each on its own HIP
     stream, so small instances' kernels overlap on the GPU."""
 
-    def __init__(self, solvers, threads=8):
+    def __init__(self, solvers, threads=None):
         self.solvers = list(solvers)
         if len({id(s) for s in self.solvers}) != len(self.solvers):
             raise ValueError("a solver appears twice in the batch")
-        self.threads = int(threads)
+        # host threads: 16 measured fastest on the 64-sheet batch (4: 0.43 s,
+        # 8: 0.35 s, 16: 0.32 s per batch step), bounded by the host's cores
+        self.threads = int(threads) if threads else max(1, min(16, os.cpu_count() or 8))
         self._lib = load_library()
 
     def step(self, dt):
