@@ -14,6 +14,19 @@
 // which yields the correctly rounded sum independent of summation order
 // (barring a tie within ~2^-100 relative), so a tree reduction on the GPU and
 // a sequential loop on the CPU agree bit for bit.
+//
+// The log, acos, cos (kernel_cos, kernel_sin, rem_pio2) and cbrt routines and
+// their coefficient tables are restated from fdlibm 5.3 and FreeBSD msun
+// (s_cbrt.c), which carry this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
+//   (s_cbrt.c: optimized by Bruce D. Evans.)
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -17,12 +17,15 @@
 //
 // Parity status of each part is recorded in tests/test_oracle_*.py, which pin
 // it against the reference's own tests (known answers, FD checks, invariants)
-// and against brute force / numpy where those tests leave it open.
-// Parity unpinned (no reference value or closed form fixes the bits): the FD
-// bending gradient and the FD membrane HVP at round-off level, PCG iteration
-// counts (emergent), and killApproachVelocity beyond its invariants (no
-// approaching normal velocity left, untouched vertices unchanged); for these
-// the claim is "the HIP path is identical to this restatement".
+// and against brute force / numpy / closed forms where those tests leave it
+// open (block-Jacobi inverses and fallback, pcg() against numpy.linalg, the FD
+// hinge gradient against a complex-step derivative, static pairs against
+// brute force, killApproachVelocity on one-contact closed forms, commit's
+// last write per key, the Armijo decrease of every accepted trial).  What no
+// check can fix is the last bits of the central differences (FD bending
+// gradient, FD membrane HVP): they are pinned to their truncation and
+// round-off bounds only.  orc_set_sum_mode(1) switches every array-wide
+// reduction to the reference's plain left-to-right sum.
 
 #include "cloth_oracle.h"
 #include "jsmath.h"
@@ -332,7 +335,7 @@ double add_hinge_gradient(std::vector<double>& tmp, double* grad, const Hinge& h
       tmp[id] = old - eps;
       const double em = hinge_energy(tmp.data(), h.v0, h.v1, h.v2, h.v3, h.rest, h.len, h.asum, k);
       tmp[id] = old;
-      grad[id] += (ep - em) / (2 * eps);
+      grad[id] += (ep - em) / (g_fault == ORC_FAULT_HINGE_HALF ? eps : 2 * eps);
     }
   }
   return e;
@@ -485,7 +488,7 @@ void build_block_factors(const Mesh& mesh, const Mat& mat, const double* x, cons
     if (ok) {
       const double s = 1 / det;
       o[0] = (B11 * B22 - B12 * B12) * s;
-      o[1] = (B02 * B12 - B01 * B22) * s;
+      o[1] = (g_fault == ORC_FAULT_BJ_COFACTOR ? B01 * B22 - B02 * B12 : B02 * B12 - B01 * B22) * s;
       o[2] = (B01 * B12 - B11 * B02) * s;
       o[3] = (B02 * B12 - B01 * B22) * s;
       o[4] = (B00 * B22 - B02 * B02) * s;
@@ -1351,6 +1354,7 @@ struct Contact {
     auto mn3 = [](double a, double b, double c) { return jmin(jmin(a, b), c); };
     auto mx3 = [](double a, double b, double c) { return jmax(jmax(a, b), c); };
     const float* s = staticPos.data();
+    if (g_fault == ORC_FAULT_STATIC_NO_PAD) pad = 0;
     return {mn3(s[i0 * 3], s[i1 * 3], s[i2 * 3]) - pad, mn3(s[i0 * 3 + 1], s[i1 * 3 + 1], s[i2 * 3 + 1]) - pad,
             mn3(s[i0 * 3 + 2], s[i1 * 3 + 2], s[i2 * 3 + 2]) - pad, mx3(s[i0 * 3], s[i1 * 3], s[i2 * 3]) + pad,
             mx3(s[i0 * 3 + 1], s[i1 * 3 + 1], s[i2 * 3 + 1]) + pad, mx3(s[i0 * 3 + 2], s[i1 * 3 + 2], s[i2 * 3 + 2]) + pad};
@@ -1751,6 +1755,9 @@ struct Solver {
   bool hasLast = false;
   SchwarzDomains domains;
   std::vector<std::vector<uint32_t>> v2t;
+  // Accepted Armijo trials of the last step: (energy before, energy after,
+  // alpha, g.dx) per Newton iteration that moved (newton.ts:204-208).
+  std::vector<std::array<double, 4>> armijo;
 
   long pin_index(uint32_t id) const {
     for (size_t i = 0; i < pinned.size(); i++)
@@ -1831,7 +1838,7 @@ PcgResult pcg(const std::vector<double>& b, Hvp hvp, const std::vector<double>& 
     if (residual < absTol) return {x, k + 1, residual, true};
     apply_m();
     const double rzNew = dot_cr(r.data(), z.data(), n);
-    const double beta = rzNew / rz;
+    const double beta = g_fault == ORC_FAULT_PCG_BETA ? rz / rzNew : rzNew / rz;
     for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
     rz = rzNew;
   }
@@ -1863,7 +1870,7 @@ void kill_approach_velocity(Contact& contact, Solver& sc, const double* x) {
       }
       const double rel = (vp[0] - tx) * c.nx + (vp[1] - ty) * c.ny + (vp[2] - tz) * c.nz;
       if (rel < 0) {
-        const double j = -0.5 * rel;
+        const double j = (g_fault == ORC_FAULT_KILL_VT_QUARTER ? -0.25 : -0.5) * rel;
         v[p * 3] += j * c.nx;
         v[p * 3 + 1] += j * c.ny;
         v[p * 3 + 2] += j * c.nz;
@@ -1929,6 +1936,7 @@ cs_step_stats implicit_step(Solver& sc, double h) {
   }
   std::vector<double> x = yHat;
   sc.enforce_pins(x.data());
+  sc.armijo.clear();
   PinFilter filter;
   for (const auto& pr : sc.pinned) filter.ids.push_back(pr.first);
   Contact* contact = sc.hasContact ? &sc.contact : nullptr;
@@ -2069,6 +2077,7 @@ cs_step_stats implicit_step(Solver& sc, double h) {
         continue;
       }
       if (eNew <= energy + 1e-4 * alpha * gtdx) {
+        sc.armijo.push_back({energy, eNew, alpha, gtdx});
         x = xt;
         energy = eNew;
         accepted = true;
@@ -2407,6 +2416,118 @@ int orc_schwarz_apply(orc_solver* o, int target, int minsz, int maxsz, const dou
   apply_schwarz(f, d, r, z, s.mesh.n);
   std::copy(f.flag.begin(), f.flag.end(), flags);
   return (int)d.members.size();
+}
+
+void orc_set_sum_mode(int faithful) { orcm::g_faithful_sums = faithful != 0; }
+
+int64_t orc_armijo_trace(orc_solver* o, double* out4, int64_t cap) {
+  const auto& a = o->s.armijo;
+  for (int64_t k = 0; k < (int64_t)a.size() && k < cap; k++)
+    for (int j = 0; j < 4; j++) out4[k * 4 + j] = a[k][j];
+  return (int64_t)a.size();
+}
+
+// buildBlockFactors() block-jacobi.ts:30-87 at x with Jacobi diagonal `diag`.
+void orc_block_factors(orc_solver* o, const double* x, const double* diag, double* inv9n, uint8_t* flags) {
+  std::vector<double> inv;
+  std::vector<uint8_t> flag;
+  build_block_factors(o->s.mesh, o->s.mat, x, diag, inv, flag);
+  std::copy(inv.begin(), inv.end(), inv9n);
+  std::copy(flag.begin(), flag.end(), flags);
+}
+
+// applyBlockFactors() block-jacobi.ts:90-104.
+void orc_apply_block_factors(const double* inv9n, const double* r, double* z, uint32_t n) {
+  const std::vector<double> inv(inv9n, inv9n + (size_t)n * 9);
+  apply_block_factors(inv, r, z, n);
+}
+
+// pcg() pcg.ts:11-95 with a dense operator out = A p (row-major ndof x ndof),
+// the Jacobi preconditioner on `diag` and the pin filter of `pins` (vertex
+// ids, xyz rows zeroed).  Returns converged; *iters / *residual as PcgResult.
+int orc_pcg_dense(const double* A, uint32_t ndof, const double* b, const double* diag, const uint32_t* pins,
+                  uint32_t npins, int max_iters, double tol, double* x, int* iters, double* residual) {
+  PinFilter filter;
+  for (uint32_t k = 0; k < npins; k++) filter.ids.push_back(pins[k]);
+  auto hvp = [&](const double* p, double* out) {
+    for (uint32_t i = 0; i < ndof; i++) {
+      double acc = 0;
+      for (uint32_t j = 0; j < ndof; j++) acc += A[(size_t)i * ndof + j] * p[j];
+      out[i] = acc;
+    }
+  };
+  struct NoPre { void operator()(const double*, double*) const {} };
+  const std::vector<double> bb(b, b + ndof), dd(diag, diag + ndof);
+  const PcgResult r = pcg(bb, hvp, dd, max_iters, tol, filter, (NoPre*)nullptr);
+  std::copy(r.x.begin(), r.x.end(), x);
+  if (iters) *iters = r.iters;
+  if (residual) *residual = r.residual;
+  return r.converged ? 1 : 0;
+}
+
+// addHingeGradient() bending.ts:19-42 on one hinge whose four vertices are
+// x12 = (x_v0, x_v1, x_v2, x_v3); grad12 receives the FD gradient, the return
+// value is hingeEnergy.
+double orc_hinge_grad_fd(const double* x12, double rest, double len, double asum, double k, double* grad12) {
+  std::vector<double> tmp(x12, x12 + 12);
+  for (int i = 0; i < 12; i++) grad12[i] = 0;
+  const Hinge h{0, 1, 2, 3, rest, len, asum};
+  return add_hinge_gradient(tmp, grad12, h, k);
+}
+
+// queryBvhAll() contact-assembly.ts:124-140 with the static boxes of
+// staticTriAabb() :597-605: (cloth triangle, static triangle) pairs, sorted.
+int64_t orc_static_pairs(const double* x0, const double* x1, const uint32_t* idx, uint32_t m, const float* spos,
+                         uint32_t ns, const uint32_t* sidx, uint32_t ms, double pad, uint32_t* pairs, int64_t cap) {
+  Contact c;
+  c.staticPos.assign(spos, spos + (size_t)ns * 3);
+  c.staticIdx.assign(sidx, sidx + (size_t)ms * 3);
+  c.bvh.build(x0, x1, idx, m, pad);
+  std::vector<Aabb> boxes(ms);
+  for (uint32_t t = 0; t < ms; t++) boxes[t] = c.static_tri_aabb(t, pad);
+  const auto out = c.query_bvh_all(boxes);
+  for (int64_t k = 0; k < (int64_t)out.size() && k < cap; k++) {
+    pairs[k * 2] = out[k].first;
+    pairs[k * 2 + 1] = out[k].second;
+  }
+  return (int64_t)out.size();
+}
+
+// killApproachVelocity() newton.ts:238-303: begin a step at x0, build the
+// active set at x and filter the velocities v (3n, in place).
+int orc_kill_velocity(orc_solver* o, const double* x0, const double* x, double* v) {
+  Solver& s = o->s;
+  if (!s.hasContact) return -1;
+  const size_t n3 = (size_t)s.mesh.n * 3;
+  s.contact.begin_step(std::vector<double>(x0, x0 + n3));
+  s.velocities.assign(v, v + n3);
+  kill_approach_velocity(s.contact, s, x);
+  std::copy(s.velocities.begin(), s.velocities.end(), v);
+  return (int)s.contact.active.size();
+}
+
+// commit() contact-assembly.ts:475-485: begin a step at x0, freeze the active
+// set at x, commit at x and dump the lagged table: key (5 ints, the
+// normalised vtKey/eeKey/floor key) and (nx, ny, nz, lambdaN) per entry, in
+// no particular order.  Returns the entry count.
+int64_t orc_commit_probe(orc_solver* o, const double* x0, const double* x, int64_t* keys5, double* vals4,
+                         int64_t cap) {
+  Solver& s = o->s;
+  if (!s.hasContact) return -1;
+  const size_t n3 = (size_t)s.mesh.n * 3;
+  s.contact.begin_step(std::vector<double>(x0, x0 + n3));
+  s.contact.update_active_set(x, s.mesh.n);
+  s.contact.commit(x);
+  int64_t k = 0;
+  for (const auto& kv : s.contact.lagged) {
+    if (k < cap) {
+      for (int j = 0; j < 5; j++) keys5[k * 5 + j] = kv.first.k[j];
+      vals4[k * 4] = kv.second.nx; vals4[k * 4 + 1] = kv.second.ny;
+      vals4[k * 4 + 2] = kv.second.nz; vals4[k * 4 + 3] = kv.second.lambdaN;
+    }
+    k++;
+  }
+  return k;
 }
 
 }  // extern "C"

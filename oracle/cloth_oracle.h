@@ -91,11 +91,36 @@ int orc_schwarz_domains(orc_solver* s, int target, int minsz, int maxsz, uint32_
 int orc_schwarz_apply(orc_solver* s, int target, int minsz, int maxsz, const double* x, const double* diag,
                       const double* r, double* z, uint8_t* flags);
 
+/* Summation mode of array-wide reductions: 0 (default) correctly rounded
+ * (double-double), 1 the reference's plain left-to-right sums. */
+void orc_set_sum_mode(int faithful);
+/* Accepted Armijo trials of the last orc_step (newton.ts:204-208): 4 doubles
+ * each (energy before, energy after, alpha, g.dx); returns the count. */
+int64_t orc_armijo_trace(orc_solver* s, double* out4, int64_t cap);
+
+/* ---- preconditioners and pcg (src/solver/block-jacobi.ts, src/math/pcg.ts) ---- */
+void orc_block_factors(orc_solver* s, const double* x, const double* diag, double* inv9n, uint8_t* flags);
+void orc_apply_block_factors(const double* inv9n, const double* r, double* z, uint32_t n);
+int orc_pcg_dense(const double* A, uint32_t ndof, const double* b, const double* diag, const uint32_t* pins,
+                  uint32_t npins, int max_iters, double tol, double* x, int* iters, double* residual);
+
+/* ---- bending, static pairs, velocity filter ---- */
+double orc_hinge_grad_fd(const double* x12, double rest, double len, double asum, double k, double* grad12);
+int64_t orc_static_pairs(const double* x0, const double* x1, const uint32_t* idx, uint32_t m, const float* spos,
+                         uint32_t ns, const uint32_t* sidx, uint32_t ms, double pad, uint32_t* pairs, int64_t cap);
+int orc_kill_velocity(orc_solver* s, const double* x0, const double* x, double* v);
+/* commit() (contact-assembly.ts:475-485) after freezing the active set at x:
+ * the lagged table as key5 (kind, ids...) + (nx, ny, nz, lambdaN) entries. */
+int64_t orc_commit_probe(orc_solver* s, const double* x0, const double* x, int64_t* keys5, double* vals4,
+                         int64_t cap);
+
 /* Fault hooks: plant a known mistake in the oracle so tests can show that the
  * checks that pin it are discriminating.  0 = none. */
 void orc_set_fault(int fault);
 enum { ORC_FAULT_NONE = 0, ORC_FAULT_SHEAR_4G = 1, ORC_FAULT_CCD_NO_FALLBACK = 2,
-       ORC_FAULT_BARRIER_SIGN = 3, ORC_FAULT_EE_TRANSPOSE = 4, ORC_FAULT_SCHWARZ_BACK_ROW = 5 };
+       ORC_FAULT_BARRIER_SIGN = 3, ORC_FAULT_EE_TRANSPOSE = 4, ORC_FAULT_SCHWARZ_BACK_ROW = 5,
+       ORC_FAULT_PCG_BETA = 6, ORC_FAULT_BJ_COFACTOR = 7, ORC_FAULT_HINGE_HALF = 8,
+       ORC_FAULT_KILL_VT_QUARTER = 9, ORC_FAULT_STATIC_NO_PAD = 10 };
 
 #ifdef __cplusplus
 }

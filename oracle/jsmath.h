@@ -7,6 +7,19 @@
 //
 // TEST INFRASTRUCTURE ONLY (part of the oracle).  Accuracy is pinned against
 // the C library in tests/test_oracle_math.py.
+//
+// The log, acos, cos (kernel_cos, kernel_sin, rem_pio2) and cbrt routines and
+// their coefficient tables are restated from fdlibm 5.3 and FreeBSD msun
+// (s_cbrt.c), which carry this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
+//   (s_cbrt.c: optimized by Bruce D. Evans.)
 #pragma once
 
 #include <stdint.h>
@@ -359,14 +372,28 @@ ORCM_INLINE void two_prod(double a, double b, double& p, double& e) {
   const double bh = t - (t - b), bl = b - bh;
   e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
 }
+// Summation mode of every array-wide reduction: false (default) rounds the
+// double-double total once; true sums left to right in plain doubles, element
+// by element, which is what the reference's loops (`s += a[i] * b[i]`,
+// `e += ...`) compute.
+inline bool g_faithful_sums = false;
+
 ORCM_INLINE dd dd_zero() { return dd{0.0, 0.0}; }
 ORCM_INLINE void dd_add(dd& acc, double x) {
+  if (g_faithful_sums) {
+    acc.hi = acc.hi + x;
+    return;
+  }
   double s, e;
   two_sum(acc.hi, x, s, e);
   e += acc.lo;
   fast_two_sum(s, e, acc.hi, acc.lo);
 }
 ORCM_INLINE void dd_add_prod(dd& acc, double a, double b) {
+  if (g_faithful_sums) {
+    acc.hi = acc.hi + a * b;
+    return;
+  }
   double p, pe;
   two_prod(a, b, p, pe);
   double s, e;

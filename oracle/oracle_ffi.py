@@ -74,6 +74,17 @@ def lib():
             "orc_schwarz_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                             C.POINTER(C.c_uint8)]),
             "orc_set_fault": (None, [C.c_int]),
+            "orc_set_sum_mode": (None, [C.c_int]),
+            "orc_armijo_trace": (C.c_int64, [C.c_void_p, _dp, C.c_int64]),
+            "orc_block_factors": (None, [C.c_void_p, _dp, _dp, _dp, C.POINTER(C.c_uint8)]),
+            "orc_apply_block_factors": (None, [_dp, _dp, _dp, C.c_uint32]),
+            "orc_pcg_dense": (C.c_int, [_dp, C.c_uint32, _dp, _dp, _up, C.c_uint32, C.c_int, C.c_double, _dp,
+                                        C.POINTER(C.c_int), _dp]),
+            "orc_hinge_grad_fd": (C.c_double, [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+            "orc_static_pairs": (C.c_int64, [_dp, _dp, _up, C.c_uint32, _fp, C.c_uint32, _up, C.c_uint32,
+                                             C.c_double, _up, C.c_int64]),
+            "orc_kill_velocity": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+            "orc_commit_probe": (C.c_int64, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int64), _dp, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -211,6 +222,32 @@ class OracleSolver:
                                       flags.ctypes.data_as(C.POINTER(C.c_uint8)))
         return z, flags[:cnt].copy()
 
+    def armijo_trace(self):
+        cap = 4096
+        out = np.zeros((cap, 4))
+        k = lib().orc_armijo_trace(self.h, P(out), cap)
+        return out[:min(k, cap)].copy()
+
+    def block_factors(self, x, diag):
+        x = d3(x); diag = d3(diag)
+        n = self.n3 // 3
+        inv = np.zeros(9 * n)
+        flags = np.zeros(n, np.uint8)
+        lib().orc_block_factors(self.h, P(x), P(diag), P(inv), flags.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return inv, flags
+
+    def kill_velocity(self, x0, x, v):
+        v = d3(v).copy()
+        k = lib().orc_kill_velocity(self.h, P(d3(x0)), P(d3(x)), P(v))
+        return v, k
+
+    def commit_table(self, x0, x):
+        cap = 1 << 16
+        keys = np.zeros((cap, 5), np.int64)
+        vals = np.zeros((cap, 4))
+        k = lib().orc_commit_probe(self.h, P(d3(x0)), P(d3(x)), P(keys, C.c_int64), P(vals), cap)
+        return keys[:k].copy(), vals[:k].copy()
+
     def check_trial(self, x0, xt):
         md = C.c_double()
         mt = C.c_double()
@@ -226,6 +263,51 @@ def self_pairs(x0, x1, idx, pad):
     while True:
         out = np.zeros((cap, 2), np.uint32)
         k = lib().orc_self_pairs(P(x0), P(x1), P(idx, C.c_uint32), m, pad, P(out, C.c_uint32), cap)
+        if k <= cap:
+            return out[:k].copy()
+        cap = int(k)
+
+
+def set_sum_mode(faithful):
+    lib().orc_set_sum_mode(1 if faithful else 0)
+
+
+def apply_block_factors(inv, r):
+    inv = d3(inv); r = d3(r)
+    z = np.zeros_like(r)
+    lib().orc_apply_block_factors(P(inv), P(r), P(z), len(r) // 3)
+    return z
+
+
+def pcg_dense(A, b, diag, pins=(), max_iters=100, tol=1e-6):
+    A = d3(A); b = d3(b); diag = d3(diag)
+    pins = np.ascontiguousarray(pins, np.uint32)
+    x = np.zeros_like(b)
+    it = C.c_int()
+    res = C.c_double()
+    ok = lib().orc_pcg_dense(P(A), len(b), P(b), P(diag), P(pins, C.c_uint32), len(pins), max_iters, tol, P(x),
+                             C.byref(it), C.byref(res))
+    return x, it.value, res.value, bool(ok)
+
+
+def hinge_grad_fd(x12, rest, ln, asum, k):
+    x12 = d3(x12)
+    g = np.zeros(12)
+    e = lib().orc_hinge_grad_fd(P(x12), rest, ln, asum, k, P(g))
+    return g, e
+
+
+def static_pairs(x0, x1, idx, spos, sidx, pad):
+    x0 = d3(x0); x1 = d3(x1)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    spos = np.ascontiguousarray(spos, np.float32)
+    sidx = np.ascontiguousarray(sidx, np.uint32)
+    cap = 1 << 16
+    while True:
+        out = np.zeros((cap, 2), np.uint32)
+        k = lib().orc_static_pairs(P(x0), P(x1), P(idx, C.c_uint32), len(idx) // 3, P(spos, C.c_float),
+                                   len(spos) // 3, P(sidx, C.c_uint32), len(sidx) // 3, pad,
+                                   P(out, C.c_uint32), cap)
         if k <= cap:
             return out[:k].copy()
         cap = int(k)

@@ -18,8 +18,11 @@ def pytest_configure(config):
 
 @pytest.fixture(autouse=True)
 def _no_planted_fault():
-    """Every test starts and ends with the oracle's fault hook cleared."""
+    """Every test starts and ends with the oracle's fault hook cleared and its
+    reductions correctly rounded."""
     import oracle_ffi
     oracle_ffi.lib().orc_set_fault(0)
+    oracle_ffi.lib().orc_set_sum_mode(0)
     yield
     oracle_ffi.lib().orc_set_fault(0)
+    oracle_ffi.lib().orc_set_sum_mode(0)

@@ -157,7 +157,7 @@ def test_head_on_patches_no_penetration():
     assert math.isfinite(stats[-1]["energy"])
 
 
-def test_strip_pull_energy_monotone_and_pins_exact():
+def test_strip_pull_pins_exact():
     scene = S.strip_pull()
     opts = _abi.options(grad_tol=1e-6)
 
