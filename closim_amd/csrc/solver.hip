@@ -18,6 +18,7 @@
 #include "fem.h"
 #include "mixed.h"
 #include "pcg.h"
+#include "lbvh.h"
 #include "schwarz.h"
 
 namespace cloth {
@@ -1368,6 +1369,23 @@ int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32
   return guarded([&]() -> int {
     CS_HIP(hipSetDevice(device));
     cloth::probe_closest(nullptr, x0, n_verts, kinds, ids4, count, out3);
+    return CS_OK;
+  });
+}
+
+int cs_broadphase_self_pairs(int device, const double* x0, const double* x1, uint32_t n_verts,
+                             const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs, uint64_t cap,
+                             uint64_t* count, double* build_ms, double* query_ms) {
+  if (!count) return fail(CS_E_INVALID, "null count");
+  *count = 0;
+  if (n_tris && (!x0 || !x1 || !indices)) return fail(CS_E_INVALID, "null argument");
+  if (cap && !pairs) return fail(CS_E_INVALID, "null pairs buffer");
+  if (n_tris >= 0x80000000u) return fail(CS_E_INVALID, "too many triangles");
+  if (!std::isfinite(pad) || pad < 0) return fail(CS_E_INVALID, "pad must be finite and non-negative");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    if (!cloth::lbvh_self_pairs(x0, x1, n_verts, indices, n_tris, pad, pairs, cap, count, build_ms, query_ms))
+      return fail(CS_E_OVERFLOW, "candidate pairs exceed the buffer capacity (count holds the number needed)");
     return CS_OK;
   });
 }

@@ -75,6 +75,9 @@ def load_library(path=LIB_PATH):
                                    C.c_uint64, C.c_double, _dp]),
         "cs_probe_closest": (C.c_int, [C.c_int, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                        C.c_uint64, _dp]),
+        "cs_broadphase_self_pairs": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                               C.c_uint32, C.c_double, C.c_void_p, C.c_uint64,
+                                               C.POINTER(C.c_uint64), _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -107,6 +110,35 @@ def schwarz_partition(scene, target=16, min_size=8, max_size=32):
                                                loc.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(cnt)))
     del keep
     return dom, loc, cnt.value
+
+
+def broadphase_self_pairs(x0, x1, idx, pad, device=0, cap=None, timing=False):
+    """BroadPhase.build (broadphase.ts:61-64) of the cloth's self pairs on the
+    device: sorted (tA, tB) pairs as an (k, 2) uint32 array; with timing, also
+    (build_ms, query_ms).  The buffer grows to the reported count on overflow."""
+    from .devmem import DeviceArray
+    L = load_library()
+    x0 = np.ascontiguousarray(x0, np.float64)
+    x1 = np.ascontiguousarray(x1, np.float64)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    n, m = len(x0) // 3, len(idx) // 3
+    d0, d1, di = DeviceArray(x0, device=device), DeviceArray(x1, device=device), DeviceArray(idx, device=device)
+    cap = max(1, 8 * m if cap is None else int(cap))
+    cnt = C.c_uint64()
+    bms, qms = C.c_double(), C.c_double()
+    while True:
+        out = DeviceArray(dtype=np.uint32, shape=(cap, 2), device=device)
+        rc = L.cs_broadphase_self_pairs(device, d0.ptr, d1.ptr, n, di.ptr, m, float(pad), out.ptr, cap,
+                                        C.byref(cnt), C.byref(bms) if timing else None,
+                                        C.byref(qms) if timing else None)
+        if rc == _abi.CS_E_OVERFLOW and cnt.value > cap:
+            cap = int(cnt.value)
+            out.free()
+            continue
+        _check(rc)
+        pairs = out.numpy(cnt.value)
+        out.free()
+        return (pairs, bms.value, qms.value) if timing else pairs
 
 
 def probe_ccd(x0, x1, kinds, ids4, thickness, device=0):

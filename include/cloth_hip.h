@@ -334,6 +334,26 @@ int cs_probe_ccd(int device, const double* x0, const double* x1, uint32_t n_vert
 int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32_t* kinds, const int32_t* ids4,
                      uint64_t count, double* out3);
 
+/* ---- standalone broadphase -------------------------------------------------
+ * BroadPhase.build(x0, x1) (src/collision/broadphase.ts:61-64) for the cloth's
+ * self pairs: the CPU golden TriBvh.build + selfPairs (src/collision/bvh.ts:
+ * 19-124, exclusion of buildAdjacencyExclusions :133-151) and the GPU mirror
+ * GpuBroadPhase.build (src/backend/webgpu/gpu-broadphase.ts:193-388).  Every
+ * pair tA < tB of triangles whose swept boxes sweptTriAabb(x0, x1, pad)
+ * (src/collision/aabb.ts:9-33) overlap and that share no vertex, sorted by
+ * (tA, tB), as `pairs` = 2 u32 each.  x0, x1 (3 n_verts doubles), indices
+ * (3 n_tris u32) and pairs (2 cap u32) are DEVICE pointers on `device`; the
+ * call is synchronous on the null stream.  *count receives the number of
+ * pairs; when it exceeds `cap` nothing is written and CS_E_OVERFLOW is
+ * returned (the reference's explicit candidateOverflow, broadphase.ts:40-41),
+ * so the caller can retry with a buffer of *count pairs.  A corner index >=
+ * n_verts is CS_E_INVALID.  The scratch (about 132 bytes per triangle) is
+ * allocated for the call; build_ms / query_ms (optional) receive the
+ * hierarchy build and the pair query times (HIP events). */
+int cs_broadphase_self_pairs(int device, const double* x0, const double* x1, uint32_t n_verts,
+                             const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs, uint64_t cap,
+                             uint64_t* count, double* build_ms, double* query_ms);
+
 #ifdef __cplusplus
 }
 #endif

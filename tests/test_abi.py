@@ -208,3 +208,17 @@ def test_hvp_chunk_layout(make):
 def test_hvp_chunks_refuse_a_vertex_of_more_than_64_corners():
     rec, md = _chunks(_fan(70))
     assert len(rec) == 0 and md == 0
+
+
+def test_broadphase_arguments_are_rejected_without_a_device():
+    L = load_library()
+    cnt = C.c_uint64(7)
+    assert L.cs_broadphase_self_pairs(0, None, None, 3, None, 1, 1e-3, None, 0, None, None, None) == _abi.CS_E_INVALID
+    assert L.cs_broadphase_self_pairs(0, None, None, 3, None, 1, 1e-3, None, 0, C.byref(cnt), None,
+                                      None) == _abi.CS_E_INVALID
+    assert cnt.value == 0 and b"null" in L.cs_last_error()
+    p = C.c_void_p(8)
+    assert L.cs_broadphase_self_pairs(0, p, p, 3, p, 1, -1.0, p, 1, C.byref(cnt), None, None) == _abi.CS_E_INVALID
+    assert L.cs_broadphase_self_pairs(0, p, p, 3, p, 1, float("nan"), p, 1, C.byref(cnt), None,
+                                      None) == _abi.CS_E_INVALID
+    assert L.cs_broadphase_self_pairs(0, p, p, 3, p, 1, 1e-3, None, 4, C.byref(cnt), None, None) == _abi.CS_E_INVALID
