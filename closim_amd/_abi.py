@@ -26,6 +26,10 @@ CS_FLAG_POLISH_ROOT = 2
 CS_FLAG_EAGER = 4
 CS_FLAG_PSD_NEWTON = 8
 
+CS_COPY_HOST_TO_DEVICE = 1
+CS_COPY_DEVICE_TO_HOST = 2
+CS_COPY_DEVICE_TO_DEVICE = 3
+
 CS_KERNEL_HVP = 0
 CS_KERNEL_GRADIENT = 1
 CS_KERNEL_BROADPHASE = 2

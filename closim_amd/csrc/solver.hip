@@ -1373,6 +1373,42 @@ int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32
   });
 }
 
+int cs_device_alloc(int device, size_t bytes, void** out) {
+  if (!out) return fail(CS_E_INVALID, "null output pointer");
+  *out = nullptr;
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    CS_HIP(hipMalloc(out, bytes ? bytes : 1));
+    return CS_OK;
+  });
+}
+
+int cs_device_free(int device, void* ptr) {
+  if (!ptr) return CS_OK;
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    CS_HIP(hipFree(ptr));
+    return CS_OK;
+  });
+}
+
+int cs_device_copy(int device, void* dst, const void* src, size_t bytes, int direction) {
+  if (!bytes) return CS_OK;
+  if (!dst || !src) return fail(CS_E_INVALID, "null pointer");
+  hipMemcpyKind kind;
+  switch (direction) {
+    case CS_COPY_HOST_TO_DEVICE: kind = hipMemcpyHostToDevice; break;
+    case CS_COPY_DEVICE_TO_HOST: kind = hipMemcpyDeviceToHost; break;
+    case CS_COPY_DEVICE_TO_DEVICE: kind = hipMemcpyDeviceToDevice; break;
+    default: return fail(CS_E_INVALID, "unknown copy direction");
+  }
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    CS_HIP(hipMemcpy(dst, src, bytes, kind));
+    return CS_OK;
+  });
+}
+
 int cs_broadphase_self_pairs(int device, const double* x0, const double* x1, uint32_t n_verts,
                              const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs, uint64_t cap,
                              uint64_t* count, double* build_ms, double* query_ms) {

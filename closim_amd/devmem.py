@@ -1,38 +1,34 @@
-"""Device buffers through the HIP runtime (hipMalloc / hipMemcpy / hipFree)
-for the entry points that take device pointers.  No CPU fallback: without a
-HIP device every call raises."""
+"""Device buffers for the entry points that take device pointers, allocated,
+copied and freed through the cloth library's own HIP runtime
+(cs_device_alloc / cs_device_copy / cs_device_free), so they belong to the
+runtime that runs the kernels even when another copy of the HIP runtime
+(torch's) is loaded in the same process.  No CPU fallback: without a HIP
+device every call raises."""
 import ctypes as C
 
 import numpy as np
 
-_hip = None
-_H2D, _D2H = 1, 2
+from . import _abi
+
+H2D, D2H, D2D = 1, 2, 3
 
 
-def hip():
-    global _hip
-    if _hip is None:
-        L = C.CDLL("libamdhip64.so")
-        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        L.hipFree.argtypes = [C.c_void_p]
-        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        L.hipSetDevice.argtypes = [C.c_int]
-        L.hipDeviceSynchronize.argtypes = []
-        L.hipGetErrorString.restype = C.c_char_p
-        _hip = L
-    return _hip
+def _lib():
+    from .solver import load_library
+    return load_library()
 
 
-def _ok(rc, what):
-    if rc != 0:
-        raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+def _ok(rc):
+    if rc != _abi.CS_OK:
+        from .solver import ClothError
+        raise ClothError(rc, _lib().cs_last_error().decode(errors="replace"))
 
 
 class DeviceArray:
     """A device copy of a numpy array (or an uninitialised buffer of `shape`)."""
 
     def __init__(self, arr=None, dtype=None, shape=None, device=0):
-        _ok(hip().hipSetDevice(device), "hipSetDevice")
+        self.device = device
         if arr is not None:
             arr = np.ascontiguousarray(arr)
             self.dtype, self.shape = arr.dtype, arr.shape
@@ -40,21 +36,22 @@ class DeviceArray:
             self.dtype, self.shape = np.dtype(dtype), tuple(shape)
         self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
         self.ptr = C.c_void_p()
-        _ok(hip().hipMalloc(C.byref(self.ptr), max(self.nbytes, 1)), "hipMalloc")
+        _ok(_lib().cs_device_alloc(device, max(self.nbytes, 1), C.byref(self.ptr)))
         if arr is not None and self.nbytes:
-            _ok(hip().hipMemcpy(self.ptr, arr.ctypes.data, self.nbytes, _H2D), "hipMemcpy H2D")
+            _ok(_lib().cs_device_copy(device, self.ptr, C.c_void_p(arr.ctypes.data), self.nbytes, H2D))
 
     def numpy(self, count=None):
+        """The buffer's contents, or its first `count` rows."""
         out = np.empty(self.shape, self.dtype)
-        nb = self.nbytes if count is None else min(self.nbytes, int(count) * self.dtype.itemsize * int(
-            np.prod(self.shape[1:], dtype=np.int64)))
+        row = self.dtype.itemsize * int(np.prod(self.shape[1:], dtype=np.int64))
+        nb = self.nbytes if count is None else min(self.nbytes, int(count) * row)
         if nb:
-            _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, nb, _D2H), "hipMemcpy D2H")
+            _ok(_lib().cs_device_copy(self.device, C.c_void_p(out.ctypes.data), self.ptr, nb, D2H))
         return out if count is None else out[:count]
 
     def free(self):
         if self.ptr:
-            hip().hipFree(self.ptr)
+            _lib().cs_device_free(self.device, self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):

@@ -75,6 +75,9 @@ def load_library(path=LIB_PATH):
                                    C.c_uint64, C.c_double, _dp]),
         "cs_probe_closest": (C.c_int, [C.c_int, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                        C.c_uint64, _dp]),
+        "cs_device_alloc": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
+        "cs_device_free": (C.c_int, [C.c_int, C.c_void_p]),
+        "cs_device_copy": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
         "cs_broadphase_self_pairs": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                                C.c_uint32, C.c_double, C.c_void_p, C.c_uint64,
                                                C.POINTER(C.c_uint64), _dp, _dp]),

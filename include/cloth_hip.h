@@ -334,6 +334,16 @@ int cs_probe_ccd(int device, const double* x0, const double* x1, uint32_t n_vert
 int cs_probe_closest(int device, const double* x0, uint32_t n_verts, const int32_t* kinds, const int32_t* ids4,
                      uint64_t count, double* out3);
 
+/* ---- device buffers (no reference counterpart) ----------------------------
+ * For callers of the device-pointer entry points below that hold no HIP
+ * runtime of their own (a Node-API addon, a ctypes caller): allocation, a
+ * synchronous copy and release through the library's own runtime, so the
+ * pointers are valid for the same runtime that runs the kernels. */
+enum { CS_COPY_HOST_TO_DEVICE = 1, CS_COPY_DEVICE_TO_HOST = 2, CS_COPY_DEVICE_TO_DEVICE = 3 };
+int cs_device_alloc(int device, size_t bytes, void** out);
+int cs_device_free(int device, void* ptr);
+int cs_device_copy(int device, void* dst, const void* src, size_t bytes, int direction);
+
 /* ---- standalone broadphase -------------------------------------------------
  * BroadPhase.build(x0, x1) (src/collision/broadphase.ts:61-64) for the cloth's
  * self pairs: the CPU golden TriBvh.build + selfPairs (src/collision/bvh.ts:
