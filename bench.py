@@ -1,8 +1,8 @@
 """Benchmark: implicit cloth step on one MI355X, mixed precision vs FP64.
 
-Workload (default): sphere + floor drape, 256x256-vertex sheet (130,050
-triangles, 1 m side) at 0.5 m with seeded +-1e-4 m jitter, static 64x32
-sphere r = 0.25 m, floor, d_hat 1e-3, mu 0.3, h = 1/60, block-Jacobi PCG.
+Workload (default): self-contact fold, 512x512-vertex sheet (522,242
+triangles, 1.5 m side) with a seeded sinusoidal crumple dropped onto the
+floor; d_hat 5e-4, mu 0.4, h = 1/60, additive Schwarz on 64-vertex domains.
 A "step" is one ClothSolver.step(h) of the whole scene.
 
 Prints one JSON line:
@@ -17,8 +17,13 @@ Prints one JSON line:
                actually moves (its index maps and prepared triangle records
                included) are kernel_bytes_per_launch, the counter-measured
                HBM traffic is traffic
-  cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample
-  fold         the 512x512 self-contact fold (additive Schwarz): mixed and
+  lbvh, ccd    the standalone broadphase (Karras LBVH) on the final state:
+               Mprims/s = triangles / hierarchy build time; CCD queries/s =
+               VT + EE primitives of its candidate pairs / kernel time
+  accuracy_converged  mixed vs FP64 where both reach their Newton tolerance
+  cpu_baseline the CPU oracle (FP64, one thread) timed on a bounded sample,
+               plus whole strip-pull steps timed beside the GPU
+  drape        the 256x256 sphere + floor drape (block-Jacobi): mixed and
                FP64 steps/s and its own HVP roofline
   batch        64 uneven drapes stepped together in mixed precision
                (cs_batch_step) vs one instance at a time in FP64
@@ -40,12 +45,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def hip_runtime():
+    """The HIP runtime the cloth library is linked against (torch may map
+    another copy into the process)."""
+    from closim_amd.solver import load_library
+    load_library()
+    path = next((ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln), "libamdhip64.so.7")
+    return C.CDLL(path)
+
+
 def hip_sync():
-    try:
-        lib = C.CDLL("libamdhip64.so")
-        lib.hipDeviceSynchronize()
-    except OSError:
-        pass
+    hip_runtime().hipDeviceSynchronize()
 
 
 def make_scene(name):
@@ -109,17 +119,31 @@ def converged_comparison(steps, h):
     return out
 
 
-def fold_section(steps, h):
-    """The 512x512 self-contact fold (additive Schwarz on 64-vertex domains):
-    mixed and FP64 steps/s from the same initial state, one warmup step each,
-    and the mixed HVP roofline on that mesh."""
+WORKLOADS = {
+    "fold": dict(name="self_contact_fold_512x512", precond="additive-schwarz 64/1/64"),
+    "sphere": dict(name="sphere_floor_drape_256x256", precond="block-jacobi"),
+    "strip": dict(name="strip_pull_20x100", precond="jacobi"),
+}
+
+
+def precond_options(workload):
     from closim_amd import _abi
-    popts = dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64))
+    return {"sphere": dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI),
+            "fold": dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64)),
+            "strip": dict(precond=_abi.CS_PRECOND_JACOBI)}[workload]
+
+
+def pair_section(workload, steps, warmup, h):
+    """A second workload: mixed and FP64 steps/s from the same initial state
+    after `warmup` steps each, the position difference, and the mixed HVP
+    roofline on that mesh."""
+    popts = precond_options(workload)
     res = {}
     for prec in ("mixed", "fp64"):
-        sc = make_scene("fold")
+        sc = make_scene(workload)
         g = solver_for(sc, prec, popts, timing=False)
-        raw_step(g, h)
+        for _ in range(warmup):
+            raw_step(g, h)
         t = timed_steps(g, h, steps)
         st = g.lastStats
         ms, nbytes = g.time_hvp(200)
@@ -131,19 +155,59 @@ def fold_section(steps, h):
                          hvp_algorithmic_bytes=nbytes, hvp_kernel_bytes=rbytes, tuned_launch=tuned,
                          hvp_gbs=nbytes / (ms / 1000) / 1e9 if ms > 0 else 0.0,
                          positions=g.getPositions().copy())
-        print(f"[bench] fold {prec}: {steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
+        print(f"[bench] {workload} {prec}: {steps} steps in {t:.3f} s", file=sys.stderr, flush=True)
         g.close()
     pm, pd = res["mixed"].pop("positions"), res["fp64"].pop("positions")
     m = res["mixed"]
-    return dict(workload="self_contact_fold_512x512", vertices=262144, triangles=522242,
-                preconditioner="additive-schwarz 64/1/64", warmup=1, steps=steps,
+    mesh = make_scene(workload).mesh
+    tr = profile_traffic(workload)
+    return dict(workload=WORKLOADS[workload]["name"], vertices=mesh.count, triangles=mesh.tri_count,
+                preconditioner=WORKLOADS[workload]["precond"], warmup=warmup, steps=steps,
                 mixed=m, fp64=res["fp64"], mixed_over_fp64=res["fp64"]["ms_per_step"] / m["ms_per_step"],
                 pos_rel_err_mixed_vs_fp64=float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd))),
                 roofline_hvp={"bound": "hbm", "achieved": m["hvp_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": m["hvp_gbs"] / HBM_PEAK_GBS, "kernel": "k_mx_hvp_ck",
                               "bytes_per_launch": m["hvp_algorithmic_bytes"],
                               "kernel_bytes_per_launch": m["hvp_kernel_bytes"],
-                              "traffic": profile_traffic("fold")[0], "traffic_source": profile_traffic("fold")[1]})
+                              "traffic": tr[0], "traffic_source": tr[1]})
+
+
+def lbvh_ccd_section(scene, h, reps=5):
+    """The standalone broadphase (cs_broadphase_self_pairs: Morton codes,
+    radix sort, Karras hierarchy, refit; then the pair query) over the swept
+    boxes of the scene's current state -> its predictor, padded by d_hat;
+    median of `reps` timed calls after one untimed call.  Then cs_ccd_queries
+    on every VT / EE primitive of those pairs (expandTriPair), thickness
+    d_min, kernel time of the second call."""
+    from closim_amd.solver import broadphase_self_pairs, ccd_queries, expand_pairs
+    x0 = scene.positions.copy()
+    g = np.asarray(scene.gravity, np.float64)
+    x1 = (x0.reshape(-1, 3) + h * scene.velocities.reshape(-1, 3) + h * h * g).reshape(-1)
+    idx = np.ascontiguousarray(scene.mesh.indices, np.uint32)
+    m = len(idx) // 3
+    pad = scene.contact.params.dHatM
+    pairs = broadphase_self_pairs(x0, x1, idx, pad)
+    builds, queries = [], []
+    for _ in range(reps):
+        _, b, q = broadphase_self_pairs(x0, x1, idx, pad, cap=len(pairs) + 1, timing=True)
+        builds.append(b)
+        queries.append(q)
+    bms, qms = float(np.median(builds)), float(np.median(queries))
+    kinds, ids = expand_pairs(pairs, idx)
+    thick = scene.contact.params.dMinM
+    ccd_queries(x0, x1, kinds, ids, thick)
+    toi, cms = ccd_queries(x0, x1, kinds, ids, thick, timing=True)
+    print(f"[bench] lbvh: {m} triangles, build {bms:.3f} ms, query {qms:.3f} ms, {len(pairs)} pairs; "
+          f"ccd: {len(kinds)} prims in {cms:.3f} ms", file=sys.stderr, flush=True)
+    lbvh = dict(triangles=m, pad=pad, build_ms=bms, query_ms=qms, candidate_pairs=int(len(pairs)),
+                mprims_per_s=m / (bms / 1000) / 1e6, reps=reps,
+                note="build = bounds, Morton codes, radix sort, leaves, Karras hierarchy, refit; query = "
+                     "count, scan, emit, per-triangle sort of the pairs (HIP events, null stream)")
+    ccd = dict(queries=int(len(kinds)), vt=int((kinds == 0).sum()), ee=int((kinds == 1).sum()),
+               thickness=thick, kernel_ms=cms, queries_per_s=len(kinds) / (cms / 1000),
+               impacts=int(np.isfinite(toi).sum()),
+               note="every VT / EE primitive of the broadphase pairs, x0 = state, x1 = predictor")
+    return lbvh, ccd
 
 
 def batch_section(steps, h, threads=8, count=64):
@@ -253,10 +317,10 @@ def adjoint_section(steps, h, fd_eps=1e-3):
     return out
 
 
-def solver_for(scene, precision, popts, timing):
+def solver_for(scene, precision, popts, timing, flags=0):
     from closim_amd import _abi
     from closim_amd.solver import HipSolver
-    flags = _abi.CS_FLAG_TIMING if timing else 0
+    flags = flags | (_abi.CS_FLAG_TIMING if timing else 0)
     g = HipSolver(precision, **popts, flags=flags)
     g.initialize(scene)
     return g
@@ -276,13 +340,15 @@ def timed_steps(g, h, steps):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(name, popts, budget_s, newton_per_step):
+def cpu_baseline(name, popts, budget_s, newton_per_step, h):
     """CPU oracle (FP64, one host thread) on a bounded sample of the same
-    workload.  Small scenes run whole steps until `budget_s` has elapsed.
-    When one step exceeds the budget (the oracle needs ~100 s per step of the
-    256x256 drape), the sample is step 0 with the Newton cap lowered until it
-    fits, and the rate is scaled to whole steps by the Newton iterations per
-    step the GPU run used."""
+    workload.  Scenes whose steps fit the budget run whole steps until
+    `budget_s` has elapsed.  Otherwise (the oracle needs ~50 s per Newton
+    iteration of the 512x512 fold, ~10 per step) the sample is step 0 with
+    the Newton cap at 1 (predictor and CCD bisection, one Newton iteration
+    with its whole PCG solve and line search, velocity update and filter),
+    timed whole, and the rate is scaled to whole steps by the Newton
+    iterations per step the GPU FP64 run used (stated in `sample`)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi as O
     from closim_amd import _abi
@@ -292,33 +358,76 @@ def cpu_baseline(name, popts, budget_s, newton_per_step):
         n = 0
         t0 = time.perf_counter()
         while True:
-            o.step(1 / 60)
+            o.step(h)
             n += 1
             if time.perf_counter() - t0 >= budget_s:
                 break
         dt = time.perf_counter() - t0
         return dict(value=n / dt, unit="steps/s", cores=1, kind="port",
-                    sample=f"first {n} step(s) of the same workload from its initial state "
+                    sample=f"first {n} whole step(s) of the same workload from its initial state "
                            f"(CPU oracle, FP64, one host thread), {dt:.1f} s")
-    cap = 1
-    o = O.OracleSolver(scene, _abi.options(**popts, newton_iters=cap))
+    o = O.OracleSolver(scene, _abi.options(**popts, newton_iters=1))
     t0 = time.perf_counter()
-    o.step(1 / 60)
+    o.step(h)
     dt = time.perf_counter() - t0
-    used = max(o.lastStats["newton_used"], 1)
-    per_newton = dt / used
-    steps_per_s = 1.0 / (per_newton * newton_per_step)
+    st = o.lastStats
+    used = max(st["newton_used"], 1)
+    steps_per_s = 1.0 / (dt / used * newton_per_step)
     return dict(value=steps_per_s, unit="steps/s", cores=1, kind="port",
-                sample=f"step 0 of the same workload with the Newton cap set to {cap} (predictor and CCD "
-                       f"bisection, {used} Newton iteration(s) with PCG and line search, velocity update and "
-                       f"filter) on one host thread, {dt:.1f} s; scaled to a whole step by the "
-                       f"{newton_per_step} Newton iterations per step of the GPU FP64 run")
+                sample=f"step 0 of the same workload with the Newton cap at 1 ({used} Newton iteration, "
+                       f"{st['pcg_iters']} PCG iterations, {st['line_search_trials']} line-search trial(s), with "
+                       f"predictor, CCD bisection, velocity update and filter) timed whole on one host thread: "
+                       f"{dt:.1f} s; scaled by the {newton_per_step} Newton iterations per step of the GPU FP64 "
+                       f"run")
+
+
+def strip_whole_steps(h, budget_s):
+    """Whole steps of the strip-pull workload (bottom row driven 5 cm down
+    over 60 steps, Newton tol 1e-6, Jacobi): the CPU oracle on one host thread
+    for up to `budget_s`, and the FP64 and mixed HipSolver over the same
+    steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi as O
+    from closim_amd import _abi, scenes as S
+    from closim_amd.solver import HipSolver
+    kw = dict(precond=_abi.CS_PRECOND_JACOBI, grad_tol=1e-6)
+    sc = S.strip_pull()
+    o = O.OracleSolver(sc, _abi.options(**kw))
+    n = 0
+    t0 = time.perf_counter()
+    while n < 60:
+        for i, p in S.strip_pull_targets(sc, n):
+            o.pinVertex(i, p)
+        o.step(h)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    t_cpu = time.perf_counter() - t0
+    ref = o.getPositions().copy()
+    out = dict(workload="strip_pull_20x100", steps=n, cpu_oracle_steps_per_s=n / t_cpu, cpu_cores=1)
+    for prec in ("fp64", "mixed"):
+        sc = S.strip_pull()
+        g = HipSolver(prec, **kw)
+        g.initialize(sc)
+        t = 0.0
+        for k in range(n):
+            for i, p in S.strip_pull_targets(sc, k):
+                g.pinVertex(i, p)
+            t0 = time.perf_counter()
+            raw_step(g, h)
+            t += time.perf_counter() - t0
+        g._copy_out()
+        out[f"gpu_{prec}_steps_per_s"] = n / t
+        out[f"pos_rel_err_{prec}_vs_oracle"] = float(np.max(np.abs(g.getPositions() - ref)) / np.max(np.abs(ref)))
+        g.close()
+    out["note"] = "whole steps, pins driven between steps (not timed); the GPU is latency-bound on 2,000 vertices"
+    return out
 
 
 def copy_calibration(nbytes=256 << 20, reps=8):
     """Device-to-device copies of a known size (hipMemcpyAsync), so that the
     FETCH_SIZE / WRITE_SIZE counters of the same run can be scaled to bytes."""
-    lib = C.CDLL("libamdhip64.so")
+    lib = hip_runtime()
     a, b = C.c_void_p(), C.c_void_p()
     assert lib.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
     assert lib.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
@@ -334,19 +443,25 @@ def copy_calibration(nbytes=256 << 20, reps=8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "fold", "strip"])
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="fold", choices=["fold", "sphere", "strip"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--hvp-reps", type=int, default=200)
+    ap.add_argument("--host-newton", action="store_true",
+                    help="decide the descent safeguard, trust region and each Armijo trial on the host "
+                         "(default: CS_FLAG_DEVICE_NEWTON, the same results with one host read per trial batch)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the mixed PCG kernels one by one instead of replaying captured graphs")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per HVP launch from the FETCH_SIZE/WRITE_SIZE counter runs")
     ap.add_argument("--accuracy-steps", type=int, default=None,
                     help="steps of the converged small-scene comparison (default 20, 0 = skip)")
-    ap.add_argument("--fold-steps", type=int, default=3,
-                    help="timed steps of the 512x512 self-contact fold section on the sphere workload (0 = skip)")
+    ap.add_argument("--drape-steps", type=int, default=5,
+                    help="timed steps of the 256x256 sphere drape section (0 = skip)")
+    ap.add_argument("--lbvh-reps", type=int, default=5, help="timed broadphase calls (0 = skip lbvh and ccd)")
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed steps of the 64-instance batched drape section (0 = skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
@@ -362,24 +477,18 @@ def main():
     from closim_amd import _abi
     from closim_amd.solver import device_info
     h = 1.0 / 60.0
-    # sphere drape: block-Jacobi; self-contact fold: additive Schwarz on
-    # 64-vertex subdomains; strip: Jacobi (the reference's default)
-    popts = {"sphere": dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI),
-             "fold": dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64)),
-             "strip": dict(precond=_abi.CS_PRECOND_JACOBI)}[args.workload]
-    pname = {_abi.CS_PRECOND_JACOBI: "jacobi", _abi.CS_PRECOND_BLOCK_JACOBI: "block-jacobi",
-             _abi.CS_PRECOND_SCHWARZ: "additive-schwarz {}/{}/{}".format(*popts.get("schwarz", (0, 0, 0)))}[
-                 popts["precond"]]
+    popts = precond_options(args.workload)
+    flags = (_abi.CS_FLAG_EAGER if args.eager else 0) | (0 if args.host_newton else _abi.CS_FLAG_DEVICE_NEWTON)
     print(f"[bench] device: {device_info(0)}", file=sys.stderr, flush=True)
     sc_m = make_scene(args.workload)
     mesh = sc_m.mesh
     print(f"[bench] workload {args.workload}: {mesh.count} vertices, {mesh.tri_count} triangles", file=sys.stderr,
           flush=True)
-    gm = solver_for(sc_m, "mixed", popts, timing=False)
+    gm = solver_for(sc_m, "mixed", popts, timing=False, flags=flags)
     gd = None
     if not args.no_fp64:
         sc_d = make_scene(args.workload)
-        gd = solver_for(sc_d, "fp64", popts, timing=False)
+        gd = solver_for(sc_d, "fp64", popts, timing=False, flags=flags)
     # warmup (both solvers advance from the same initial state)
     for k in range(args.warmup):
         raw_step(gm, h)
@@ -410,9 +519,8 @@ def main():
         # sensitivity of the same capped solve to the Hessian operator alone:
         # FP64 with the analytic HVP instead of the FD one
         sc_a = make_scene(args.workload)
-        from closim_amd import _abi as A_
         from closim_amd.solver import HipSolver
-        ga = HipSolver("fp64", **popts, hvp_mode=A_.CS_HVP_ANALYTIC)
+        ga = HipSolver("fp64", **popts, hvp_mode=_abi.CS_HVP_ANALYTIC)
         ga.initialize(sc_a)
         for _ in range(args.warmup + args.steps):
             raw_step(ga, h)
@@ -452,9 +560,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 inner solve / f64 Newton",
         "data": "synthetic",
-        "config": {"workload": sc_m.meta.get("name", args.workload),
+        "config": {"workload": sc_m.meta.get("name", WORKLOADS[args.workload]["name"]),
                    "vertices": mesh.count, "triangles": mesh.tri_count, "h": h,
-                   "preconditioner": pname,
+                   "preconditioner": WORKLOADS[args.workload]["precond"],
+                   "pcg_launch": "eager" if args.eager else "graph replay",
+                   "newton_control": "host" if args.host_newton else "device (batched Armijo trials)",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -473,20 +583,25 @@ def main():
         "tuned_launch": gm.tuned_launch(),
     }
     out.update(res)
-    if args.workload == "sphere" and args.fold_steps > 0 and gd:
-        gm.close()
+    if args.lbvh_reps > 0 and sc_m.contact is not None:
+        gm._copy_out()
+        out["lbvh"], out["ccd"] = lbvh_ccd_section(sc_m, h, reps=args.lbvh_reps)
+    gm.close()
+    if gd:
         gd.close()
-        out["fold"] = fold_section(args.fold_steps, h)
-    if args.workload == "sphere" and args.batch_steps > 0 and gd:
+    if args.workload == "fold" and args.drape_steps > 0 and gd:
+        out["drape"] = pair_section("sphere", args.drape_steps, 2, h)
+    if args.batch_steps > 0 and gd:
         out["batch"] = batch_section(args.batch_steps, h, threads=args.batch_threads)
-    if args.workload == "sphere" and args.adjoint_steps > 0 and gd:
+    if args.adjoint_steps > 0 and gd:
         out["adjoint"] = adjoint_section(args.adjoint_steps, h)
     if args.copy_calibration:
         copy_calibration()
     if not args.no_cpu:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         nps = res.get("fp64", {}).get("newton_used_last_step", st_m["newton_used"])
-        out["cpu_baseline"] = cpu_baseline(args.workload, popts, args.cpu_budget, nps)
+        out["cpu_baseline"] = cpu_baseline(args.workload, popts, args.cpu_budget, nps, h)
+        out["cpu_baseline"]["whole_steps_strip_pull"] = strip_whole_steps(h, args.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

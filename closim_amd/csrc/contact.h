@@ -1,6 +1,8 @@
 // contact.h — device-resident ContactSystem (contact-assembly.ts:46-595).
 #pragma once
 
+#include <vector>
+
 #include <stdexcept>
 #include <string>
 
@@ -9,6 +11,9 @@
 
 namespace cloth {
 
+struct UnsupportedError : std::runtime_error {
+  explicit UnsupportedError(const std::string& s) : std::runtime_error(s) {}
+};
 struct OverflowError : std::runtime_error {
   explicit OverflowError(const std::string& s) : std::runtime_error(s) {}
 };
@@ -30,6 +35,23 @@ class ContactSystem {
   void add_diag_estimate(double* diag, const double* x);   // addDiagEstimate :436-471
   double merit_energy(const double* x);                    // meritEnergy :419-428
   void add_rejection();                                    // :83
+  // checkTrial for a batched line search: enqueue trials into slots, fetch
+  // the slots' counters with one synchronisation, then judge the slots in
+  // order (each judged slot updates the diagnostics as check_trial does)
+  void trial_enqueue(const double* xt, int slot);
+  void trial_fetch(int count);
+  bool trial_result(int slot);
+  // the triangle pairs buildCandidates :87-121 expands at x (after begin_step)
+  void probe_pairs(const double* x, double pad, std::vector<uint32_t>& self2, std::vector<uint32_t>& stat2);
+  double pad(bool d_min) const;  // dMinM or dHatM
+  // meritEnergy :419-428 into a device double, without synchronising
+  void merit_enqueue(const double* x, double* out);
+  // setStaticMesh :71-73 between steps (same triangles, new positions)
+  void set_static_mesh(const float* pos, uint32_t nverts, const uint32_t* idx, uint32_t ntris);
+  // out += dL/dx_step of the lagged friction for the step adjoint lambda at
+  // the step's root x (x_step = the begin_step positions); no-op without
+  // friction or active contacts.
+  void friction_xstep_adjoint(const double* x, const double* lam, double* out);
   void commit(const double* x);                            // commit :475-485
   void kill_approach_velocity(const double* x, double* v); // newton.ts:238-303
   cs_contact_diag diagnostics() const;
@@ -48,6 +70,10 @@ class ContactSystem {
   ContactImpl* im_;
 };
 
+// Batched CCD over device arrays (ids4 16-byte aligned), synchronous on the
+// null stream; *kernel_ms (optional) is the kernel's HIP-event time.
+void ccd_queries(const double* x0, const double* x1, uint32_t n, const int32_t* kinds, const int32_t* ids4,
+                 uint64_t count, double thickness, double* toi, double* kernel_ms);
 // Batched device primitives for parity probes.
 void probe_ccd(hipStream_t st, const double* x0, const double* x1, uint32_t n, const int32_t* kinds,
                const int32_t* ids4, uint64_t count, double thickness, double* toi);

@@ -1158,6 +1158,96 @@ __global__ void __launch_bounds__(kBlock) k_grad_vsum(uint32_t nv, const uint32_
   ceg[(size_t)v * 3 + q] = ordered_sum<6, 3>(gterm + q, v_off[v], v_off[v + 1], 0.0);
 }
 
+// Adjoint of the lagged friction with respect to the step's start positions.
+// The friction term -f(u) that energyGrad adds (contact-assembly.ts:272-279,
+// distributed by addFrictionForce :318-341 with weights wW) depends on the
+// start positions x_step only through the slip u = sum_k wS_k (x - x_step)_k
+// (relativeSlip :344-367: the VT slip leaves out the triangle's third
+// vertex, the EE slip adds all four), the lagged normal and normal force
+// held.  With f = -c t / |t~| (coulombForce, friction.ts:14-23), t = P u,
+// P = I - n n^T, |t~| = sqrt(|t|^2 + eps^2), c = mu max(0, lambdaN):
+//   -df/du = K = c (P / |t~| - t t^T / |t~|^3)   (symmetric),
+// so for the adjoint lambda of the step's root
+//   dL/dx_step[v] += wS_v K (sum_k wW_k lambda_k)      (vertex v of the stencil).
+// Pass 1 forms y = K (W^T lambda) per contact inside dHat; pass 2 the
+// per-entry terms wS y; pass 3 adds them per vertex in list order.
+__global__ void __launch_bounds__(kBlock) k_fric_adj_contact(DGeom G, const double* __restrict__ x,
+                                                             const double* __restrict__ xstep,
+                                                             const Frozen* __restrict__ act,
+                                                             const uint64_t* __restrict__ anke, uint32_t C,
+                                                             const uint64_t* __restrict__ lag_keys,
+                                                             const uint32_t* __restrict__ lag_idx, uint32_t L,
+                                                             const double* __restrict__ lag_vals,
+                                                             const double* __restrict__ lam, double* __restrict__ y) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const Frozen c = act[i];
+  double out[3] = {0, 0, 0};
+  const double d = jsm::jmax(frozen_dist(G, x, c), 1e-12);
+  if (d < G.d_hat) {
+    double nx = c.nx, ny = c.ny, nz = c.nz, lamN;
+    const int64_t li = lag_find(lag_keys, L, anke[i]);
+    if (li >= 0) {
+      const double* lv = lag_vals + (size_t)lag_idx[li] * 4;
+      nx = lv[0]; ny = lv[1]; nz = lv[2]; lamN = lv[3];
+    } else {
+      lamN = barrier_normal_force(d, G.d_hat, G.kappa, 1);
+    }
+    const double cc = G.mu * jsm::jmax(0, lamN);
+    if (cc != 0) {
+      double u[3];
+      relative_slip(x, xstep, c, u);
+      const double vn = u[0] * nx + u[1] * ny + u[2] * nz;
+      const double t[3] = {u[0] - vn * nx, u[1] - vn * ny, u[2] - vn * nz};
+      const double mag = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + G.feps * G.feps);
+      // w = W^T lambda with addFrictionForce's weights (static vertices skipped)
+      double w[3] = {0, 0, 0};
+      auto acc = [&](int v, double wt) {
+        if (v < 0) return;
+        for (int q = 0; q < 3; q++) w[q] += wt * lam[(size_t)v * 3 + q];
+      };
+      if (c.kind == 2) acc(c.p, 1.0);
+      else if (c.kind == 0) {
+        acc(c.p, 1.0); acc(c.a, -c.w0); acc(c.b, -c.w1); acc(c.c, -c.w2);
+      } else {
+        acc(c.a, c.w0); acc(c.b, c.w1); acc(c.c, -c.w2); acc(c.d, -c.w3);
+      }
+      const double wn = w[0] * nx + w[1] * ny + w[2] * nz;
+      const double pw[3] = {w[0] - wn * nx, w[1] - wn * ny, w[2] - wn * nz};
+      const double tw = t[0] * w[0] + t[1] * w[1] + t[2] * w[2];
+      for (int q = 0; q < 3; q++) out[q] = cc * (pw[q] / mag - t[q] * tw / (mag * mag * mag));
+    }
+  }
+  for (int q = 0; q < 3; q++) y[(size_t)i * 3 + q] = out[q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_fric_adj_terms(uint32_t E, const Frozen* __restrict__ act,
+                                                           const uint32_t* __restrict__ v_ent,
+                                                           const double* __restrict__ y, double* __restrict__ term) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  const uint32_t ent = v_ent[k];
+  const uint32_t ci = ent >> 3;
+  const int slot = ent & 7;
+  const Frozen c = act[ci];
+  // relativeSlip's weight of this stencil slot
+  double ws;
+  if (c.kind == 2) ws = 1.0;
+  else if (c.kind == 0) ws = slot == 0 ? 1.0 : slot == 1 ? -c.w0 : slot == 2 ? -c.w1 : 0.0;
+  else ws = slot == 0 ? c.w0 : slot == 1 ? c.w1 : slot == 2 ? c.w2 : c.w3;
+  for (int q = 0; q < 3; q++) term[(size_t)k * 3 + q] = ws * y[(size_t)ci * 3 + q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_fric_adj_vsum(uint32_t nv, const uint32_t* __restrict__ cverts,
+                                                          const uint32_t* __restrict__ v_off,
+                                                          const double* __restrict__ term, double* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nv * 3) return;
+  const uint32_t v = cverts[t / 3];
+  const int q = t % 3;
+  out[(size_t)v * 3 + q] += ordered_sum<3, 0>(term + q, v_off[v], v_off[v + 1], 0.0);
+}
+
 // addDiagEstimate (contact-assembly.ts:436-471): accumulate into diag, the
 // same two passes (a floor contact touches only the y component; +0 added to
 // a positive diagonal leaves it unchanged).
@@ -1830,6 +1920,7 @@ struct ContactImpl {
       if (d.static_idx[k] >= ns) throw std::invalid_argument("static triangle index out of range");
     idx.upload(d.indices, (size_t)m * 3, st);
     if (ms) {
+      h_sidx.assign(d.static_idx, d.static_idx + (size_t)ms * 3);
       sidx.upload(d.static_idx, (size_t)ms * 3, st);
       std::vector<double> sp(d.static_pos, d.static_pos + (size_t)ns * 3);
       spos.upload(sp.data(), sp.size(), st);
@@ -1940,7 +2031,13 @@ struct ContactImpl {
   // Fixed hierarchy over the static triangles: unpadded exact boxes
   // (Math.min / Math.max of the three corners, staticTriAabb :597-605) in
   // Morton order of their centroids, inner boxes as NaN-ignoring unions.
-  void build_static_tree(const cs_scene_desc& d) {
+  std::vector<uint32_t> h_sidx;  // the static triangles (their topology is fixed for the solver's life)
+  void build_static_tree(const cs_scene_desc& d) { build_static_tree(d.static_pos, d.static_idx); }
+  void build_static_tree(const float* static_pos, const uint32_t* static_idx) {
+    struct {
+      const float* static_pos;
+      const uint32_t* static_idx;
+    } d{static_pos, static_idx};
     size_t nn = 0;
     SL = make_levels(ms, &nn);
     std::vector<Leaf> sl(ms);
@@ -2229,6 +2326,35 @@ struct ContactImpl {
     }
   }
 
+  // setStaticMesh (contact-assembly.ts:71-73) between steps: new positions of
+  // the same static triangles (a moving rigid collider, avatar-collider.ts:
+  // 205-230 samples its mesh per tick and attaches it this way).
+  void set_static_mesh(const float* pos, uint32_t nverts, const uint32_t* sidx_h, uint32_t ntris) {
+    if (ntris != ms || (ms && nverts != ns)) throw UnsupportedError("the static mesh's vertex and triangle counts are fixed at creation");
+    for (size_t k = 0; k < (size_t)ms * 3; k++)
+      if (sidx_h[k] != h_sidx[k]) throw UnsupportedError("the static mesh's triangles are fixed at creation");
+    if (!ms) return;
+    std::vector<double> sp(pos, pos + (size_t)ns * 3);
+    CS_HIP(hipStreamSynchronize(st));
+    spos.upload(sp.data(), sp.size(), st);
+    build_static_tree(pos, sidx_h);
+    CS_HIP(hipStreamSynchronize(st));
+  }
+
+  DevBuf<double> fadj_y, fadj_term;
+  void friction_xstep_adjoint(const double* x, const double* lam, double* out) {
+    if (!C || !E || !(prm.friction_mu > 0)) return;
+    fadj_y.alloc((size_t)C * 3);
+    fadj_term.alloc((size_t)E * 3);
+    hipLaunchKernelGGL(k_fric_adj_contact, dim3(grid_for(C)), dim3(kBlock), 0, st, geom(), x, xstep, act.ptr,
+                       anke.ptr, C, lag_keys.ptr, lag_idx.ptr, lag_L, lag_vals.ptr, lam, fadj_y.ptr);
+    hipLaunchKernelGGL(k_fric_adj_terms, dim3(grid_for(E)), dim3(kBlock), 0, st, E, act.ptr, v_ent.ptr, fadj_y.ptr,
+                       fadj_term.ptr);
+    hipLaunchKernelGGL(k_fric_adj_vsum, dim3(grid_for((size_t)nv * 3)), dim3(kBlock), 0, st, nv, cverts.ptr,
+                       v_off.ptr, fadj_term.ptr, out);
+    CS_LAUNCH_CHECK();
+  }
+
   void add_diag_estimate(double* dg) {
     if (!C) return;
     if (E) {
@@ -2319,12 +2445,39 @@ struct ContactImpl {
     CS_LAUNCH_CHECK();
   }
 
-  bool check_trial(const double* xt) {
+  // ---- trial validation queued for a batched line search ----------------
+  // trial_enqueue() launches one trial's validation and keeps its counters in
+  // slot `slot` without reading them; trial_fetch() reads every slot with one
+  // synchronisation; trial_result() then judges a slot exactly as
+  // check_trial() judges its own counters (diagnostics included), so a caller
+  // that judges the slots in order up to its first accepted trial reproduces
+  // the sequential line search.
+  DevBuf<CScalars> cs_slots;
+  std::vector<CScalars> hslots;
+  std::vector<PairSrc> slot_src;
+  void trial_enqueue(const double* xt, int slot) {
+    if ((size_t)slot >= cs_slots.count) {
+      cs_slots.alloc((size_t)slot + 8);
+      hslots.resize((size_t)slot + 8);
+      slot_src.resize((size_t)slot + 8);
+    }
+    slot_src[slot] = launch_trial_kernels(xt);
+    CS_HIP(hipMemcpyAsync(cs_slots.ptr + slot, cs.ptr, sizeof(CScalars), hipMemcpyDeviceToDevice, st));
+  }
+  void trial_fetch(int count) {
+    if (count <= 0) return;
+    CS_HIP(hipMemcpyAsync(hslots.data(), cs_slots.ptr, (size_t)count * sizeof(CScalars), hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    tune_trial.collect();
+    tune_ccd.collect();
+  }
+  bool trial_result(int slot) { return judge_trial(hslots[slot], slot_src[slot]); }
+
+  PairSrc launch_trial_kernels(const double* xt) {
     const PairSrc src = candidate_source(xt, prm.d_min);
     const DGeom G = geom();
     const uint32_t total = src.np + src.nsp;
     if (total) {
-      // every prim of every pair at most once in the tail queue
       if ((size_t)total * 15 > tail_cap) {
         tail_cap = (size_t)total * 15;
         tailq.alloc(tail_cap);
@@ -2345,32 +2498,81 @@ struct ContactImpl {
     if (has_floor)
       hipLaunchKernelGGL(k_floor_trial, dim3(red_grid(n)), dim3(kBlock), 0, st, G, xstep, xt, dpart.ptr + 2048, cs.ptr);
     CS_LAUNCH_CHECK();
-    sync_cs();
-    if (total) {
-      tune_trial.collect();
-      tune_ccd.collect();
+    return src;
+  }
+
+  static double mink_of(const CScalars& h, int k) {
+    const uint64_t key = h.mink[k];
+    if (key == ~0ull) return INFINITY;
+    const uint64_t u = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
+  }
+
+  // checkTrial's verdict (contact-assembly.ts:489-553) from a trial's counters
+  bool judge_trial(const CScalars& h, const PairSrc& src) {
+    if (src.filter) {
+      np = h.pair_count;
+      nsp = h.spair_count;
     }
-    candidate_counts(src);
-    double minDist = mink_value(5);
-    double minTOI = mink_value(6);
-    if (hcs->nonfinite & 1u) {
+    diag.candidate_pairs = (uint64_t)np * 15 + (uint64_t)nsp * 12;
+    double minDist = mink_of(h, 5);
+    double minTOI = mink_of(h, 6);
+    if (h.nonfinite & 1u) {
       diag.ccd_failures++;
       return false;
     }
-    if (has_floor && mink_value(7) < minDist) minDist = mink_value(7);
+    if (has_floor && mink_of(h, 7) < minDist) minDist = mink_of(h, 7);
     if (minDist <= prm.d_min) return false;
-    if (hcs->nan_key != SENT && hcs->nan_key < hcs->cross_key) {
+    if (h.nan_key != SENT && h.nan_key < h.cross_key) {
       diag.ccd_failures++;
       return false;
     }
-    if (hcs->cross_key != SENT) return false;
+    if (h.cross_key != SENT) return false;
     if (has_floor) {
-      if (mink_value(8) < minTOI) minTOI = mink_value(8);
-      if (hcs->floor_fail) return false;
+      if (mink_of(h, 8) < minTOI) minTOI = mink_of(h, 8);
+      if (h.floor_fail) return false;
     }
     if (minDist < diag.min_distance) diag.min_distance = minDist;
     if (minTOI < diag.min_toi) diag.min_toi = minTOI;
     return true;
+  }
+
+  // The cloth/cloth and cloth/static triangle pairs of buildCandidates at x
+  // (swept from the step start, padded by `pad`) from a fresh traversal, in
+  // the traversal's order.
+  void probe_pairs(const double* x, double pad, std::vector<uint32_t>& self2, std::vector<uint32_t>& stat2) {
+    sup_valid = false;
+    const PairSrc src = candidate_source(x, pad);
+    self2.resize((size_t)src.np * 2);
+    stat2.resize((size_t)src.nsp * 2);
+    if (src.np) CS_HIP(hipMemcpyAsync(self2.data(), src.p, (size_t)src.np * 8, hipMemcpyDeviceToHost, st));
+    if (src.nsp) CS_HIP(hipMemcpyAsync(stat2.data(), src.s, (size_t)src.nsp * 8, hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+  }
+
+  // meritEnergy (contact-assembly.ts:419-428) at x into the device double `out`
+  void merit_enqueue(const double* x, double* out) {
+    if (!C) {
+      CS_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
+      return;
+    }
+    hipLaunchKernelGGL(k_merit, dim3(red_grid(C)), dim3(kBlock), 0, st, geom(), x, act.ptr, C, ddpart.ptr, cs.ptr);
+    CS_LAUNCH_CHECK();
+    FinishJobs f{};
+    f.j[0] = FinishJob{ddpart.ptr, red_grid(C), FIN_DD, 0, out};
+    launch_finish(st, f, 1);
+  }
+
+  bool check_trial(const double* xt) {
+    const PairSrc src = launch_trial_kernels(xt);
+    sync_cs();
+    if (src.np + src.nsp) {
+      tune_trial.collect();
+      tune_ccd.collect();
+    }
+    return judge_trial(*hcs, src);
   }
 };
 
@@ -2393,6 +2595,21 @@ void ContactSystem::update_active_set(const double* x) { im_->update_active_set(
 void ContactSystem::energy_grad(const double* x, double* ceg) { im_->energy_grad(x, ceg); }
 void ContactSystem::add_diag_estimate(double* diag, const double*) { im_->add_diag_estimate(diag); }
 double ContactSystem::merit_energy(const double* x) { return im_->merit_energy(x); }
+void ContactSystem::probe_pairs(const double* x, double pad, std::vector<uint32_t>& self2,
+                                std::vector<uint32_t>& stat2) {
+  im_->probe_pairs(x, pad, self2, stat2);
+}
+double ContactSystem::pad(bool d_min) const { return d_min ? im_->prm.d_min : im_->prm.d_hat; }
+void ContactSystem::trial_enqueue(const double* xt, int slot) { im_->trial_enqueue(xt, slot); }
+void ContactSystem::trial_fetch(int count) { im_->trial_fetch(count); }
+bool ContactSystem::trial_result(int slot) { return im_->trial_result(slot); }
+void ContactSystem::merit_enqueue(const double* x, double* out) { im_->merit_enqueue(x, out); }
+void ContactSystem::set_static_mesh(const float* pos, uint32_t nverts, const uint32_t* idx, uint32_t ntris) {
+  im_->set_static_mesh(pos, nverts, idx, ntris);
+}
+void ContactSystem::friction_xstep_adjoint(const double* x, const double* lam, double* out) {
+  im_->friction_xstep_adjoint(x, lam, out);
+}
 void ContactSystem::add_rejection() { im_->diag.rejected_line_search_steps++; }
 void ContactSystem::commit(const double* x) { im_->commit(x); }
 void ContactSystem::kill_approach_velocity(const double* x, double* v) { im_->kill_approach_velocity(x, v); }
@@ -2451,6 +2668,63 @@ __global__ void k_probe_closest(const double* x0, const int32_t* kinds, const in
   out3[i * 3] = c.s;
   out3[i * 3 + 1] = c.t;
   out3[i * 3 + 2] = c.dist;
+}
+
+// Batched CCD over device arrays: the ids are checked in the kernel (an id
+// outside [0, n) gives NaN and sets *bad).
+__global__ void __launch_bounds__(kBlock) k_ccd_queries(const double* __restrict__ x0, const double* __restrict__ x1,
+                                                        uint32_t n, const int32_t* __restrict__ kinds,
+                                                        const int32_t* __restrict__ ids4, uint64_t count,
+                                                        double thickness, double* toi, uint32_t* bad) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int4 id = reinterpret_cast<const int4*>(ids4)[i];
+  const int32_t v4[4] = {id.x, id.y, id.z, id.w};
+  for (int s = 0; s < 4; s++)
+    if (v4[s] < 0 || (uint32_t)v4[s] >= n) {
+      toi[i] = __builtin_nan("");
+      atomicOr(bad, 1u);
+      return;
+    }
+  Seg4 S;
+  for (int s = 0; s < 4; s++)
+    for (int k = 0; k < 3; k++) {
+      S.x0[s][k] = x0[(size_t)v4[s] * 3 + k];
+      S.x1[s][k] = x1[(size_t)v4[s] * 3 + k];
+    }
+  toi[i] = kinds[i] == 0 ? vt_ccd(S, thickness) : ee_ccd(S, thickness);
+}
+
+void ccd_queries(const double* x0, const double* x1, uint32_t n, const int32_t* kinds, const int32_t* ids4,
+                 uint64_t count, double thickness, double* toi, double* kernel_ms) {
+  hipStream_t st = nullptr;
+  DevBuf<uint32_t> bad;
+  bad.alloc(1);
+  CS_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(uint32_t), st));
+  hipEvent_t e[2] = {nullptr, nullptr};
+  if (kernel_ms)
+    for (auto& x : e) CS_HIP(hipEventCreate(&x));
+  struct G {
+    hipEvent_t* e;
+    ~G() {
+      for (int k = 0; k < 2; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    }
+  } guard{e};
+  if (kernel_ms) CS_HIP(hipEventRecord(e[0], st));
+  if (count)
+    hipLaunchKernelGGL(k_ccd_queries, dim3(grid_for(count)), dim3(kBlock), 0, st, x0, x1, n, kinds, ids4, count,
+                       thickness, toi, bad.ptr);
+  CS_LAUNCH_CHECK();
+  if (kernel_ms) CS_HIP(hipEventRecord(e[1], st));
+  uint32_t hb = 0;
+  CS_HIP(hipMemcpy(&hb, bad.ptr, sizeof(hb), hipMemcpyDeviceToHost));
+  if (hb) throw std::invalid_argument("primitive vertex id out of range");
+  if (kernel_ms) {
+    float ms = 0;
+    CS_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+    *kernel_ms = ms;
+  }
 }
 
 void probe_ccd(hipStream_t st, const double* x0, const double* x1, uint32_t n, const int32_t* kinds,

@@ -79,6 +79,18 @@ enum HvpMode { HVP_FD = 0, HVP_ANALYTIC = 1 };
 void launch_energy(hipStream_t st, const DMesh& me, const DMat& mat, const double* x, const double* yhat,
                    double h, jsm::dd* partials, double* out2);
 
+// Analytic hinge bending (bend.h; not in the reference, which differences the
+// energy): out (+)= the bending gradient, or its Hessian-vector product along
+// p, at x for hinge stiffness k; `term` holds 12 doubles per hinge.  Pinned
+// rows are zero (or left unchanged when accumulating).
+void launch_bend_grad(hipStream_t st, const DMesh& me, double k, const double* x, double* term, double* out,
+                      bool accumulate);
+void launch_bend_hvp(hipStream_t st, const DMesh& me, double k, const double* x, const double* p, double* term,
+                     double* out, bool accumulate);
+// Independent hinges (12 positions each) for component tests, host arrays.
+void probe_hinges(const double* X12, const double* P12, const double* rest, const double* len, const double* asum,
+                  double k, uint64_t count, double* grad12, double* hvp12);
+
 // maxStrain over triangles (JS Math.max semantics).
 void launch_max_strain(hipStream_t st, const DMesh& me, const double* x, double* partials, double* out);
 

@@ -25,6 +25,8 @@ namespace cloth {
 
 thread_local std::string g_last_error;
 
+constexpr int kArmijoBatch = 8;  // largest batch of line-search trials evaluated before one host read
+
 struct NewtonScalars {
   PcgScalars pcg;
   double grad_sumsq;
@@ -34,7 +36,42 @@ struct NewtonScalars {
   double maxdx;
   double max_strain;
   double adj[4];  // adjoint PCG: r.z, p.Ap, r.r, a.a
+  // device Newton control (CS_FLAG_DEVICE_NEWTON)
+  double tscale;                   // trust-region scale applied to dx
+  int32_t fallback;                // 1 when the descent safeguard took -g / diag
+  double etrial[kArmijoBatch][2];  // inertia and internal energy of each trial of a batch
+  double mtrial[kArmijoBatch];     // merit (barrier) energy of each trial
 };
+
+// Device Newton control: the descent safeguard (newton.ts:154-160) and the
+// trust region (:167-179) decided on the device from the reductions the
+// preceding kernels wrote, so dx and g.dx are final without a host read.
+__global__ void k_ctl_descent(NewtonScalars* s) {
+  const double g = s->gtdx;
+  s->fallback = (!isfinite(g) || g >= 0) ? 1 : 0;
+}
+
+__global__ void k_fallback_pred(const double* __restrict__ g, const double* __restrict__ diag, double* dx, size_t n,
+                                const NewtonScalars* s) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n && s->fallback) dx[i] = -g[i] / diag[i];
+}
+
+__global__ void k_ctl_trust(NewtonScalars* s, double trust) {
+  const double m = s->maxdx;
+  if (m > trust) {
+    const double sc = trust / m;
+    s->tscale = sc;
+    s->gtdx *= sc;
+  } else {
+    s->tscale = 1.0;
+  }
+}
+
+__global__ void k_scale_pred(double* v, size_t n, const NewtonScalars* s, double trust) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n && s->maxdx > trust) v[i] *= s->tscale;
+}
 
 struct NonFinite {};
 
@@ -405,7 +442,8 @@ struct Solver {
   // implicit function theorem gives, with H = dg/dx at x1 and
   // a = dL/dx1 + (damp / h) dL/dv1 on the free DOFs,
   //   H lambda = a,   dL/dtheta_j = -lambda . dg/dtheta_j,
-  //   dL/dx0 = lambda M / h^2 - (damp / h) dL/dv1,   dL/dv0 = lambda M / h.
+  //   dL/dx0 = lambda M / h^2 - (damp / h) dL/dv1 - lambda . dg_fric/dx0,   dL/dv0 = lambda M / h,
+  // the last term from the lagged friction, whose slip is measured from x0.
   // H p is the analytic inertia + membrane product plus the central
   // difference of the bending and contact gradients along p (full_hvp), so H
   // includes the FD bending gradient the step uses, which the reference's
@@ -436,38 +474,41 @@ struct Solver {
       b->alloc(n3);
   }
 
-  // H p at `base`: the inertia and membrane terms with the analytic membrane
-  // Hessian (exact), plus the central difference (1e-6 m along p / |p|_inf)
-  // of the remaining gradient terms, the FD bending gradient and the contact
-  // gradient; H is the derivative of the whole Newton gradient.
+  // H p at `base`: the inertia, membrane and bending terms analytically (the
+  // membrane Hessian in closed form, the hinge Hessian-vector product in dual
+  // numbers, bend.h), plus the central difference (1e-6 m along p / |p|_inf)
+  // of the contact gradient; H is the derivative of the whole Newton
+  // gradient (the step's FD bending gradient differs from the analytic one by
+  // its truncation and round-off only).
+  DevBuf<double> bend_term;
   void full_hvp(const double* base, const double* p, double* out) {
     const size_t n3 = (size_t)n * 3;
     launch_hvp64_analytic(st, dmesh(), dmat(), 1.0 / (hadj * hadj), base, p, out);
     const double kb = 0.5 * (mat.bend_warp + mat.bend_weft);
-    if (kb == 0 && !contact) return;
+    if (kb > 0 && nh) {
+      bend_term.alloc((size_t)nh * 12);
+      launch_bend_hvp(st, dmesh(), kb, base, p, bend_term.ptr, out, true);
+    }
+    if (!contact) return;
     launch_max_abs(st, p, n3, mpartials.ptr, &ds()->maxdx);
     const double pmax = read(&ds()->maxdx);
     if (!(pmax > 0)) return;
     const double eps = 1e-6;  // metres along p / |p|_inf
     hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, eps / pmax, adj_xs.ptr, n3);
-    rest_gradient(adj_xs.ptr, adj_gp.ptr);
+    contact_gradient(adj_xs.ptr, adj_gp.ptr);
     hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, -eps / pmax, adj_xs.ptr, n3);
-    rest_gradient(adj_xs.ptr, adj_gm.ptr);
+    contact_gradient(adj_xs.ptr, adj_gm.ptr);
     hipLaunchKernelGGL(k_adj_fd_add, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr,
                        pmax / (2 * eps), out, n3);
     CS_LAUNCH_CHECK();
   }
 
-  // the gradient terms full_hvp differentiates numerically: bending (the
-  // material with its membrane moduli zeroed) and contact, no inertia
-  void rest_gradient(const double* xx, double* out) {
-    const double* cg = nullptr;
-    if (contact) {
-      contact->energy_grad(xx, ceg.ptr);
-      cg = ceg.ptr;
-    }
-    const DMat bend{mat.thickness, 0, 0, 0, 0, 0.5 * (mat.bend_warp + mat.bend_weft)};
-    launch_gradient(st, dmesh(), bend, xx, yhat.ptr, 0.0, cg, out, partials.ptr, &ds()->grad_sumsq);
+  // the gradient term full_hvp differentiates numerically: contact alone
+  // (the internal terms with every modulus zero), pins zeroed
+  void contact_gradient(const double* xx, double* out) {
+    contact->energy_grad(xx, ceg.ptr);
+    const DMat none{mat.thickness, 0, 0, 0, 0, 0};
+    launch_gradient(st, dmesh(), none, xx, yhat.ptr, 0.0, ceg.ptr, out, partials.ptr, &ds()->grad_sumsq);
   }
 
   double adj_dot(const double* a, const double* b, double* slot) {
@@ -559,6 +600,8 @@ struct Solver {
     hipLaunchKernelGGL(k_adj_out, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_lam.ptr, adj_gv.ptr, mass.ptr, h,
                        damp, adj_gp.ptr, adj_gm.ptr, n3);
     CS_LAUNCH_CHECK();
+    // the lagged friction's slip is measured from x_prev: its share of dL/dx_prev
+    if (contact) contact->friction_xstep_adjoint(adj_x1.ptr, adj_lam.ptr, adj_gp.ptr);
     CS_HIP(hipMemcpyAsync(hdx0, adj_gp.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     CS_HIP(hipMemcpyAsync(hdv0, adj_gm.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
@@ -608,6 +651,7 @@ struct Solver {
     const int newtonIters = opt.newton_iters, pcgMaxIters = opt.pcg_max_iters, lsIters = opt.line_search_iters;
     const double gradTol = opt.grad_tol;
     const bool polish = (opt.flags & CS_FLAG_POLISH_ROOT) != 0;
+    const bool devctl = (opt.flags & CS_FLAG_DEVICE_NEWTON) != 0;
     CS_HIP(hipMemcpyAsync(x0.ptr, pos.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_predictor, dim3(grid_for(n)), dim3(kBlock), 0, st, x0.ptr, vel.ptr, h, gravity[0],
                        gravity[1], gravity[2], yhat.ptr, x.ptr, n);
@@ -673,22 +717,37 @@ struct Solver {
         iters = pcg_solve(invH2, pcgMaxIters);
       }
       pcgItersTotal += iters;
-      // descent safeguard (newton.ts:154-160)
-      launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
-      double gtdx = read(&ds()->gtdx);
-      if (!std::isfinite(gtdx) || gtdx >= 0) {
-        launch_fallback_dir(st, g.ptr, diag.ptr, dx.ptr, n3);
+      const double TRUST = 0.002;
+      double gtdx = 0;
+      if (devctl) {
+        // descent safeguard and trust region decided on the device; g.dx is
+        // read with the first line-search batch
+        launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
+        hipLaunchKernelGGL(k_ctl_descent, dim3(1), dim3(1), 0, st, ds());
+        hipLaunchKernelGGL(k_fallback_pred, dim3(grid_for(n3)), dim3(kBlock), 0, st, g.ptr, diag.ptr, dx.ptr, n3,
+                           ds());
+        launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
+        launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->maxdx);
+        hipLaunchKernelGGL(k_ctl_trust, dim3(1), dim3(1), 0, st, ds(), TRUST);
+        hipLaunchKernelGGL(k_scale_pred, dim3(grid_for(n3)), dim3(kBlock), 0, st, dx.ptr, n3, ds(), TRUST);
+        CS_LAUNCH_CHECK();
+      } else {
+        // descent safeguard (newton.ts:154-160)
         launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
         gtdx = read(&ds()->gtdx);
-      }
-      // trust region (newton.ts:167-179)
-      launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->maxdx);
-      const double maxDx = read(&ds()->maxdx);
-      const double TRUST = 0.002;
-      if (maxDx > TRUST) {
-        const double s = TRUST / maxDx;
-        launch_scale(st, dx.ptr, s, n3);
-        gtdx *= s;
+        if (!std::isfinite(gtdx) || gtdx >= 0) {
+          launch_fallback_dir(st, g.ptr, diag.ptr, dx.ptr, n3);
+          launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
+          gtdx = read(&ds()->gtdx);
+        }
+        // trust region (newton.ts:167-179)
+        launch_max_abs(st, dx.ptr, n3, mpartials.ptr, &ds()->maxdx);
+        const double maxDx = read(&ds()->maxdx);
+        if (maxDx > TRUST) {
+          const double s = TRUST / maxDx;
+          launch_scale(st, dx.ptr, s, n3);
+          gtdx *= s;
+        }
       }
       // every trial x + alpha dx (and the accepted iterate) lies on x -> x + dx
       if (contact) {
@@ -699,7 +758,8 @@ struct Solver {
       // Armijo (newton.ts:184-219)
       double alpha = 1;
       bool accepted = false;
-      for (int li = 0; li < lsIters; li++) {
+      if (devctl) accepted = armijo_batched(h, lsIters, energy, alpha, gtdx, lsTrials);
+      for (int li = 0; li < lsIters && !devctl; li++) {
         lsTrials++;
         launch_trial(st, x.ptr, dx.ptr, alpha, xt.ptr, n3);
         enforce_pins(xt.ptr);
@@ -757,6 +817,71 @@ struct Solver {
 
   // xt = x0 + t (x - x0) (predictor bisection probe, newton.ts:62-63)
   void launch_lerp(double t, double* out);
+
+  // The Armijo loop of newton.ts:184-219 with its trials evaluated in
+  // batches of 1, 2, 4, 8: each batch launches every trial's positions, contact
+  // validation and energies, then reads them with one synchronisation; the
+  // trials are then judged in order with the reference's rules, up to the
+  // first accepted one, so the result, the trial count and the contact
+  // diagnostics are those of the sequential loop (trials past the accepted
+  // one were computed but are never judged).  g.dx comes from the device
+  // control with the first batch.  Returns whether a trial was accepted;
+  // `alpha` and `energy` as the sequential loop leaves them.
+  DevBuf<double> tbuf[kArmijoBatch];
+  bool armijo_batched(double h, int lsIters, double& energy, double& alpha, double& gtdx, int& lsTrials) {
+    const size_t n3 = (size_t)n * 3;
+    // the full step is usually accepted, and a trial computed past the
+    // accepted one costs a whole contact check: the first batch is one trial
+    int li = 0, batch = 1;
+    bool first = true;
+    while (li < lsIters) {
+      const int K = std::min(std::min(batch, lsIters - li), kArmijoBatch);
+      double a = alpha;
+      for (int j = 0; j < K; j++) {
+        tbuf[j].alloc(n3);
+        launch_trial(st, x.ptr, dx.ptr, a, tbuf[j].ptr, n3);
+        enforce_pins(tbuf[j].ptr);
+        if (contact) contact->trial_enqueue(tbuf[j].ptr, j);
+        launch_energy(st, dmesh(), dmat(), tbuf[j].ptr, yhat.ptr, h, partials.ptr, ds()->etrial[j]);
+        if (contact) contact->merit_enqueue(tbuf[j].ptr, &ds()->mtrial[j]);
+        a *= 0.5;
+      }
+      CS_HIP(hipMemcpyAsync(hscal, scal.ptr, sizeof(NewtonScalars), hipMemcpyDeviceToHost, st));
+      if (contact) contact->trial_fetch(K);  // synchronises the stream, the scalars included
+      else CS_HIP(hipStreamSynchronize(st));
+      if (first) {
+        gtdx = hscal->gtdx;
+        first = false;
+      }
+      for (int j = 0; j < K; j++) {
+        lsTrials++;
+        if (contact && !contact->trial_result(j)) {
+          contact->add_rejection();
+          alpha *= 0.5;
+          continue;
+        }
+        double eNew = hscal->etrial[j][0];
+        eNew += hscal->etrial[j][1];
+        if (contact) eNew += hscal->mtrial[j];
+        if (!std::isfinite(eNew)) {
+          if (contact) contact->add_rejection();
+          alpha *= 0.5;
+          continue;
+        }
+        if (eNew <= energy + 1e-4 * alpha * gtdx) {
+          std::swap(x.ptr, tbuf[j].ptr);
+          energy = eNew;
+          if (contact) contact->commit(x.ptr);
+          return true;
+        }
+        if (contact) contact->add_rejection();
+        alpha *= 0.5;
+      }
+      li += K;
+      batch *= 2;
+    }
+    return false;
+  }
 
   PcgArgs pcg_args(double invH2) {
     PcgArgs a;
@@ -916,6 +1041,8 @@ int guarded(F&& f) {
     return fail(CS_E_NONFINITE, "non-finite gradient \xe2\x80\x94 blow up; reduce dt/stiffness");
   } catch (const cloth::OverflowError& e) {
     return fail(CS_E_OVERFLOW, e.what());
+  } catch (const cloth::UnsupportedError& e) {
+    return fail(CS_E_UNSUPPORTED, e.what());
   } catch (const cloth::HipError& e) {
     return fail(CS_E_HIP, e.what());
   } catch (const std::invalid_argument& e) {
@@ -1089,6 +1216,17 @@ int cs_set_material(cs_solver* s, const cs_material* mat) {
   if (!mat) return fail(CS_E_INVALID, "null material");
   s->s.mat = *mat;
   return CS_OK;
+}
+
+int cs_set_static_mesh(cs_solver* s, const float* pos, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (n_tris && (!pos || !idx)) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(s->s.device));
+    s->s.contact->set_static_mesh(pos, n_verts, idx, n_tris);
+    return CS_OK;
+  });
 }
 
 int cs_pin_vertex(cs_solver* s, uint32_t vid, const double* pos) {
@@ -1332,6 +1470,56 @@ int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t
   });
 }
 
+int cs_probe_candidates(cs_solver* s, const double* x0, const double* x, int32_t pad_is_dmin, uint32_t* self_pairs,
+                        uint64_t self_cap, uint64_t* n_self, uint32_t* static_pairs, uint64_t static_cap,
+                        uint64_t* n_static) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (!x0 || !x || !n_self || !n_static) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x0.ptr, x0, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.contact->begin_step(S.x0.ptr);
+    std::vector<uint32_t> a, b;
+    S.contact->probe_pairs(S.x.ptr, S.contact->pad(pad_is_dmin != 0), a, b);
+    *n_self = a.size() / 2;
+    *n_static = b.size() / 2;
+    if (self_pairs) std::copy(a.begin(), a.begin() + std::min<size_t>(a.size(), 2 * self_cap), self_pairs);
+    if (static_pairs) std::copy(b.begin(), b.begin() + std::min<size_t>(b.size(), 2 * static_cap), static_pairs);
+    return CS_OK;
+  });
+}
+
+int cs_probe_contact_adjoint(cs_solver* s, const double* x_step, const double* x, const double* lam, double* grad,
+                             double* vjp) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (!x_step || !x || !grad || (lam && !vjp)) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    S.adj_alloc();
+    CS_HIP(hipMemcpyAsync(S.x0.ptr, x_step, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.contact->begin_step(S.x0.ptr);
+    S.contact->update_active_set(S.x.ptr);
+    S.contact->energy_grad(S.x.ptr, S.ceg.ptr);
+    CS_HIP(hipMemcpyAsync(grad, S.ceg.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, S.st));
+    if (lam) {
+      CS_HIP(hipMemcpyAsync(S.adj_lam.ptr, lam, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+      CS_HIP(hipMemsetAsync(S.adj_gp.ptr, 0, n3 * sizeof(double), S.st));
+      S.contact->friction_xstep_adjoint(S.x.ptr, S.adj_lam.ptr, S.adj_gp.ptr);
+      CS_HIP(hipMemcpyAsync(vjp, S.adj_gp.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost, S.st));
+    }
+    CS_HIP(hipStreamSynchronize(S.st));
+    return CS_OK;
+  });
+}
+
 int cs_probe_check_trial(cs_solver* s, const double* x0, const double* xt, int32_t* valid, double* min_dist,
                          double* min_toi) {
   if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
@@ -1405,6 +1593,28 @@ int cs_device_copy(int device, void* dst, const void* src, size_t bytes, int dir
   return guarded([&]() -> int {
     CS_HIP(hipSetDevice(device));
     CS_HIP(hipMemcpy(dst, src, bytes, kind));
+    return CS_OK;
+  });
+}
+
+int cs_probe_hinges(int device, const double* X12, const double* P12, const double* rest, const double* len,
+                    const double* asum, double k, uint64_t count, double* grad12, double* hvp12) {
+  if (count && (!X12 || !P12 || !rest || !len || !asum || !grad12 || !hvp12)) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    cloth::probe_hinges(X12, P12, rest, len, asum, k, count, grad12, hvp12);
+    return CS_OK;
+  });
+}
+
+int cs_ccd_queries(int device, const double* x0, const double* x1, uint32_t n_verts, const int32_t* kinds,
+                   const int32_t* ids4, uint64_t count, double thickness, double* toi, double* kernel_ms) {
+  if (count && (!x0 || !x1 || !kinds || !ids4 || !toi)) return fail(CS_E_INVALID, "null argument");
+  if (count && (reinterpret_cast<uintptr_t>(ids4) & 15)) return fail(CS_E_INVALID, "ids4 must be 16-byte aligned");
+  if (!std::isfinite(thickness)) return fail(CS_E_INVALID, "thickness must be finite");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    cloth::ccd_queries(x0, x1, n_verts, kinds, ids4, count, thickness, toi, kernel_ms);
     return CS_OK;
   });
 }

@@ -21,6 +21,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libclothhip.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_up = C.POINTER(C.c_uint32)
 
 
 class ClothError(RuntimeError):
@@ -45,6 +47,7 @@ def load_library(path=LIB_PATH):
         "cs_step": (C.c_int, [C.c_void_p, C.c_double]),
         "cs_set_material": (C.c_int, [C.c_void_p, C.POINTER(_abi.CsMaterial)]),
         "cs_pin_vertex": (C.c_int, [C.c_void_p, C.c_uint32, _dp]),
+        "cs_set_static_mesh": (C.c_int, [C.c_void_p, _fp, C.c_uint32, _up, C.c_uint32]),
         "cs_unpin_vertex": (C.c_int, [C.c_void_p, C.c_uint32]),
         "cs_get_positions": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
         "cs_get_velocities": (C.c_int, [C.c_void_p, _dp, C.c_size_t]),
@@ -70,7 +73,10 @@ def load_library(path=LIB_PATH):
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "cs_probe_contact_adjoint": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
         "cs_probe_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, _dp]),
+        "cs_probe_candidates": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int32, _up, C.c_uint64,
+                                          C.POINTER(C.c_uint64), _up, C.c_uint64, C.POINTER(C.c_uint64)]),
         "cs_probe_ccd": (C.c_int, [C.c_int, _dp, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                    C.c_uint64, C.c_double, _dp]),
         "cs_probe_closest": (C.c_int, [C.c_int, _dp, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -78,6 +84,9 @@ def load_library(path=LIB_PATH):
         "cs_device_alloc": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]),
         "cs_device_free": (C.c_int, [C.c_int, C.c_void_p]),
         "cs_device_copy": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+        "cs_probe_hinges": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_uint64, _dp, _dp]),
+        "cs_ccd_queries": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                     C.c_double, C.c_void_p, _dp]),
         "cs_broadphase_self_pairs": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                                C.c_uint32, C.c_double, C.c_void_p, C.c_uint64,
                                                C.POINTER(C.c_uint64), _dp, _dp]),
@@ -144,6 +153,54 @@ def broadphase_self_pairs(x0, x1, idx, pad, device=0, cap=None, timing=False):
         return (pairs, bms.value, qms.value) if timing else pairs
 
 
+def probe_hinges(X, P, rest, length, asum, k, device=0):
+    """Analytic gradient and Hessian-vector product of hingeEnergy on the
+    device for independent hinges: X, P (count, 12) -> (grad, hvp) (count, 12)."""
+    L = load_library()
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 12)
+    P = np.ascontiguousarray(P, np.float64).reshape(-1, 12)
+    r, ln, a = (np.ascontiguousarray(np.broadcast_to(v, (len(X),)), np.float64) for v in (rest, length, asum))
+    g = np.zeros_like(X)
+    hv = np.zeros_like(X)
+    _check(L.cs_probe_hinges(device, X.ctypes.data_as(_dp), P.ctypes.data_as(_dp), r.ctypes.data_as(_dp),
+                             ln.ctypes.data_as(_dp), a.ctypes.data_as(_dp), float(k), len(X), g.ctypes.data_as(_dp),
+                             hv.ctypes.data_as(_dp)))
+    return g, hv
+
+
+def expand_pairs(pairs, idx):
+    """expandTriPair (self-collision.ts:23-41) of sorted triangle pairs as
+    CCD primitives: per pair the 3 + 3 vertex-triangle prims (each vertex of
+    one triangle against the other) and the 3 x 3 edge-edge prims; returns
+    (kinds int32, ids4 int32 (k, 4))."""
+    T = np.asarray(idx, np.int64).reshape(-1, 3)
+    A, B = T[pairs[:, 0]], T[pairs[:, 1]]
+    vt = [np.stack([P[:, c], Q[:, 0], Q[:, 1], Q[:, 2]], 1) for P, Q in ((A, B), (B, A)) for c in range(3)]
+    ea = [(A[:, 0], A[:, 1]), (A[:, 1], A[:, 2]), (A[:, 2], A[:, 0])]
+    eb = [(B[:, 0], B[:, 1]), (B[:, 1], B[:, 2]), (B[:, 2], B[:, 0])]
+    ee = [np.stack([a, b, c, d], 1) for a, b in ea for c, d in eb]
+    ids = np.concatenate(vt + ee).astype(np.int32)
+    kinds = np.concatenate([np.zeros(6 * len(pairs), np.int32), np.ones(9 * len(pairs), np.int32)])
+    return kinds, np.ascontiguousarray(ids)
+
+
+def ccd_queries(x0, x1, kinds, ids4, thickness, device=0, timing=False):
+    """vtCCD / eeCCD over device copies of the inputs; with timing, also the
+    kernel time in ms."""
+    from .devmem import DeviceArray
+    L = load_library()
+    x0 = np.ascontiguousarray(x0, np.float64)
+    d = [DeviceArray(a, device=device) for a in (x0, np.ascontiguousarray(x1, np.float64),
+                                                  np.ascontiguousarray(kinds, np.int32),
+                                                  np.ascontiguousarray(ids4, np.int32))]
+    out = DeviceArray(dtype=np.float64, shape=(len(kinds),), device=device)
+    ms = C.c_double()
+    _check(L.cs_ccd_queries(device, d[0].ptr, d[1].ptr, len(x0) // 3, d[2].ptr, d[3].ptr, len(kinds),
+                            float(thickness), out.ptr, C.byref(ms) if timing else None))
+    toi = out.numpy()
+    return (toi, ms.value) if timing else toi
+
+
 def probe_ccd(x0, x1, kinds, ids4, thickness, device=0):
     """vtCCD / eeCCD on the device for a batch of primitives."""
     L = load_library()
@@ -191,7 +248,29 @@ class HipSolver:
         _check(self._lib.cs_create(C.byref(desc), C.byref(self._opts), C.byref(h)))
         self._h = h
         self.scene = scene
+        self._static_seen = self._static_state()
         del keep
+
+    def _static_state(self):
+        c = self.scene.contact
+        if c is None or c.staticPos is None:
+            return None
+        return c.staticPos, np.array(c.staticPos, np.float32, copy=True)
+
+    def _sync_static(self):
+        """The device copy of scene.contact's static mesh follows the caller's
+        setStaticMesh calls (a moving collider), checked before every step."""
+        c = self.scene.contact
+        if c is None or c.staticPos is None:
+            return
+        seen = self._static_seen
+        if seen is not None and c.staticPos is seen[0] and np.array_equal(c.staticPos, seen[1]):
+            return
+        pos = np.ascontiguousarray(c.staticPos, np.float32)
+        idx = np.ascontiguousarray(c.staticIdx, np.uint32)
+        _check(self._lib.cs_set_static_mesh(self._h, pos.ctypes.data_as(_fp), len(pos) // 3,
+                                            idx.ctypes.data_as(_up), len(idx) // 3))
+        self._static_seen = self._static_state()
 
     def _need(self):
         if self._h is None:
@@ -199,6 +278,7 @@ class HipSolver:
 
     def step(self, dt):
         self._need()
+        self._sync_static()
         _check(self._lib.cs_step(self._h, float(dt)))
         self._copy_out()
 
@@ -361,6 +441,39 @@ class HipSolver:
         _check(self._lib.cs_probe_check_trial(self._h, x0.ctypes.data_as(_dp), xt.ctypes.data_as(_dp), C.byref(v),
                                               C.byref(md), C.byref(mt)))
         return bool(v.value), md.value, mt.value
+
+    def probe_contact_adjoint(self, x_step, x, lam=None):
+        """(contact gradient at x for a step begun at x_step, and -(d grad /
+        d x_step)^T lam) through cs_probe_contact_adjoint."""
+        self._need()
+        xs = np.ascontiguousarray(x_step, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        grad = np.zeros_like(x)
+        lam = None if lam is None else np.ascontiguousarray(lam, np.float64)
+        vjp = None if lam is None else np.zeros_like(x)
+        lam_p = None if lam is None else lam.ctypes.data_as(_dp)
+        _check(self._lib.cs_probe_contact_adjoint(self._h, xs.ctypes.data_as(_dp), x.ctypes.data_as(_dp), lam_p,
+                                                  grad.ctypes.data_as(_dp),
+                                                  vjp.ctypes.data_as(_dp) if vjp is not None else None))
+        return grad, vjp
+
+    def probe_candidates(self, x0, x, pad_is_dmin=False):
+        """The cloth and cloth/static triangle pairs the contact system
+        expands into candidates at x for a step begun at x0 (pad dHatM, or
+        dMinM with pad_is_dmin), through cs_probe_candidates: two (k, 2)
+        uint32 arrays in the traversal's order."""
+        self._need()
+        x0 = np.ascontiguousarray(x0, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        ns, nt = C.c_uint64(), C.c_uint64()
+        _check(self._lib.cs_probe_candidates(self._h, x0.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                             int(bool(pad_is_dmin)), None, 0, C.byref(ns), None, 0, C.byref(nt)))
+        sp = np.zeros((max(ns.value, 1), 2), np.uint32)
+        tp = np.zeros((max(nt.value, 1), 2), np.uint32)
+        _check(self._lib.cs_probe_candidates(self._h, x0.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                             int(bool(pad_is_dmin)), sp.ctypes.data_as(_up), ns.value,
+                                             C.byref(ns), tp.ctypes.data_as(_up), nt.value, C.byref(nt)))
+        return sp[:ns.value], tp[:nt.value]
 
     def _copy_out(self, velocities=True):
         sc = self.scene

@@ -76,6 +76,14 @@ enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 
  * Takes effect with CS_HVP_ANALYTIC; the gradient, energy, line search and
  * the adjoint's Hessian are unchanged. */
 #define CS_FLAG_PSD_NEWTON 8u
+/* Device Newton control (no reference counterpart; the reference's GPU path
+ * does the same, src/backend/webgpu/gpu-newton.ts:1889-2200): the descent
+ * safeguard and the trust region are decided on the device, and the Armijo
+ * trials are evaluated in batches of 1, 2, 4, 8 with one host read per batch
+ * instead of two to three per trial.  The trials are judged in the
+ * reference's order and rules, so positions, energies, trial counts and
+ * contact diagnostics are those of the sequential line search. */
+#define CS_FLAG_DEVICE_NEWTON 32u
 
 /* ClothMaterial (src/physics/types.ts:1-14). */
 typedef struct cs_material {
@@ -200,6 +208,15 @@ int cs_set_material(cs_solver* s, const cs_material* mat);
  * position; pins are enforced on the positions immediately. */
 int cs_pin_vertex(cs_solver* s, uint32_t vertex_id, const double* pos_or_null);
 
+/* ContactSystem.setStaticMesh(pos, idx) — contact-assembly.ts:71-73, called
+ * between steps; a moving rigid collider is the collider's mesh sampled at
+ * each tick and attached this way (toCollisionObject + attachCollisionObject,
+ * avatar-collider.ts:205-230).  The static triangles (count and corner ids)
+ * are fixed at cs_create; new positions (f32, 3 n_verts) take effect at the
+ * next step.  CS_E_UNSUPPORTED for another topology, CS_E_INVALID without a
+ * contact system. */
+int cs_set_static_mesh(cs_solver* s, const float* pos, uint32_t n_verts, const uint32_t* idx, uint32_t n_tris);
+
 /* ClothSolver.unpinVertex — cpu-solver.ts:35-37. */
 int cs_unpin_vertex(cs_solver* s, uint32_t vertex_id);
 
@@ -271,13 +288,16 @@ int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int thre
  * state the step produced (x_next; the velocity follows from it), the same
  * pins and material as that step, and a loss's gradients dL/dx_next,
  * dL/dv_next (3n each), solves H lambda = dL/dx_next + (damp/dt) dL/dv_next on
- * the free DOFs (H = d(Newton gradient)/dx at x_next, applied as a central
- * difference of the step's own gradient; Jacobi PCG to relative residual
- * tol, at most max_iters iterations) and writes dL/dx_prev, dL/dv_prev (3n)
+ * the free DOFs, where H = d(Newton gradient)/dx at x_next is applied as the
+ * analytic inertia + membrane Hessian plus central differences of the
+ * bending and contact gradients, by block-Jacobi PCG to relative residual
+ * tol (at most max_iters iterations), and writes dL/dx_prev, dL/dv_prev (3n)
  * and dL_dtheta[5] = dL/d(stretchWarp, stretchWeft, stretchCoupling, shear,
  * kb) with kb = (bendWarp + bendWeft) / 2, the hinge stiffness of
  * src/physics/fem.ts:44.  *iters / *rel_residual report the solve.  Contact,
- * when the scene has it, enters with the active set frozen at x_next; the
+ * when the scene has it, enters with the active set frozen at x_next and the
+ * lagged friction normal and force held; the friction slip is measured from
+ * x_prev (contact-assembly.ts:344-367), which dL/dx_prev includes; the
  * velocity filter is not differentiated. */
 int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double* v_prev, const double* x_next,
                     const double* dL_dx_next, const double* dL_dv_next, double tol, int max_iters,
@@ -320,6 +340,24 @@ int cs_probe_schwarz(cs_solver* s, const double* x, const double* diag, const do
  * order: ids6 = (kind, p, a, b, c, d), vals7 = (w0, w1, w2, w3, nx, ny, nz). */
 int cs_probe_active_set(cs_solver* s, const double* x0, const double* x, int32_t* ids6, double* vals7,
                         uint64_t cap, uint64_t* count, uint64_t* n_candidates);
+/* Begin a step at x0 and list the triangle pairs the contact system expands
+ * into candidates at x (buildCandidates, contact-assembly.ts:87-121: swept
+ * boxes x0 -> x padded by dMinM when pad_is_dmin, else by dHatM): cloth pairs
+ * (tA < tB) and cloth/static pairs (cloth triangle, static triangle), 2 u32
+ * each, unordered; *n_self / *n_static receive the counts (up to the caps are
+ * written). */
+int cs_probe_candidates(cs_solver* s, const double* x0, const double* x, int32_t pad_is_dmin, uint32_t* self_pairs,
+                        uint64_t self_cap, uint64_t* n_self, uint32_t* static_pairs, uint64_t static_cap,
+                        uint64_t* n_static);
+/* Component probe of the step adjoint's contact terms (no reference
+ * counterpart): begins a step at x_step (beginStep, contact-assembly.ts:75-81,
+ * which clears the lagged table), builds the active set at x and writes grad =
+ * energyGrad at x (barrier + lagged friction, contact-assembly.ts:262-285,
+ * 3n), and, when lam is not NULL, vjp = -(d grad / d x_step)^T lam: the lagged
+ * friction's share of dL/dx_prev that cs_step_adjoint adds (the friction slip
+ * is measured from x_step, :344-367). */
+int cs_probe_contact_adjoint(cs_solver* s, const double* x_step, const double* x, const double* lam, double* grad,
+                             double* vjp);
 /* Begin a step at x0 and validate trial xt (checkTrial, contact-assembly.ts:489-553).
  * min_dist / min_toi are the minima recorded when the trial is valid. */
 int cs_probe_check_trial(cs_solver* s, const double* x0, const double* xt, int32_t* valid, double* min_dist,
@@ -343,6 +381,28 @@ enum { CS_COPY_HOST_TO_DEVICE = 1, CS_COPY_DEVICE_TO_HOST = 2, CS_COPY_DEVICE_TO
 int cs_device_alloc(int device, size_t bytes, void** out);
 int cs_device_free(int device, void* ptr);
 int cs_device_copy(int device, void* dst, const void* src, size_t bytes, int direction);
+
+/* Component probe of the analytic hinge bending terms (no reference
+ * counterpart: the reference differences hingeEnergy, bending.ts:19-42): for
+ * `count` independent hinges (X12: the positions of v0, v1 (shared edge), v2,
+ * v3 (opposite vertices), 12 doubles each; rest angle, edge length, area sum
+ * and stiffness k as in hingeEnergy, bending.ts:7-16), grad12 = the analytic
+ * gradient of the hinge energy and hvp12 = its Hessian times P12.  Host
+ * arrays. */
+int cs_probe_hinges(int device, const double* X12, const double* P12, const double* rest, const double* len,
+                    const double* asum, double k, uint64_t count, double* grad12, double* hvp12);
+
+/* ---- batched CCD queries ---------------------------------------------------
+ * vtCCD / eeCCD (src/collision/ccd-vt.ts:20-115, ccd-ee.ts:24-101, the
+ * time-of-impact queries checkTrial issues per candidate primitive,
+ * contact-assembly.ts:489-553) over DEVICE arrays on `device`: kinds[i] 0 = VT
+ * (p, a, b, c), 1 = EE (a, b, c, d); ids4 (16-byte aligned) index the
+ * interleaved xyz arrays x0 (start) and x1 (end) of n_verts vertices; toi[i]
+ * receives the time of impact in {0} u (0, 1] u {Infinity}.  An id outside
+ * [0, n_verts) is CS_E_INVALID.  Synchronous on the null stream; kernel_ms
+ * (optional) receives the kernel's HIP-event time. */
+int cs_ccd_queries(int device, const double* x0, const double* x1, uint32_t n_verts, const int32_t* kinds,
+                   const int32_t* ids4, uint64_t count, double thickness, double* toi, double* kernel_ms);
 
 /* ---- standalone broadphase -------------------------------------------------
  * BroadPhase.build(x0, x1) (src/collision/broadphase.ts:61-64) for the cloth's

@@ -2315,6 +2315,15 @@ int orc_step(orc_solver* o, double dt) {
   }
 }
 
+// ContactSystem.setStaticMesh (contact-assembly.ts:71-73): the arrays replaced.
+int orc_set_static_mesh(orc_solver* o, const float* pos, uint32_t ns, const uint32_t* idx, uint32_t ms) {
+  Solver& s = o->s;
+  if (!s.hasContact) { g_err = "scene has no contact system"; return CS_E_INVALID; }
+  s.contact.staticPos.assign(pos, pos + (size_t)ns * 3);
+  s.contact.staticIdx.assign(idx, idx + (size_t)ms * 3);
+  return CS_OK;
+}
+
 int orc_pin_vertex(orc_solver* o, uint32_t id, const double* pos) {
   Solver& s = o->s;
   if (id >= s.mesh.n) { g_err = "vertex id out of range"; return CS_E_INVALID; }
@@ -2360,6 +2369,20 @@ void orc_membrane_hvp_analytic(orc_solver* o, const double* x, const double* p, 
 }
 void orc_membrane_blocks(orc_solver* o, const double* x, double* blocks) {
   membrane_blocks(o->s.mesh, o->s.mat, x, blocks);
+}
+
+// buildCandidates (contact-assembly.ts:87-121) at x with the given pad after beginStep(x0).
+int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap) {
+  Solver& s = o->s;
+  if (!s.hasContact) return -1;
+  const size_t n3 = (size_t)s.mesh.n * 3;
+  s.contact.begin_step(std::vector<double>(x0, x0 + n3));
+  const auto c = s.contact.build_candidates(x, pad_is_dmin ? s.contact.params.d_min : s.contact.params.d_hat);
+  for (int64_t k = 0; k < (int64_t)c.size() && k < cap; k++) {
+    const int v[6] = {c[k].kind, c[k].p, c[k].a, c[k].b, c[k].c, c[k].d};
+    for (int j = 0; j < 6; j++) ids6[k * 6 + j] = v[j];
+  }
+  return (int64_t)c.size();
 }
 
 int64_t orc_contact_probe(orc_solver* o, const double* x0, const double* x, double pad_is_dmin, int32_t* ids6,

@@ -65,6 +65,7 @@ int64_t orc_self_pairs(const double* x0, const double* x1, const uint32_t* idx, 
 orc_solver* orc_create(const cs_scene_desc* scene, const cs_options* opt);
 void orc_destroy(orc_solver* s);
 int orc_step(orc_solver* s, double dt);           /* 0 ok, CS_E_NONFINITE on blow-up */
+int orc_set_static_mesh(orc_solver* o, const float* pos, uint32_t ns, const uint32_t* idx, uint32_t ms);
 int orc_pin_vertex(orc_solver* s, uint32_t id, const double* pos_or_null);
 int orc_unpin_vertex(orc_solver* s, uint32_t id);
 void orc_set_material(orc_solver* s, const cs_material* mat);
@@ -77,6 +78,7 @@ const char* orc_last_error(void);
 /* Contact-system probes for set-equality tests: begin a step at x0 and build the
  * frozen active set at x.  Returns the active count; records are written as
  * 6 ints (kind,p,a,b,c,d) + 7 doubles (w0..w3,nx,ny,nz) per contact. */
+int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap);
 int64_t orc_contact_probe(orc_solver* s, const double* x0, const double* x, double pad_is_dmin,
                           int32_t* ids6, double* vals7, int64_t cap, int64_t* n_candidates);
 /* checkTrial(xT) from step start x0: returns valid (0/1); fills minDist/minTOI. */

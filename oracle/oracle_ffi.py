@@ -60,6 +60,7 @@ def lib():
             "orc_destroy": (None, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_double]),
             "orc_pin_vertex": (C.c_int, [C.c_void_p, C.c_uint32, _dp]),
+            "orc_set_static_mesh": (C.c_int, [C.c_void_p, _fp, C.c_uint32, _up, C.c_uint32]),
             "orc_unpin_vertex": (C.c_int, [C.c_void_p, C.c_uint32]),
             "orc_set_material": (None, [C.c_void_p, C.POINTER(_abi.CsMaterial)]),
             "orc_get_positions": (None, [C.c_void_p, _dp]),
@@ -70,6 +71,7 @@ def lib():
             "orc_contact_probe": (C.c_int64, [C.c_void_p, _dp, _dp, C.c_double, _ip, _dp, C.c_int64,
                                               C.POINTER(C.c_int64)]),
             "orc_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+            "orc_candidates": (C.c_int64, [C.c_void_p, _dp, _dp, C.c_int, _ip, C.c_int64]),
             "orc_schwarz_domains": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _up, _up]),
             "orc_schwarz_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                             C.POINTER(C.c_uint8)]),
@@ -130,6 +132,15 @@ class OracleSolver:
             raise RuntimeError(lib().orc_last_error().decode(errors="replace"))
         lib().orc_get_positions(self.h, P(self.scene.positions))
         lib().orc_get_velocities(self.h, P(self.scene.velocities))
+
+    def set_static_mesh(self, pos, idx):
+        """ContactSystem.setStaticMesh (contact-assembly.ts:71-73)."""
+        pos = np.ascontiguousarray(pos, np.float32)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        rc = lib().orc_set_static_mesh(self.h, pos.ctypes.data_as(_fp), len(pos) // 3, idx.ctypes.data_as(_up),
+                                       len(idx) // 3)
+        assert rc == 0
+        self.scene.contact.staticPos = pos
 
     def pinVertex(self, vid, position=None):
         if position is None:
@@ -205,6 +216,20 @@ class OracleSolver:
         if k > cap:
             return self.active_set(x0, x, cap=int(k), candidates_dmin=candidates_dmin)
         return ids[:k].copy(), vals[:k].copy(), nc.value
+
+    def candidates(self, x0, x, dmin=False, cap=None):
+        """buildCandidates (contact-assembly.ts:87-121) at x after beginStep(x0),
+        padded by dMinM (dmin) or dHatM: sorted (k, 6) int32 rows
+        (kind, p, a, b, c, d)."""
+        x0 = d3(x0); x = d3(x)
+        cap = cap or 1 << 16
+        ids = np.zeros((cap, 6), np.int32)
+        k = lib().orc_candidates(self.h, P(x0), P(x), 1 if dmin else 0, P(ids, C.c_int32), cap)
+        if k < 0:
+            raise RuntimeError("scene has no contact system")
+        if k > cap:
+            return self.candidates(x0, x, dmin=dmin, cap=int(k))
+        return ids[:k].copy()
 
     def schwarz_domains(self, target=16, minsz=8, maxsz=32):
         n = len(self.getPositions()) // 3

@@ -66,6 +66,80 @@ def test_ccd_known_answers_and_bit_exact_vs_oracle():
     assert np.isfinite(got).sum() > 100  # the sample exercises real impacts
 
 
+def test_moving_collider_trajectory_matches_oracle():
+    """A moving rigid collider: the static plate of static-collider.test.ts,
+    once the patch rests on it, slides 0.2 mm in x and 0.1 mm in z per step,
+    attached before every step through ContactSystem.setStaticMesh
+    (contact-assembly.ts:71-73), the way avatar-collider.ts:205-230 attaches
+    a sampled avatar mesh.  (The static mesh is not swept within a step, so
+    motion that closes the gap by more than it is between steps would start
+    a step inside d_min, in the reference as here.)  The FP64 HipSolver
+    follows the oracle bit for bit, the contacts hold on every step and the
+    patch stays above the plate by more than d_min."""
+    sg, so = S.static_plate_patch(), S.static_plate_patch()
+    g = HipSolver("fp64")
+    g.initialize(sg)
+    o = O.OracleSolver(so, _abi.options())
+    base = sg.contact.staticPos.reshape(-1, 3).astype(np.float64)
+    idx = sg.contact.staticIdx.copy()
+    for k in range(45):
+        pos = (base + np.array([2e-4, 0.0, 1e-4]) * max(0, k - 14)).astype(np.float32).reshape(-1)
+        sg.contact.setStaticMesh(pos, idx)
+        o.set_static_mesh(pos, idx)
+        g.step(H)
+        o.step(H)
+        sgs, sos = g.lastStats, o.lastStats
+        assert np.array_equal(sg.positions, so.positions), k
+        assert np.array_equal(sg.velocities, so.velocities), k
+        for key in ("newton_used", "pcg_iters", "line_search_trials"):
+            assert sgs[key] == sos[key], (k, key)
+        assert sgs["contact"]["active_pairs"] == sos["contact"]["active_pairs"]
+        assert sgs["contact"]["min_distance"] == sos["contact"]["min_distance"]
+        if k >= 14:
+            assert sgs["contact"]["active_pairs"] > 0, k
+            assert sgs["contact"]["min_distance"] > sg.contact.params.dMinM, k
+    top = float(pos.reshape(-1, 3)[:, 1].max())
+    assert float(np.min(sg.positions[1::3])) - top > sg.contact.params.dMinM
+
+
+def test_static_mesh_topology_is_fixed():
+    sg = S.static_plate_patch()
+    g = HipSolver("fp64")
+    g.initialize(sg)
+    from closim_amd.solver import ClothError
+    sg.contact.setStaticMesh(sg.contact.staticPos, np.array([0, 1, 2], np.uint32))
+    with pytest.raises(ClothError, match="fixed at creation"):
+        g.step(H)
+
+
+def test_device_ccd_queries_match_oracle_on_expanded_candidates():
+    """cs_ccd_queries (device arrays) on the VT/EE prims of a crumpled sheet's
+    candidate pairs, bit-exact against the oracle's vtCCD / eeCCD; an id out
+    of range is rejected."""
+    from closim_amd.solver import broadphase_self_pairs, ccd_queries, expand_pairs
+    from closim_amd.mesh import build_grid
+    rng = np.random.default_rng(11)
+    pos, uv, idx = build_grid(15, 13, 0.1, 0.08)
+    X = pos.astype(np.float64).reshape(-1, 3)
+    X[:, 1] = 0.01 * np.sin(60 * X[:, 0]) * np.cos(50 * X[:, 2])
+    x0 = X.reshape(-1).copy()
+    x1 = x0 + rng.normal(0, 0.003, x0.shape)
+    pairs = broadphase_self_pairs(x0, x1, idx, 1e-3)
+    kinds, ids = expand_pairs(pairs, idx)
+    assert len(kinds) == 15 * len(pairs) > 1000
+    got, ms = ccd_queries(x0, x1, kinds, ids, 1e-4, timing=True)
+    assert ms > 0
+    for i in range(len(kinds)):
+        f = O.lib().orc_vt_ccd if kinds[i] == 0 else O.lib().orc_ee_ccd
+        ref = f(O.P(x0), O.P(x1), *[int(v) for v in ids[i]], 1e-4)
+        assert got[i] == ref or (math.isnan(got[i]) and math.isnan(ref)), (i, got[i], ref)
+    assert (got < math.inf).sum() > 10
+    bad = ids.copy()
+    bad[3, 2] = len(x0) // 3
+    with pytest.raises(Exception, match="out of range"):
+        ccd_queries(x0, x1, kinds, bad, 1e-4)
+
+
 @pytest.mark.parametrize("name", sorted(ALL))
 def test_active_set_bit_exact_and_ordered(name):
     scene = ALL[name]()
