@@ -33,6 +33,8 @@
 // Inner boxes are conservative (outward rounding; a NaN coordinate is ignored
 // by the FP32 min/max, and a leaf with a NaN coordinate fails the exact test
 // anyway), so the result is exactly the set of the FP64 predicate.
+#include <mutex>
+
 #include "../../include/cloth_hip.h"
 #include "lbvh.h"
 #include "sort.h"
@@ -581,10 +583,21 @@ bool run_self_pairs(hipStream_t st, unsigned char* base, const double* x0, const
   return true;
 }
 
+constexpr size_t kNoAllocBytes = layout_for(kLbvhMaxTris).total;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
 
 }  // namespace
 
+// Scratch of the allocation-free entry point: one copy per GPU (each device
+// holds its own instance at its own address), for up to kLbvhMaxTris
+// triangles.  Host code never passes this name to a kernel: it passes the
+// device address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 size_t lbvh_scratch_bytes(uint32_t m) { return layout_for(m).total; }
+size_t lbvh_noalloc_bytes() { return kNoAllocBytes; }
 
 bool lbvh_self_pairs(const double* x0, const double* x1, uint32_t n, const uint32_t* idx, uint32_t m, double pad,
                      uint32_t* pairs, uint64_t cap, uint64_t* count, double* build_ms, double* query_ms) {
@@ -613,6 +626,27 @@ bool lbvh_self_pairs(const double* x0, const double* x1, uint32_t n, const uint3
     if (query_ms) *query_ms = b;
   }
   return ok;
+}
+
+bool lbvh_self_pairs_noalloc(int device, const double* x0, const double* x1, uint32_t n, const uint32_t* idx,
+                             uint32_t m, double pad, uint32_t* pairs, uint64_t cap, uint64_t* count) {
+  *count = 0;
+  if (m > kLbvhMaxTris)
+    throw std::invalid_argument("n_tris exceeds the allocation-free scratch (at most " +
+                                std::to_string(kLbvhMaxTris) + " triangles)");
+  if (device < 0 || device >= kMaxDevices) throw std::invalid_argument("device ordinal out of range");
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      // the array's device address on the current device (each GPU holds its own copy)
+      CS_HIP(hipGetSymbolAddress(&g_scratch_ptr[device], HIP_SYMBOL(scratch)));
+    }
+    ptr = g_scratch_ptr[device];
+  }
+  if (m == 0) return true;
+  return run_self_pairs(nullptr, static_cast<unsigned char*>(ptr), x0, x1, n, idx, m, pad, pairs, cap, count,
+                        nullptr);
 }
 
 }  // namespace cloth

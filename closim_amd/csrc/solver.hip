@@ -1636,4 +1636,23 @@ int cs_broadphase_self_pairs(int device, const double* x0, const double* x1, uin
   });
 }
 
+int cs_broadphase_self_pairs_noalloc(int device, const double* x0, const double* x1, uint32_t n_verts,
+                                     const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs,
+                                     uint64_t cap, uint64_t* count) {
+  if (!count) return fail(CS_E_INVALID, "null count");
+  *count = 0;
+  if (n_tris && (!x0 || !x1 || !indices)) return fail(CS_E_INVALID, "null argument");
+  if (cap && !pairs) return fail(CS_E_INVALID, "null pairs buffer");
+  if (n_tris > cloth::kLbvhMaxTris)
+    return fail(CS_E_INVALID, "n_tris exceeds the allocation-free scratch (at most " +
+                                  std::to_string(cloth::kLbvhMaxTris) + " triangles)");
+  if (!std::isfinite(pad) || pad < 0) return fail(CS_E_INVALID, "pad must be finite and non-negative");
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(device));
+    if (!cloth::lbvh_self_pairs_noalloc(device, x0, x1, n_verts, indices, n_tris, pad, pairs, cap, count))
+      return fail(CS_E_OVERFLOW, "candidate pairs exceed the buffer capacity (count holds the number needed)");
+    return CS_OK;
+  });
+}
+
 }  // extern "C"

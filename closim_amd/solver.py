@@ -90,6 +90,9 @@ def load_library(path=LIB_PATH):
         "cs_broadphase_self_pairs": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
                                                C.c_uint32, C.c_double, C.c_void_p, C.c_uint64,
                                                C.POINTER(C.c_uint64), _dp, _dp]),
+        "cs_broadphase_self_pairs_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                                       C.c_uint32, C.c_double, C.c_void_p, C.c_uint64,
+                                                       C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -424,6 +424,18 @@ int cs_broadphase_self_pairs(int device, const double* x0, const double* x1, uin
                              const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs, uint64_t cap,
                              uint64_t* count, double* build_ms, double* query_ms);
 
+/* No counterpart in the reference: cs_broadphase_self_pairs for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results, without the timing outputs; it
+ * calls no hipMalloc / hipFree.  Its scratch is a fixed device array inside
+ * the library (one copy per GPU, about 69 MB, held until the process ends),
+ * which limits n_tris to 524,288 (CS_E_INVALID beyond).  Every call on a GPU
+ * uses that GPU's one array: calls that overlap on one GPU (several host
+ * threads, or work still running on another stream) race. */
+int cs_broadphase_self_pairs_noalloc(int device, const double* x0, const double* x1, uint32_t n_verts,
+                                     const uint32_t* indices, uint32_t n_tris, double pad, uint32_t* pairs,
+                                     uint64_t cap, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif
