@@ -29,6 +29,8 @@ Prints one JSON line:
                (cs_batch_step) vs one instance at a time in FP64
   adjoint      the 128x128 material adjoint: loss gradient by checkpointed
                step adjoints (FP64 and mixed forwards), FD check in shear, kb
+  coarse       the two-level Schwarz preconditioner on the workload: steps/s
+               and PCG iterations per step, one-level vs each coarse level
 """
 import argparse
 import ctypes as C
@@ -131,6 +133,38 @@ def precond_options(workload):
     return {"sphere": dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI),
             "fold": dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64)),
             "strip": dict(precond=_abi.CS_PRECOND_JACOBI)}[workload]
+
+
+def coarse_section(workload, steps, warmup, h):
+    """The two-level Schwarz preconditioner on the workload (mixed
+    precision): steps/s, PCG iterations and Newton iterations per step, and
+    the last step's gradient norm, one-level against the coarse corrections
+    (damped Jacobi, block Jacobi, 8 coarse PCG iterations), each from the
+    same initial state after `warmup` steps."""
+    from closim_amd import _abi
+    popts = precond_options(workload)
+    if popts.get("precond") != _abi.CS_PRECOND_SCHWARZ:
+        return None
+    res = {}
+    for name, mode in (("one_level", _abi.CS_COARSE_NONE), ("mas2", _abi.CS_COARSE_JACOBI),
+                       ("c0", _abi.CS_COARSE_BLOCK), ("c1_8", _abi.CS_COARSE_PCG)):
+        sc = make_scene(workload)
+        g = solver_for(sc, "mixed", dict(popts, coarse=mode), timing=False)
+        for _ in range(warmup):
+            raw_step(g, h)
+        pcg, newton = 0, 0
+        t = 0.0
+        for _ in range(steps):
+            t += timed_steps(g, h, 1)
+            pcg += g.lastStats["pcg_iters"]
+            newton += g.lastStats["newton_used"]
+        res[name] = dict(steps_per_s=steps / t, pcg_iters_per_step=pcg / steps, newton_per_step=newton / steps,
+                         grad_norm_last_step=g.lastStats["grad_norm"])
+        print(f"[bench] coarse {name}: {steps} steps in {t:.3f} s, {pcg / steps:.1f} PCG iterations per step",
+              file=sys.stderr, flush=True)
+        g.close()
+    res["steps"], res["warmup"], res["precision"] = steps, warmup, "mixed"
+    return res
 
 
 def pair_section(workload, steps, warmup, h):
@@ -462,6 +496,8 @@ def main():
     ap.add_argument("--drape-steps", type=int, default=5,
                     help="timed steps of the 256x256 sphere drape section (0 = skip)")
     ap.add_argument("--lbvh-reps", type=int, default=5, help="timed broadphase calls (0 = skip lbvh and ccd)")
+    ap.add_argument("--coarse-steps", type=int, default=5,
+                    help="steps per preconditioner of the two-level Schwarz comparison (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed steps of the 64-instance batched drape section (0 = skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
@@ -589,6 +625,8 @@ def main():
     gm.close()
     if gd:
         gd.close()
+    if args.coarse_steps > 0:
+        out["coarse"] = coarse_section(args.workload, args.coarse_steps, 2, h)
     if args.workload == "fold" and args.drape_steps > 0 and gd:
         out["drape"] = pair_section("sphere", args.drape_steps, 2, h)
     if args.batch_steps > 0 and gd:

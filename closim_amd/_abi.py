@@ -21,6 +21,10 @@ CS_HVP_ANALYTIC = 1
 CS_PRECOND_JACOBI = 0
 CS_PRECOND_BLOCK_JACOBI = 1
 CS_PRECOND_SCHWARZ = 2
+CS_COARSE_NONE = 0    # one-level Schwarz
+CS_COARSE_JACOBI = 1  # + omega P Dc^-1 R (applyMasTwoLevel, mas.ts:126-143)
+CS_COARSE_BLOCK = 2   # + P Bdd^-1 R (the GPU path's C0)
+CS_COARSE_PCG = 3     # + P coarsePcg(Ac, R r, K) (C1, coarse-csr.ts:243-303)
 CS_FLAG_TIMING = 1
 CS_FLAG_POLISH_ROOT = 2
 CS_FLAG_EAGER = 4
@@ -78,6 +82,7 @@ class CsOptions(C.Structure):
         ("precision", C.c_int32), ("hvp_mode", C.c_int32), ("precond", C.c_int32),
         ("schwarz_target", C.c_int32), ("schwarz_min", C.c_int32), ("schwarz_max", C.c_int32),
         ("device", C.c_int32), ("flags", C.c_uint32),
+        ("coarse", C.c_int32), ("coarse_iters", C.c_int32), ("coarse_omega", C.c_double),
     ]
 
 
@@ -104,7 +109,7 @@ class CsStepStats(C.Structure):
 
 def options(newton_iters=10, pcg_max_iters=60, line_search_iters=10, grad_tol=1e-5,
             precision=CS_PREC_FP64, hvp_mode=None, precond=CS_PRECOND_JACOBI,
-            schwarz=(16, 8, 32), device=0, flags=0):
+            schwarz=(16, 8, 32), device=0, flags=0, coarse=CS_COARSE_NONE, coarse_iters=8, coarse_omega=0.5):
     """StepperOptions defaults of the reference (newton.ts:30-33).
 
     hvp_mode defaults to the reference CpuSolver operator (FD) in FP64 and to
@@ -123,6 +128,9 @@ def options(newton_iters=10, pcg_max_iters=60, line_search_iters=10, grad_tol=1e
     o.schwarz_target, o.schwarz_min, o.schwarz_max = schwarz
     o.device = device
     o.flags = flags
+    o.coarse = coarse
+    o.coarse_iters = coarse_iters
+    o.coarse_omega = coarse_omega
     return o
 
 

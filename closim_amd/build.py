@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libclothhip.so")
-SOURCES = ["solver.hip", "fem.hip", "pcg.hip", "contact.hip", "mixed.hip", "sort.hip", "reduce.hip", "schwarz.hip", "lbvh.hip", "bend.hip"]
+SOURCES = ["solver.hip", "fem.hip", "pcg.hip", "contact.hip", "mixed.hip", "sort.hip", "reduce.hip", "schwarz.hip", "lbvh.hip", "bend.hip", "coarse.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: the FP64 path must round every product and sum exactly

@@ -41,7 +41,10 @@ class MixedSolve {
   double time_hvp(int reps);
   // One launch of the solve's HVP kernel (no contact) at (x, p) into out
   // (FP64 device arrays); for component parity tests.
-  void probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p, double* out);
+  // One HVP launch at (x, p) into out (FP64); with `contact`, the frozen
+  // barrier term of its active set in FP32 as the solve applies it.
+  void probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p, double* out,
+                 const DContactHvp* contact = nullptr);
   void set_material(const cs_material&) {}
 
  private:

@@ -771,7 +771,7 @@ __global__ void k_mx_probe_in(const double* x, const double* p, float* x32, floa
 // One HVP launch (the kernel the solve uses, no contact) at (x, p), for
 // component parity tests; overwrites the solve's x, p and Ap buffers.
 void MixedSolve::probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p,
-                           double* out) {
+                           double* out, const DContactHvp* contact) {
   MixedImpl& M = *im_;
   hipStream_t st = M.st;
   const size_t n3 = (size_t)me.n * 3;
@@ -792,6 +792,22 @@ void MixedSolve::probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, cons
     a.tri = M.tri.ptr;
     hipLaunchKernelGGL(k_mx_tri_prep, dim3(grid_for(std::max<uint32_t>(a.me.m, 1))), dim3(kBlock), 0, st, a,
                        M.tri.ptr);
+  }
+  const uint32_t C = contact && contact->act ? contact->C : 0;
+  const uint32_t E = C ? contact->E : 0;
+  if (C) {
+    M.act32.alloc(C);
+    M.scan.alloc((size_t)std::max<uint32_t>(E, 1) * 3);
+    hipLaunchKernelGGL(k_mx_contacts, dim3(grid_for(C)), dim3(kBlock), 0, st, contact->act, contact->inr,
+                       contact->coeff, C, M.act32.ptr);
+    a.C = C;
+    a.E = E;
+    a.act = M.act32.ptr;
+    a.ent_vtx = contact->ent_vtx;
+    a.v_off = contact->v_off;
+    a.v_ent = contact->v_ent;
+    a.scan = E ? M.scan.ptr : nullptr;
+    if (E) hipLaunchKernelGGL(k_mx_cscan, dim3(grid_for(E)), dim3(kBlock), 0, st, a);
   }
   launch_mx_hvp(a, st);
   hipLaunchKernelGGL(k_mx_out, dim3(grid_for(n3)), dim3(kBlock), 0, st, M.Ap.ptr, out, n3);

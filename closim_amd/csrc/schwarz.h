@@ -6,6 +6,7 @@
 #pragma once
 
 #include "../../include/cloth_hip.h"
+#include "coarse.h"
 #include "fem.h"
 
 namespace cloth {
@@ -17,17 +18,22 @@ int schwarz_partition_host(const cs_scene_desc& d, int target, int min_size, uin
 
 class Schwarz {
  public:
-  // Domains from the rest positions (mesh.restPositions) and the triangles.
-  Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st);
+  // Domains from the rest positions (mesh.restPositions) and the triangles;
+  // coarse_mode CS_COARSE_* adds the coarse level.
+  Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st,
+          int coarse_mode = CS_COARSE_NONE, int coarse_iters = 0, double coarse_omega = 0);
   ~Schwarz();
   // buildSchwarzFactors (schwarz.ts:154-252) at x with the Jacobi diagonal:
   // the FP64 factors in the reference's operation order, or (fp32) the FP32
-  // factors of the mixed-precision solve only.
+  // factors of the mixed-precision solve only.  With a coarse level, also
+  // Ac at x (FP64).
   void factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool fp32);
-  // applySchwarz (schwarz.ts:290-309), FP64, in the reference's operation order.
+  // applySchwarz (schwarz.ts:290-309), FP64, in the reference's operation
+  // order; with a coarse level, then z += P C(R r).
   void apply64(const double* r, double* z, const int32_t* done);
   // The same operator with the FP32 factors (mixed-precision inner solve).
   void apply32(const float* r, float* z, const int32_t* done);
+  const Coarse* coarse() const;
   uint32_t domain_count() const;
   // 1 = Cholesky, 0 = Jacobi fallback, per domain, of the last factor call
   void flags(uint8_t* out) const;

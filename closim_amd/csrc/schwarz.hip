@@ -20,11 +20,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
 #include "elements.h"
 #include "schwarz.h"
+#include "hessian.h"
 
 namespace cloth {
 
@@ -40,6 +42,7 @@ struct SchwarzImpl {
   DevBuf<double> W;      // factorisation workspace when the triangle does not fit in LDS
   DevBuf<uint8_t> flag;  // 1 = Cholesky, 0 = Jacobi fallback
   size_t lds_max = 0;    // dynamic LDS the factor kernel may use
+  std::unique_ptr<Coarse> coarse;  // two-level: the coarse level
 };
 
 // ---------------------------------------------------------------- host
@@ -161,71 +164,6 @@ static void build_domains(const cs_scene_desc& d, int target, int min_size, std:
 }
 
 // ---------------------------------------------------------------- device
-
-// Rows A = ci*3 + ra (ra = 0..2), all columns B, of triangleHessian9
-// (membrane-blocks.ts:21-98), same loop nest and accumulation order.
-__device__ void tri_hessian_rows(const double* x0, const double* x1, const double* x2, const float* inv,
-                                 double area, const DMat& mat, int ci, double* h27) {
-  const double a = inv[0], b = inv[1], c = inv[2], d = inv[3];
-  const double* X[3] = {x0, x1, x2};
-  const double K[3][2] = {{-(a + c), -(b + d)}, {a, b}, {c, d}};
-  double F[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 2; j++)
-      for (int v = 0; v < 3; v++) F[i][j] += K[v][j] * X[v][i];
-  double Cm[2][2] = {{0, 0}, {0, 0}};
-  for (int l = 0; l < 2; l++)
-    for (int m = 0; m < 2; m++)
-      for (int i = 0; i < 3; i++) Cm[l][m] += F[i][l] * F[i][m];
-  const double E00 = 0.5 * (Cm[0][0] - 1), E11 = 0.5 * (Cm[1][1] - 1), E01 = 0.5 * Cm[0][1];
-  const double S00 = mat.c00 * E00 + mat.c01 * E11;
-  const double S11 = mat.c11 * E11 + mat.c01 * E00;
-  const double S01 = 2 * mat.g * E01;
-  const double S[2][2] = {{S00, S01}, {S01, S11}};
-  double D[2][2][2][2];
-  for (int q = 0; q < 16; q++) (&D[0][0][0][0])[q] = 0;
-  D[0][0][0][0] = mat.c00;
-  D[1][1][1][1] = mat.c11;
-  D[0][0][1][1] = mat.c01; D[1][1][0][0] = mat.c01;
-  D[0][1][0][1] = mat.g; D[0][1][1][0] = mat.g; D[1][0][0][1] = mat.g; D[1][0][1][0] = mat.g;
-  const double s = mat.thick * area;
-  // M[A][i][j] = (i == A % 3) ? K[A / 3][j] : 0
-  for (int ra = 0; ra < 3; ra++) {
-    const int A = ci * 3 + ra, av = A / 3, ai = A % 3;
-    for (int B = 0; B < 9; B++) {
-      const int bv = B / 3, bi = B % 3;
-      double sum = 0;
-      for (int i = 0; i < 3; i++) {
-        for (int j = 0; j < 2; j++) {
-          const double mA = (i == ai) ? K[av][j] : 0.0;
-          if (mA == 0) continue;
-          double dP = 0;
-          for (int k = 0; k < 2; k++) {
-            const double mBik = (i == bi) ? K[bv][k] : 0.0;
-            dP += mBik * S[k][j];
-            double dE = 0;
-            for (int l = 0; l < 2; l++) {
-              for (int m2 = 0; m2 < 2; m2++) {
-                const double dd = D[k][j][l][m2];
-                if (dd == 0) continue;
-                double dElm = 0;
-                for (int p = 0; p < 3; p++) {
-                  const double mBpl = (p == bi) ? K[bv][l] : 0.0;
-                  const double mBpm = (p == bi) ? K[bv][m2] : 0.0;
-                  dElm += mBpl * F[p][m2] + F[p][l] * mBpm;
-                }
-                dE += dd * 0.5 * dElm;
-              }
-            }
-            dP += F[i][k] * dE;
-          }
-          sum += mA * dP;
-        }
-      }
-      h27[ra * 9 + B] = s * sum;
-    }
-  }
-}
 
 constexpr int kSwBlock = 128;
 
@@ -826,7 +764,8 @@ __global__ void __launch_bounds__(kWave) k_sw_apply32(SwArgs s, const float* __r
 
 // ---------------------------------------------------------------- facade
 
-Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st)
+Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st,
+                 int coarse_mode, int coarse_iters, double coarse_omega)
     : im_(new SchwarzImpl) {
   (void)max_size;  // growth stops at the target size (schwarz.ts:138)
   SchwarzImpl& S = *im_;
@@ -867,6 +806,8 @@ Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)((size_t)optin - stat)) == hipSuccess)
     S.lds_max = (size_t)optin - stat;
   if (need > S.lds_max) S.W.alloc((size_t)std::max<uint32_t>(S.D, 1) * tri_size(S.DW));
+  if (coarse_mode != CS_COARSE_NONE && S.D)
+    S.coarse.reset(new Coarse(d, mem, S.h_domain_of, coarse_mode, coarse_iters, coarse_omega, st));
 }
 
 Schwarz::~Schwarz() { delete im_; }
@@ -882,6 +823,7 @@ static SwArgs sw_args(SchwarzImpl& S) {
 void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const double* diag, bool fp32) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
+  if (S.coarse) S.coarse->assemble(me, mat, x, diag);
   if (fp32 && !S.L32.ptr) {
     S.L32.alloc((size_t)S.D * tri_size(S.DW));
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
@@ -943,6 +885,7 @@ void Schwarz::apply64(const double* r, double* z, const int32_t* done) {
     default: hipLaunchKernelGGL(k_sw_apply64<8>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
   }
   CS_LAUNCH_CHECK();
+  if (S.coarse) S.coarse->apply64(r, z, done);
 }
 
 void Schwarz::apply32(const float* r, float* z, const int32_t* done) {
@@ -960,6 +903,7 @@ void Schwarz::apply32(const float* r, float* z, const int32_t* done) {
     default: hipLaunchKernelGGL(k_sw_apply32<8>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
   }
   CS_LAUNCH_CHECK();
+  if (S.coarse) S.coarse->apply32(r, z, done);
 }
 
 int schwarz_partition_host(const cs_scene_desc& d, int target, int min_size, uint32_t* domain_of, uint32_t* local_of) {
@@ -972,6 +916,7 @@ int schwarz_partition_host(const cs_scene_desc& d, int target, int min_size, uin
 }
 
 uint32_t Schwarz::domain_count() const { return im_->D; }
+const Coarse* Schwarz::coarse() const { return im_->coarse.get(); }
 uint32_t Schwarz::max_domain() const { return im_->dmax; }
 
 void Schwarz::partition(uint32_t* domain_of, uint32_t* local_of) const {

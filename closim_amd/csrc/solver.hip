@@ -398,7 +398,8 @@ struct Solver {
     if (has_contact) contact.reset(new ContactSystem(*d, st));
     if (opt.precision == CS_PREC_MIXED) mixed.reset(new MixedSolve(*d, opt, st));
     if (opt.precond == CS_PRECOND_SCHWARZ)
-      schwarz.reset(new Schwarz(*d, opt.schwarz_target, opt.schwarz_min, opt.schwarz_max, st));
+      schwarz.reset(new Schwarz(*d, opt.schwarz_target, opt.schwarz_min, opt.schwarz_max, st, opt.coarse,
+                                opt.coarse_iters, opt.coarse_omega));
     CS_HIP(hipStreamSynchronize(st));
   }
 
@@ -1071,6 +1072,9 @@ void cs_options_default(cs_options* o) {
   o->schwarz_max = 32;
   o->device = 0;
   o->flags = 0;
+  o->coarse = CS_COARSE_NONE;
+  o->coarse_iters = 8;
+  o->coarse_omega = 0.5;
 }
 
 int cs_abi_version(void) { return CS_ABI_VERSION; }
@@ -1111,6 +1115,10 @@ int cs_create(const cs_scene_desc* scene, const cs_options* opt, cs_solver** out
     return fail(CS_E_INVALID, "unknown preconditioner");
   if (opt->precond == CS_PRECOND_SCHWARZ && (opt->schwarz_target < 1 || opt->schwarz_min < 0))
     return fail(CS_E_INVALID, "Schwarz domain sizes must be positive");
+  if (opt->coarse < CS_COARSE_NONE || opt->coarse > CS_COARSE_PCG) return fail(CS_E_INVALID, "unknown coarse level");
+  if (opt->coarse != CS_COARSE_NONE && opt->precond != CS_PRECOND_SCHWARZ)
+    return fail(CS_E_INVALID, "a coarse level needs the Schwarz preconditioner");
+  if (opt->coarse_iters < 0 || !(opt->coarse_omega >= 0)) return fail(CS_E_INVALID, "invalid coarse parameters");
   *out = nullptr;
   auto* h = new cs_solver;
   const int rc = guarded([&]() -> int {
@@ -1412,6 +1420,76 @@ int cs_probe_hvp(cs_solver* s, const double* x, const double* p, double inv_h2, 
     CS_HIP(hipMemcpyAsync(S.p.ptr, p, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
     S.mixed->probe_hvp(S.dmesh(), S.dmat(), inv_h2, S.x.ptr, S.p.ptr, S.Ap.ptr);
     CS_HIP(hipMemcpy(out, S.Ap.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return CS_OK;
+  });
+}
+
+int cs_probe_hvp_contact(cs_solver* s, const double* x0, const double* x, const double* p, double inv_h2,
+                         double* out, uint32_t* active) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!s->s.mixed) return fail(CS_E_INVALID, "solver was not created in mixed precision");
+  if (!s->s.contact) return fail(CS_E_INVALID, "scene has no contact system");
+  if (!x0 || !x || !p || !out) return fail(CS_E_INVALID, "null argument");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    S.upload_pins();
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x0.ptr, x0, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.p.ptr, p, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.contact->begin_step(S.x0.ptr);
+    S.contact->update_active_set(S.x.ptr);
+    S.contact->energy_grad(S.x.ptr, S.ceg.ptr);  // the frozen coefficients
+    const cloth::DContactHvp h = S.contact->hvp_hook();
+    S.mixed->probe_hvp(S.dmesh(), S.dmat(), inv_h2, S.x.ptr, S.p.ptr, S.Ap.ptr, &h);
+    CS_HIP(hipMemcpy(out, S.Ap.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (active) *active = S.contact->active_count();
+    return CS_OK;
+  });
+}
+
+int cs_coarse_pattern(cs_solver* s, uint32_t* off, uint32_t* col, uint64_t cap, uint64_t* nnz, uint32_t* domains) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!nnz || !domains) return fail(CS_E_INVALID, "null argument");
+  const cloth::Coarse* c = s->s.schwarz ? s->s.schwarz->coarse() : nullptr;
+  if (!c) return fail(CS_E_INVALID, "solver was not created with a coarse level");
+  *nnz = c->nnz();
+  *domains = c->domains();
+  if (col && cap < c->nnz()) return fail(CS_E_OVERFLOW, "coarse blocks exceed the buffer capacity");
+  c->pattern(off, col);
+  return CS_OK;
+}
+
+int cs_probe_two_level(cs_solver* s, const double* x, const double* diag, const double* r, double* z64, double* z32,
+                       double* raw, double* sym, double* dc) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (!x || !diag || !r) return fail(CS_E_INVALID, "null argument");
+  const cloth::Coarse* c = s->s.schwarz ? s->s.schwarz->coarse() : nullptr;
+  if (!c) return fail(CS_E_INVALID, "solver was not created with a coarse level");
+  return guarded([&]() -> int {
+    cloth::Solver& S = s->s;
+    CS_HIP(hipSetDevice(S.device));
+    const size_t n3 = (size_t)S.n * 3;
+    CS_HIP(hipMemcpyAsync(S.x.ptr, x, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.diag.ptr, diag, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    CS_HIP(hipMemcpyAsync(S.r.ptr, r, n3 * sizeof(double), hipMemcpyHostToDevice, S.st));
+    S.schwarz->factor(S.dmesh(), S.dmat(), S.x.ptr, S.diag.ptr, false);
+    S.schwarz->apply64(S.r.ptr, S.z.ptr, nullptr);
+    CS_HIP(hipStreamSynchronize(S.st));
+    if (z64) CS_HIP(hipMemcpy(z64, S.z.ptr, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    c->values(raw, sym, dc);
+    if (z32) {
+      std::vector<float> h32(r, r + n3), o(n3);
+      cloth::DevBuf<float> r32, o32;
+      r32.upload(h32.data(), n3, S.st);
+      o32.alloc(n3);
+      S.schwarz->factor(S.dmesh(), S.dmat(), S.x.ptr, S.diag.ptr, true);
+      S.schwarz->apply32(r32.ptr, o32.ptr, nullptr);
+      CS_HIP(hipStreamSynchronize(S.st));
+      CS_HIP(hipMemcpy(o.data(), o32.ptr, n3 * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n3; i++) z32[i] = o[i];
+    }
     return CS_OK;
   });
 }

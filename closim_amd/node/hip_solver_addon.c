@@ -235,6 +235,7 @@ static int str_eq(napi_env env, napi_value v, const char* s) {
 
 /* { precision: "fp64" | "mixed", preconditioner: "jacobi" | "block-jacobi" |
  *   "schwarz", schwarz: [target, min, max], hvp: "fd" | "analytic",
+ *   coarse: "none" | "mas2" | "c0" | "c1", coarseIters, coarseOmega,
  *   newtonIters, pcgMaxIters, lineSearchIters, gradTol, flags, device } */
 static int options_of(napi_env env, napi_value o, cs_options* opt) {
   cs_options_default(opt);
@@ -282,6 +283,22 @@ static int options_of(napi_env env, napi_value o, cs_options* opt) {
   }
   if ((r = opt_prop(env, o, "gradTol", &v)) < 0) return -1;
   if (r == 0 && num_of(env, v, "gradTol", &opt->grad_tol)) return -1;
+  if ((r = opt_prop(env, o, "coarse", &v)) < 0) return -1;
+  if (r == 0) {
+    if (str_eq(env, v, "none")) opt->coarse = CS_COARSE_NONE;
+    else if (str_eq(env, v, "mas2")) opt->coarse = CS_COARSE_JACOBI;
+    else if (str_eq(env, v, "c0")) opt->coarse = CS_COARSE_BLOCK;
+    else if (str_eq(env, v, "c1")) opt->coarse = CS_COARSE_PCG;
+    else { throw_type(env, "expected \"none\", \"mas2\", \"c0\" or \"c1\"", "coarse"); return -1; }
+  }
+  if ((r = opt_prop(env, o, "coarseIters", &v)) < 0) return -1;
+  if (r == 0) {
+    double x;
+    if (num_of(env, v, "coarseIters", &x)) return -1;
+    opt->coarse_iters = (int32_t)x;
+  }
+  if ((r = opt_prop(env, o, "coarseOmega", &v)) < 0) return -1;
+  if (r == 0 && num_of(env, v, "coarseOmega", &opt->coarse_omega)) return -1;
   return 0;
 }
 

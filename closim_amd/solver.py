@@ -67,6 +67,10 @@ def load_library(path=LIB_PATH):
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
         "cs_probe_hvp": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp]),
+        "cs_probe_hvp_contact": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, _dp, C.POINTER(C.c_uint32)]),
+        "cs_coarse_pattern": (C.c_int, [C.c_void_p, _up, _up, C.c_uint64, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint32)]),
+        "cs_probe_two_level": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "cs_hvp_chunks": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint32)]),
         "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
@@ -400,6 +404,46 @@ class HipSolver:
         out = np.zeros_like(x)
         _check(self._lib.cs_probe_hvp(self._h, x.ctypes.data_as(_dp), p.ctypes.data_as(_dp), float(inv_h2),
                                       out.ctypes.data_as(_dp)))
+        return out
+
+    def probe_hvp_contact(self, x0, x, p, inv_h2=0.0):
+        """probe_hvp plus the frozen barrier term of the active set built at x
+        for a step begun at x0 (cs_probe_hvp_contact): (out, active count)."""
+        self._need()
+        x0 = np.ascontiguousarray(x0, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        out = np.zeros_like(x)
+        k = C.c_uint32()
+        _check(self._lib.cs_probe_hvp_contact(self._h, x0.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                              p.ctypes.data_as(_dp), float(inv_h2), out.ctypes.data_as(_dp),
+                                              C.byref(k)))
+        return out, k.value
+
+    def coarse_pattern(self):
+        """(block row offsets, block columns) of the coarse level
+        (cs_coarse_pattern)."""
+        self._need()
+        nnz, D = C.c_uint64(), C.c_uint32()
+        _check(self._lib.cs_coarse_pattern(self._h, None, None, 0, C.byref(nnz), C.byref(D)))
+        off = np.zeros(D.value + 1, np.uint32)
+        col = np.zeros(max(nnz.value, 1), np.uint32)
+        _check(self._lib.cs_coarse_pattern(self._h, off.ctypes.data_as(_up), col.ctypes.data_as(_up), nnz.value,
+                                           C.byref(nnz), C.byref(D)))
+        return off, col[:nnz.value]
+
+    def probe_two_level(self, x, diag, r):
+        """The two-level preconditioner at (x, diag) applied to r
+        (cs_probe_two_level): dict with z64, z32, raw, sym (nnz x 9) and dc
+        (3D)."""
+        self._need()
+        off, col = self.coarse_pattern()
+        nnz, D = len(col), len(off) - 1
+        x, diag, r = (np.ascontiguousarray(a, np.float64) for a in (x, diag, r))
+        out = dict(z64=np.zeros_like(r), z32=np.zeros_like(r), raw=np.zeros((nnz, 9)), sym=np.zeros((nnz, 9)),
+                   dc=np.zeros(D * 3))
+        _check(self._lib.cs_probe_two_level(self._h, *(a.ctypes.data_as(_dp) for a in (x, diag, r)),
+                                            *(out[k].ctypes.data_as(_dp) for k in ("z64", "z32", "raw", "sym", "dc"))))
         return out
 
     def probe_active_set(self, x0, x, cap=1 << 20):

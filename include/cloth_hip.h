@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define CS_ABI_VERSION 1
+#define CS_ABI_VERSION 2
 
 /* Opaque handle: one simulated ClothScene resident on one GPU. */
 typedef struct cs_solver cs_solver;
@@ -54,6 +54,15 @@ enum { CS_PREC_FP64 = 0, CS_PREC_MIXED = 1 };
 enum { CS_HVP_FD = 0, CS_HVP_ANALYTIC = 1 };
 /* pcg() preconditioner (src/math/pcg.ts:11-95 `applyPreconditioner`). */
 enum { CS_PRECOND_JACOBI = 0, CS_PRECOND_BLOCK_JACOBI = 1, CS_PRECOND_SCHWARZ = 2 };
+/* Coarse level added to the Schwarz preconditioner (two-level additive
+ * Schwarz, src/solver/mas.ts, src/solver/coarse-csr.ts; the reference's GPU
+ * path selects them with useMas / useCoarseC0 / useCoarsePcg,
+ * src/backend/webgpu/gpu-newton.ts:213-242): z = Schwarz1(r) + P C(R r) with
+ * one coarse dof per domain and axis and C the damped Jacobi omega Dc^-1
+ * (applyMasTwoLevel), the diagonal-block inverse (C0) or coarse_iters
+ * iterations of block-Jacobi PCG on the assembled Ac (C1).  Needs
+ * CS_PRECOND_SCHWARZ. */
+enum { CS_COARSE_NONE = 0, CS_COARSE_JACOBI = 1, CS_COARSE_BLOCK = 2, CS_COARSE_PCG = 3 };
 
 #define CS_FLAG_TIMING 1u   /* record per-kernel HIP event timings (cs_kernel_timing) */
 /* Root polish (not in the reference): when every Armijo trial of a Newton
@@ -153,6 +162,9 @@ typedef struct cs_options {
   int32_t schwarz_max;
   int32_t device;             /* HIP device ordinal */
   uint32_t flags;             /* CS_FLAG_* */
+  int32_t coarse;             /* CS_COARSE_* (default none = one-level Schwarz) */
+  int32_t coarse_iters;       /* CS_COARSE_PCG iterations (0: 8, the reference's c1-8) */
+  double coarse_omega;        /* CS_COARSE_JACOBI damping (0: 0.5, masOmega) */
 } cs_options;
 
 /* ContactDiagnostics (src/collision/types.ts:49-61). */
@@ -325,7 +337,26 @@ int cs_hvp_chunks(const cs_scene_desc* scene, uint32_t* records, uint64_t cap, u
  * out = M p inv_h2 + H_membrane(x) p in FP32, returned as FP64.  Mixed
  * precision solvers only. */
 int cs_probe_hvp(cs_solver* s, const double* x, const double* p, double inv_h2, double* out);
+/* The same launch with the frozen barrier term (applyHvp,
+ * contact-assembly.ts:371-392) of the active set built at x for a step begun
+ * at x0 (updateActiveSet, :164-220), its coefficients and Jacobians rounded
+ * to FP32 as the mixed solve applies them; *active (optional) receives the
+ * active contact count.  Mixed precision solvers with a contact system. */
+int cs_probe_hvp_contact(cs_solver* s, const double* x0, const double* x, const double* p, double inv_h2,
+                         double* out, uint32_t* active);
 
+/* The coarse level's block pattern (buildCoarsePattern, coarse-csr.ts:30-50):
+ * block rows `off` (domains + 1) and up to `cap` block columns; *nnz and
+ * *domains receive the sizes.  Solvers created with a coarse level only. */
+int cs_coarse_pattern(cs_solver* s, uint32_t* off, uint32_t* col, uint64_t cap, uint64_t* nnz, uint32_t* domains);
+/* Component probe of the two-level preconditioner at (x, diag): the FP64
+ * factors and Ac, then z64 = Schwarz1(r) + P C(R r) in FP64 and z32 the same
+ * with the mixed solve's FP32 factors (r rounded to FP32); raw / sym receive
+ * the raw and symmetrised coarse blocks (assembleCoarseRaw,
+ * assembleCoarseValues, 9 per block), dc the coarse diagonal (buildCoarseDiag,
+ * mas.ts:68-101).  Outputs may be null. */
+int cs_probe_two_level(cs_solver* s, const double* x, const double* diag, const double* r, double* z64, double* z32,
+                       double* raw, double* sym, double* dc);
 /* buildSchwarzFactors + applySchwarz (src/solver/schwarz.ts:154-309) with the
  * solver's domains (created with CS_PRECOND_SCHWARZ) at positions x and
  * Jacobi diagonal diag: z64 = the FP64 apply in the reference's order, z32 =

@@ -37,6 +37,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -747,6 +748,356 @@ void apply_schwarz(const SchwarzFactors& f, const SchwarzDomains& dom, const dou
     for (size_t l = 0; l < dv; l++)
       for (int a = 0; a < 3; a++) out[(size_t)mem[l] * 3 + a] += sol[l * 3 + a];
   }
+}
+
+// ------------------------------------------- two-level Schwarz (coarse level)
+// Coarse space: one constant vector per Schwarz domain and axis, normalised
+// restriction R[(d*3+a)][v*3+a] = 1/sqrt(|d|) for v in d, P = R^T
+// (mas.ts:1-13, coarse-csr.ts:1-13).
+
+// restrictCoarse() mas.ts:23-40
+void restrict_coarse(const double* r, const SchwarzDomains& dom, double* rc) {
+  const size_t D = dom.members.size();
+  std::fill(rc, rc + D * 3, 0.0);
+  for (size_t d = 0; d < D; d++) {
+    const double s = 1 / std::sqrt((double)dom.members[d].size());
+    for (uint32_t v : dom.members[d]) {
+      rc[d * 3] += r[(size_t)v * 3] * s;
+      rc[d * 3 + 1] += r[(size_t)v * 3 + 1] * s;
+      rc[d * 3 + 2] += r[(size_t)v * 3 + 2] * s;
+    }
+  }
+}
+
+// prolongateAdd() mas.ts:43-58
+void prolongate_add(const double* c, const SchwarzDomains& dom, double* z) {
+  for (size_t d = 0; d < dom.members.size(); d++) {
+    const double s = 1 / std::sqrt((double)dom.members[d].size());
+    for (uint32_t v : dom.members[d]) {
+      z[(size_t)v * 3] += c[d * 3] * s;
+      z[(size_t)v * 3 + 1] += c[d * 3 + 1] * s;
+      z[(size_t)v * 3 + 2] += c[d * 3 + 2] * s;
+    }
+  }
+}
+
+// buildCoarseDiag() mas.ts:68-101: diag(R A P), intra-domain corner pairs of
+// the membrane Hessian plus the Jacobi diagonal spread.
+std::vector<double> build_coarse_diag(const Mesh& mesh, const Mat& mat, const double* x, const double* diag,
+                                      const SchwarzDomains& dom) {
+  const size_t D = dom.members.size();
+  std::vector<double> dc(D * 3, 0.0);
+  double h[81];
+  for (size_t t = 0; t < mesh.m; t++) {
+    const uint32_t cvs[3] = {mesh.idx[t * 3], mesh.idx[t * 3 + 1], mesh.idx[t * 3 + 2]};
+    double x0[3], x1[3], x2[3], inv[4];
+    load3(x, cvs[0], x0); load3(x, cvs[1], x1); load3(x, cvs[2], x2); inv4(mesh, t, inv);
+    tri_hessian9(x0, x1, x2, inv, mesh.areas[t], mat, h);
+    for (int ci = 0; ci < 3; ci++)
+      for (int cj = 0; cj < 3; cj++) {
+        const uint32_t d1 = dom.domainOf[cvs[ci]], d2 = dom.domainOf[cvs[cj]];
+        if (d1 != d2) continue;
+        const double sc = 1.0 / (double)dom.members[d1].size();
+        for (int a = 0; a < 3; a++) dc[d1 * 3 + a] += sc * h[(ci * 3 + a) * 9 + (cj * 3 + a)];
+      }
+  }
+  for (size_t d = 0; d < D; d++) {
+    const double sc = 1.0 / (double)dom.members[d].size();
+    for (uint32_t v : dom.members[d])
+      for (int a = 0; a < 3; a++) dc[d * 3 + a] += sc * diag[(size_t)v * 3 + a];
+  }
+  return dc;
+}
+
+struct CoarsePattern {
+  std::vector<uint32_t> off, col;  // block rows (D + 1), block columns sorted per row
+};
+
+// buildCoarsePattern() coarse-csr.ts:30-50: one block per pair of domains
+// sharing a triangle.
+CoarsePattern build_coarse_pattern(const Mesh& mesh, const SchwarzDomains& dom) {
+  const size_t D = dom.members.size();
+  std::vector<std::vector<uint32_t>> rows(D);
+  for (size_t t = 0; t < mesh.m; t++) {
+    const uint32_t ds[3] = {dom.domainOf[mesh.idx[t * 3]], dom.domainOf[mesh.idx[t * 3 + 1]],
+                            dom.domainOf[mesh.idx[t * 3 + 2]]};
+    for (uint32_t a : ds)
+      for (uint32_t b : ds) rows[a].push_back(b);
+  }
+  CoarsePattern p;
+  p.off.assign(D + 1, 0);
+  for (size_t d = 0; d < D; d++) {
+    auto& r = rows[d];
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    p.off[d + 1] = p.off[d] + (uint32_t)r.size();
+    p.col.insert(p.col.end(), r.begin(), r.end());
+  }
+  return p;
+}
+
+// blockPos() coarse-csr.ts:53-62
+int64_t block_pos(const CoarsePattern& p, uint32_t d1, uint32_t d2) {
+  for (uint32_t q = p.off[d1]; q < p.off[d1 + 1]; q++) {
+    if (p.col[q] == d2) return q;
+    if (p.col[q] > d2) break;
+  }
+  return -1;
+}
+
+// assembleCoarseRaw() coarse-csr.ts:70-110: membrane corner-pair blocks scaled
+// by 1/sqrt(|d1||d2|), then the Jacobi-diagonal spread on the diagonal blocks.
+std::vector<double> assemble_coarse_raw(const Mesh& mesh, const Mat& mat, const double* x, const double* diag,
+                                        const SchwarzDomains& dom, const CoarsePattern& pat) {
+  std::vector<double> blocks(pat.col.size() * 9, 0.0);
+  double h[81];
+  for (size_t t = 0; t < mesh.m; t++) {
+    const uint32_t cvs[3] = {mesh.idx[t * 3], mesh.idx[t * 3 + 1], mesh.idx[t * 3 + 2]};
+    double x0[3], x1[3], x2[3], inv[4];
+    load3(x, cvs[0], x0); load3(x, cvs[1], x1); load3(x, cvs[2], x2); inv4(mesh, t, inv);
+    tri_hessian9(x0, x1, x2, inv, mesh.areas[t], mat, h);
+    for (int ci = 0; ci < 3; ci++)
+      for (int cj = 0; cj < 3; cj++) {
+        const uint32_t d1 = dom.domainOf[cvs[ci]], d2 = dom.domainOf[cvs[cj]];
+        const int64_t q = block_pos(pat, d1, d2);
+        if (q < 0) throw std::runtime_error("coarse pattern misses a block");
+        const double sc = 1 / std::sqrt((double)dom.members[d1].size() * (double)dom.members[d2].size());
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) blocks[q * 9 + a * 3 + b] += sc * h[(ci * 3 + a) * 9 + (cj * 3 + b)];
+      }
+  }
+  for (size_t d = 0; d < dom.members.size(); d++) {
+    const int64_t q = block_pos(pat, (uint32_t)d, (uint32_t)d);
+    const double sc = 1.0 / (double)dom.members[d].size();
+    for (uint32_t v : dom.members[d])
+      for (int a = 0; a < 3; a++) blocks[q * 9 + a * 3 + a] += sc * diag[(size_t)v * 3 + a];
+  }
+  return blocks;
+}
+
+// assembleCoarseValues() coarse-csr.ts:117-138: B(d1,d2) = 0.5 (B + B(d2,d1)^T).
+std::vector<double> symmetrize_coarse(const CoarsePattern& pat, const std::vector<double>& raw) {
+  std::vector<double> out(raw);
+  const size_t D = pat.off.size() - 1;
+  for (uint32_t d1 = 0; d1 < D; d1++)
+    for (uint32_t q = pat.off[d1]; q < pat.off[d1 + 1]; q++) {
+      const int64_t w = block_pos(pat, pat.col[q], d1);
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) out[q * 9 + a * 3 + b] = 0.5 * (raw[q * 9 + a * 3 + b] + raw[w * 9 + b * 3 + a]);
+    }
+  return out;
+}
+
+// coarseSpmv() coarse-csr.ts:141-157
+void coarse_spmv(const CoarsePattern& pat, const std::vector<double>& B, const double* x, double* out) {
+  const size_t D = pat.off.size() - 1;
+  std::fill(out, out + D * 3, 0.0);
+  for (uint32_t d1 = 0; d1 < D; d1++)
+    for (uint32_t q = pat.off[d1]; q < pat.off[d1 + 1]; q++) {
+      const uint32_t d2 = pat.col[q];
+      for (int a = 0; a < 3; a++) {
+        double s = 0;
+        for (int b = 0; b < 3; b++) s += B[q * 9 + a * 3 + b] * x[d2 * 3 + b];
+        out[d1 * 3 + a] += s;
+      }
+    }
+}
+
+// invertSym33() coarse-csr.ts:170-205 (the block-Jacobi thresholds; diagonal
+// fallback 1/fb).  Returns whether the block was positive definite.
+bool invert_sym33(double B00, double B01, double B02, double B11, double B12, double B22, double fb0, double fb1,
+                  double fb2, double* inv) {
+  const double tr = std::fabs(B00) + std::fabs(B11) + std::fabs(B22);
+  const double tiny = 1e-20 * (1 + tr);
+  bool ok = true;
+  if (!(B00 > tiny)) ok = false;
+  if (ok && !(B00 * B11 - B01 * B01 > tiny * (1 + std::fabs(B00) + std::fabs(B11)))) ok = false;
+  const double det = B00 * (B11 * B22 - B12 * B12) - B01 * (B01 * B22 - B12 * B02) + B02 * (B01 * B12 - B11 * B02);
+  if (ok && !(det > tiny * (1 + tr) * (1 + tr))) ok = false;
+  std::fill(inv, inv + 9, 0.0);
+  if (ok) {
+    const double s = 1 / det;
+    inv[0] = (B11 * B22 - B12 * B12) * s;
+    inv[1] = (B02 * B12 - B01 * B22) * s;
+    inv[2] = (B01 * B12 - B11 * B02) * s;
+    inv[3] = inv[1];
+    inv[4] = (B00 * B22 - B02 * B02) * s;
+    inv[5] = (B01 * B02 - B00 * B12) * s;
+    inv[6] = inv[2];
+    inv[7] = inv[5];
+    inv[8] = (B00 * B11 - B01 * B01) * s;
+    return true;
+  }
+  inv[0] = fb0 > 1e-12 ? 1 / fb0 : 0;
+  inv[4] = fb1 > 1e-12 ? 1 / fb1 : 0;
+  inv[8] = fb2 > 1e-12 ? 1 / fb2 : 0;
+  return false;
+}
+
+// The diagonal-block inverse of domain d as applyCoarseBlockJacobi and
+// coarsePcg read it (coarse-csr.ts:218-224, 257-263).
+void coarse_diag_inverse(const CoarsePattern& pat, const std::vector<double>& B, const std::vector<double>& dc,
+                         uint32_t d, double* inv) {
+  const int64_t q = block_pos(pat, d, d);
+  const double* b = &B[q * 9];
+  invert_sym33(b[0], 0.5 * (b[1] + b[3]), 0.5 * (b[2] + b[6]), b[4], 0.5 * (b[5] + b[7]), b[8], dc[d * 3],
+               dc[d * 3 + 1], dc[d * 3 + 2], inv);
+}
+
+void coarse_block_jacobi(const CoarsePattern& pat, const std::vector<double>& B, const std::vector<double>& dc,
+                         const double* r, double* out) {
+  const size_t D = pat.off.size() - 1;
+  double inv[9];
+  for (uint32_t d = 0; d < D; d++) {
+    coarse_diag_inverse(pat, B, dc, d, inv);
+    for (int a = 0; a < 3; a++)
+      out[d * 3 + a] = inv[a * 3] * r[d * 3] + inv[a * 3 + 1] * r[d * 3 + 1] + inv[a * 3 + 2] * r[d * 3 + 2];
+  }
+}
+
+// coarsePcg() coarse-csr.ts:243-303: fixed-K block-Jacobi PCG on Ac from x = 0
+// (relative tolerance, pAp breakdown latch); sums left to right as b.reduce.
+int coarse_pcg(const CoarsePattern& pat, const std::vector<double>& B, const std::vector<double>& dc,
+               const double* b, int maxIters, double tol, double* x, double* residual_out) {
+  const size_t n = (pat.off.size() - 1) * 3;
+  std::vector<double> r(b, b + n), z(n), p(n), Ap(n);
+  std::fill(x, x + n, 0.0);
+  coarse_block_jacobi(pat, B, dc, r.data(), z.data());
+  p = z;
+  double rz = 0;
+  for (size_t i = 0; i < n; i++) rz += r[i] * z[i];
+  double bb = 0;
+  for (size_t i = 0; i < n; i++) bb += b[i] * b[i];
+  const double absTol = jmax(tol * jmax(std::sqrt(bb), 1e-12), 1e-12);
+  auto norm = [&]() {
+    double s = 0;
+    for (size_t i = 0; i < n; i++) s += r[i] * r[i];
+    return std::sqrt(s);
+  };
+  double residual = norm();
+  if (residual_out) *residual_out = residual;
+  if (residual < absTol) return 0;
+  for (int k = 0; k < maxIters; k++) {
+    coarse_spmv(pat, B, p.data(), Ap.data());
+    double pAp = 0;
+    for (size_t i = 0; i < n; i++) pAp += p[i] * Ap[i];
+    if (!(pAp > 1e-30)) return k;
+    const double alpha = rz / pAp;
+    for (size_t i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+    }
+    residual = norm();
+    if (residual_out) *residual_out = residual;
+    if (residual < absTol) return k + 1;
+    coarse_block_jacobi(pat, B, dc, r.data(), z.data());
+    double rzNew = 0;
+    for (size_t i = 0; i < n; i++) rzNew += r[i] * z[i];
+    const double beta = rzNew / rz;
+    for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    rz = rzNew;
+  }
+  return maxIters;
+}
+
+// buildCoarseExact() mas.ts:155-226: dense Ac = R A P, symmetrised, and its
+// Cholesky factor (flag 1) or the Jacobi fallback (flag 0).
+int build_coarse_exact(const Mesh& mesh, const Mat& mat, const double* x, const double* diag,
+                       const SchwarzDomains& dom, std::vector<double>& ac, std::vector<double>& chol) {
+  const size_t D = dom.members.size(), dc = D * 3;
+  ac.assign(dc * dc, 0.0);
+  double h[81];
+  for (size_t t = 0; t < mesh.m; t++) {
+    const uint32_t cvs[3] = {mesh.idx[t * 3], mesh.idx[t * 3 + 1], mesh.idx[t * 3 + 2]};
+    double x0[3], x1[3], x2[3], inv[4];
+    load3(x, cvs[0], x0); load3(x, cvs[1], x1); load3(x, cvs[2], x2); inv4(mesh, t, inv);
+    tri_hessian9(x0, x1, x2, inv, mesh.areas[t], mat, h);
+    for (int ci = 0; ci < 3; ci++)
+      for (int cj = 0; cj < 3; cj++) {
+        const uint32_t d1 = dom.domainOf[cvs[ci]], d2 = dom.domainOf[cvs[cj]];
+        const double sc = 1 / std::sqrt((double)dom.members[d1].size() * (double)dom.members[d2].size());
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) ac[(d1 * 3 + a) * dc + (d2 * 3 + b)] += sc * h[(ci * 3 + a) * 9 + (cj * 3 + b)];
+      }
+  }
+  for (size_t d = 0; d < D; d++) {
+    const double sc = 1.0 / (double)dom.members[d].size();
+    for (uint32_t v : dom.members[d])
+      for (int a = 0; a < 3; a++) ac[(d * 3 + a) * dc + (d * 3 + a)] += sc * diag[(size_t)v * 3 + a];
+  }
+  for (size_t i = 0; i < dc; i++)
+    for (size_t j = i + 1; j < dc; j++) {
+      const double sv = 0.5 * (ac[i * dc + j] + ac[j * dc + i]);
+      ac[i * dc + j] = sv; ac[j * dc + i] = sv;
+    }
+  chol = ac;
+  double scale = 0;
+  for (size_t i = 0; i < dc; i++) scale = jmax(scale, std::fabs(chol[i * dc + i]));
+  const double tiny = 1e-20 * (1 + scale);
+  bool ok = true;
+  for (size_t k = 0; k < dc && ok; k++) {
+    double sum = chol[k * dc + k];
+    for (size_t s2 = 0; s2 < k; s2++) sum -= chol[k * dc + s2] * chol[k * dc + s2];
+    if (!(sum > tiny)) { ok = false; break; }
+    chol[k * dc + k] = std::sqrt(sum);
+    for (size_t i = k + 1; i < dc; i++) {
+      double v = chol[i * dc + k];
+      for (size_t s2 = 0; s2 < k; s2++) v -= chol[i * dc + s2] * chol[k * dc + s2];
+      chol[i * dc + k] = v / chol[k * dc + k];
+    }
+  }
+  if (ok) {
+    for (size_t i = 0; i < dc; i++)
+      for (size_t j = i + 1; j < dc; j++) chol[i * dc + j] = 0;
+    return 1;
+  }
+  chol.assign(dc * dc, 0.0);
+  for (size_t i = 0; i < dc; i++) chol[i * dc + i] = ac[i * dc + i] > 1e-12 ? 1 / ac[i * dc + i] : 0;
+  return 0;
+}
+
+// solveCoarseExact() mas.ts:229-250
+void solve_coarse_exact(const std::vector<double>& chol, int flag, size_t dc, const double* rhs, double* out) {
+  if (flag == 1) {
+    std::vector<double> y(dc);
+    for (size_t i = 0; i < dc; i++) {
+      double s = rhs[i];
+      for (size_t j = 0; j < i; j++) s -= chol[i * dc + j] * y[j];
+      y[i] = s / chol[i * dc + i];
+    }
+    for (size_t i = dc; i-- > 0;) {
+      double s = y[i];
+      for (size_t j = i + 1; j < dc; j++) s -= chol[j * dc + i] * out[j];
+      out[i] = s / chol[i * dc + i];
+    }
+  } else {
+    for (size_t i = 0; i < dc; i++) {
+      double s = 0;
+      for (size_t j = 0; j < dc; j++) s += chol[i * dc + j] * rhs[j];
+      out[i] = s;
+    }
+  }
+}
+
+// Two-level apply z = Schwarz1(r) + P C(R r) (applyMasTwoLevel mas.ts:126-143
+// and the GPU path's C0 / C1 corrections, gpu-newton.ts:1493-1524):
+// mode 1 damped Jacobi C = omega Dc^-1, mode 2 block-Jacobi C = Bdd^-1,
+// mode 3 coarse PCG of `iters` iterations (tolerance 1e-3).
+void apply_two_level(const SchwarzFactors& f, const SchwarzDomains& dom, const CoarsePattern& pat,
+                     const std::vector<double>& B, const std::vector<double>& dc, int mode, int iters, double omega,
+                     const double* r, double* z, size_t n) {
+  apply_schwarz(f, dom, r, z, n);
+  const size_t D = dom.members.size();
+  std::vector<double> rc(D * 3), c(D * 3, 0.0);
+  restrict_coarse(r, dom, rc.data());
+  if (mode == 1) {
+    for (size_t i = 0; i < D * 3; i++) c[i] = dc[i] > 1e-12 ? (omega * rc[i]) / dc[i] : 0;
+  } else if (mode == 2) {
+    coarse_block_jacobi(pat, B, dc, rc.data(), c.data());
+  } else if (mode == 3) {
+    coarse_pcg(pat, B, dc, rc.data(), iters, 1e-3, c.data(), nullptr);
+  }
+  prolongate_add(c.data(), dom, z);
 }
 
 // ------------------------------------------------------------ collision
@@ -2371,6 +2722,18 @@ void orc_membrane_blocks(orc_solver* o, const double* x, double* blocks) {
   membrane_blocks(o->s.mesh, o->s.mat, x, blocks);
 }
 
+// applyHvp (contact-assembly.ts:371-392) of the active set updateActiveSet
+// builds at x after beginStep(x0); returns the active contact count.
+int64_t orc_contact_hvp(orc_solver* o, const double* x0, const double* x, const double* p, double* out) {
+  Solver& s = o->s;
+  if (!s.hasContact) return -1;
+  const size_t n3 = (size_t)s.mesh.n * 3;
+  s.contact.begin_step(std::vector<double>(x0, x0 + n3));
+  s.contact.update_active_set(x, s.mesh.n);
+  s.contact.apply_hvp(x, p, out, n3);
+  return (int64_t)s.contact.active.size();
+}
+
 // buildCandidates (contact-assembly.ts:87-121) at x with the given pad after beginStep(x0).
 int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap) {
   Solver& s = o->s;
@@ -2438,6 +2801,100 @@ int orc_schwarz_apply(orc_solver* o, int target, int minsz, int maxsz, const dou
   const SchwarzFactors f = build_schwarz_factors(s.mesh, s.mat, x, diag, d, v2t);
   apply_schwarz(f, d, r, z, s.mesh.n);
   std::copy(f.flag.begin(), f.flag.end(), flags);
+  return (int)d.members.size();
+}
+
+// ---- two-level Schwarz probes (mas.ts, coarse-csr.ts) ----
+namespace {
+SchwarzDomains probe_domains(orc_solver* o, int target, int minsz, int maxsz) {
+  return build_schwarz_domains(o->s.mesh, o->s.mesh.restPos.data(), target, minsz, maxsz);
+}
+}  // namespace
+
+// buildCoarsePattern: off (D + 1) and up to cap block columns; returns nnz.
+int64_t orc_coarse_pattern(orc_solver* o, int target, int minsz, int maxsz, uint32_t* off, uint32_t* col,
+                           int64_t cap) {
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  const CoarsePattern p = build_coarse_pattern(o->s.mesh, d);
+  if (off) std::copy(p.off.begin(), p.off.end(), off);
+  if (col)
+    for (int64_t q = 0; q < (int64_t)p.col.size() && q < cap; q++) col[q] = p.col[q];
+  return (int64_t)p.col.size();
+}
+
+// assembleCoarseRaw / assembleCoarseValues (9 nnz each, either may be null)
+// and buildCoarseDiag (3D); returns D.
+int orc_coarse_values(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                      double* raw, double* sym, double* dc) {
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  const CoarsePattern p = build_coarse_pattern(o->s.mesh, d);
+  const std::vector<double> r = assemble_coarse_raw(o->s.mesh, o->s.mat, x, diag, d, p);
+  if (raw) std::copy(r.begin(), r.end(), raw);
+  if (sym) {
+    const std::vector<double> v = symmetrize_coarse(p, r);
+    std::copy(v.begin(), v.end(), sym);
+  }
+  if (dc) {
+    const std::vector<double> c = build_coarse_diag(o->s.mesh, o->s.mat, x, diag, d);
+    std::copy(c.begin(), c.end(), dc);
+  }
+  return (int)d.members.size();
+}
+
+// dir 0: out (3D) = R in (3n); dir 1: out (3n) += P in (3D).  Returns D.
+int orc_coarse_transfer(orc_solver* o, int target, int minsz, int maxsz, int dir, const double* in, double* out) {
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  if (dir == 0) restrict_coarse(in, d, out);
+  else prolongate_add(in, d, out);
+  return (int)d.members.size();
+}
+
+// buildCoarseExact: dense (3D)^2 Ac and its Cholesky factor (or the Jacobi
+// fallback); returns the flag.
+int orc_coarse_exact(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                     double* ac, double* chol) {
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  std::vector<double> a, c;
+  const int flag = build_coarse_exact(o->s.mesh, o->s.mat, x, diag, d, a, c);
+  if (ac) std::copy(a.begin(), a.end(), ac);
+  if (chol) std::copy(c.begin(), c.end(), chol);
+  return flag;
+}
+
+// Coarse solves of rhs (3D): mode 2 applyCoarseBlockJacobi, 3 coarsePcg of
+// `iters` iterations at tolerance `tol` (returns its iteration count), 4 the
+// exact Cholesky solve (returns the flag).
+int orc_coarse_solve(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag, int mode,
+                     int iters, double tol, const double* rhs, double* out, double* residual) {
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  const CoarsePattern p = build_coarse_pattern(o->s.mesh, d);
+  const std::vector<double> B = symmetrize_coarse(p, assemble_coarse_raw(o->s.mesh, o->s.mat, x, diag, d, p));
+  const std::vector<double> dc = build_coarse_diag(o->s.mesh, o->s.mat, x, diag, d);
+  if (mode == 2) {
+    coarse_block_jacobi(p, B, dc, rhs, out);
+    return 0;
+  }
+  if (mode == 3) return coarse_pcg(p, B, dc, rhs, iters, tol, out, residual);
+  std::vector<double> a, c;
+  const int flag = build_coarse_exact(o->s.mesh, o->s.mat, x, diag, d, a, c);
+  solve_coarse_exact(c, flag, d.members.size() * 3, rhs, out);
+  return flag;
+}
+
+// z = Schwarz1(r) + P C(R r) with C by `mode` (1 damped Jacobi, 2 block
+// Jacobi, 3 coarse PCG of `iters` iterations); returns D.
+int orc_two_level_apply(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                        int mode, int iters, double omega, const double* r, double* z) {
+  Solver& s = o->s;
+  const SchwarzDomains d = probe_domains(o, target, minsz, maxsz);
+  std::vector<std::vector<uint32_t>> v2t(s.mesh.n);
+  for (uint32_t t = 0; t < s.mesh.m; t++)
+    for (int k = 0; k < 3; k++) v2t[s.mesh.idx[t * 3 + k]].push_back(t);
+  const SchwarzFactors f = build_schwarz_factors(s.mesh, s.mat, x, diag, d, v2t);
+  const CoarsePattern p = build_coarse_pattern(s.mesh, d);
+  const std::vector<double> B = symmetrize_coarse(p, assemble_coarse_raw(s.mesh, s.mat, x, diag, d, p));
+  const std::vector<double> dc = build_coarse_diag(s.mesh, s.mat, x, diag, d);
+  apply_two_level(f, d, p, B, dc, mode, iters, omega, r, z, s.mesh.n);
   return (int)d.members.size();
 }
 

@@ -78,6 +78,18 @@ const char* orc_last_error(void);
 /* Contact-system probes for set-equality tests: begin a step at x0 and build the
  * frozen active set at x.  Returns the active count; records are written as
  * 6 ints (kind,p,a,b,c,d) + 7 doubles (w0..w3,nx,ny,nz) per contact. */
+int64_t orc_coarse_pattern(orc_solver* o, int target, int minsz, int maxsz, uint32_t* off, uint32_t* col,
+                           int64_t cap);
+int orc_coarse_values(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                      double* raw, double* sym, double* dc);
+int orc_coarse_transfer(orc_solver* o, int target, int minsz, int maxsz, int dir, const double* in, double* out);
+int orc_coarse_exact(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                     double* ac, double* chol);
+int orc_coarse_solve(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag, int mode,
+                     int iters, double tol, const double* rhs, double* out, double* residual);
+int orc_two_level_apply(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
+                        int mode, int iters, double omega, const double* r, double* z);
+int64_t orc_contact_hvp(orc_solver* o, const double* x0, const double* x, const double* p, double* out);
 int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap);
 int64_t orc_contact_probe(orc_solver* s, const double* x0, const double* x, double pad_is_dmin,
                           int32_t* ids6, double* vals7, int64_t cap, int64_t* n_candidates);

@@ -72,6 +72,15 @@ def lib():
                                               C.POINTER(C.c_int64)]),
             "orc_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
             "orc_candidates": (C.c_int64, [C.c_void_p, _dp, _dp, C.c_int, _ip, C.c_int64]),
+            "orc_contact_hvp": (C.c_int64, [C.c_void_p, _dp, _dp, _dp, _dp]),
+            "orc_coarse_pattern": (C.c_int64, [C.c_void_p, C.c_int, C.c_int, C.c_int, _up, _up, C.c_int64]),
+            "orc_coarse_values": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
+            "orc_coarse_transfer": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+            "orc_coarse_exact": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+            "orc_coarse_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                           C.c_double, _dp, _dp, _dp]),
+            "orc_two_level_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                              C.c_double, _dp, _dp]),
             "orc_schwarz_domains": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _up, _up]),
             "orc_schwarz_apply": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                             C.POINTER(C.c_uint8)]),
@@ -216,6 +225,61 @@ class OracleSolver:
         if k > cap:
             return self.active_set(x0, x, cap=int(k), candidates_dmin=candidates_dmin)
         return ids[:k].copy(), vals[:k].copy(), nc.value
+
+    def contact_hvp(self, x0, x, p):
+        """(applyHvp of the active set built at x after beginStep(x0), active
+        contact count)."""
+        out = np.zeros(self.n3)
+        k = lib().orc_contact_hvp(self.h, P(d3(x0)), P(d3(x)), P(d3(p)), P(out))
+        if k < 0:
+            raise RuntimeError("scene has no contact system")
+        return out, int(k)
+
+    # ---- two-level Schwarz (mas.ts, coarse-csr.ts); dom = (target, min, max)
+    def coarse_pattern(self, dom):
+        D = self.schwarz_domains(*dom)[2]
+        off = np.zeros(D + 1, np.uint32)
+        nnz = lib().orc_coarse_pattern(self.h, *dom, P(off, C.c_uint32), None, 0)
+        col = np.zeros(max(nnz, 1), np.uint32)
+        lib().orc_coarse_pattern(self.h, *dom, P(off, C.c_uint32), P(col, C.c_uint32), nnz)
+        return off, col[:nnz]
+
+    def coarse_values(self, dom, x, diag):
+        """(raw blocks, symmetrised blocks (nnz x 9), buildCoarseDiag (3D))."""
+        off, col = self.coarse_pattern(dom)
+        raw, sym = np.zeros((len(col), 9)), np.zeros((len(col), 9))
+        dc = np.zeros((len(off) - 1) * 3)
+        lib().orc_coarse_values(self.h, *dom, P(d3(x)), P(d3(diag)), P(raw), P(sym), P(dc))
+        return raw, sym, dc
+
+    def coarse_restrict(self, dom, r, D):
+        out = np.zeros(D * 3)
+        lib().orc_coarse_transfer(self.h, *dom, 0, P(d3(r)), P(out))
+        return out
+
+    def coarse_prolongate_add(self, dom, c, z):
+        z = d3(z).copy()
+        lib().orc_coarse_transfer(self.h, *dom, 1, P(d3(c)), P(z))
+        return z
+
+    def coarse_exact(self, dom, x, diag, D):
+        """(dense Ac (3D x 3D), its Cholesky factor or Jacobi fallback, flag)."""
+        ac, ch = np.zeros((D * 3, D * 3)), np.zeros((D * 3, D * 3))
+        flag = lib().orc_coarse_exact(self.h, *dom, P(d3(x)), P(d3(diag)), P(ac), P(ch))
+        return ac, ch, flag
+
+    def coarse_solve(self, dom, x, diag, mode, rhs, iters=8, tol=1e-3):
+        """mode 2 block Jacobi, 3 coarsePcg (iters, tol), 4 exact: (out, returned count or flag, residual)."""
+        out = np.zeros(len(rhs))
+        res = C.c_double(np.nan)
+        k = lib().orc_coarse_solve(self.h, *dom, P(d3(x)), P(d3(diag)), mode, iters, tol, P(d3(rhs)), P(out),
+                                   C.byref(res))
+        return out, k, res.value
+
+    def two_level_apply(self, dom, x, diag, r, mode, iters=8, omega=0.5):
+        z = np.zeros(self.n3)
+        lib().orc_two_level_apply(self.h, *dom, P(d3(x)), P(d3(diag)), mode, iters, omega, P(d3(r)), P(z))
+        return z
 
     def candidates(self, x0, x, dmin=False, cap=None):
         """buildCandidates (contact-assembly.ts:87-121) at x after beginStep(x0),

@@ -12,7 +12,7 @@ const { HipSolver, packScene, addon } = require(path.join(__dirname, "..", "..",
 const scene = readScene(process.argv[2]);
 const expected = JSON.parse(fs.readFileSync(process.argv[3], "utf8"));
 
-assert.strictEqual(addon.abiVersion(), 1);
+assert.strictEqual(addon.abiVersion(), 2);
 
 // the contract before initialize (cpu-solver.ts:15-43)
 const s = new HipSolver();

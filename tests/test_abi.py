@@ -73,7 +73,7 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert L.cs_step(None, 1 / 60) == _abi.CS_E_NOT_INIT
     assert L.cs_last_error() == b"CpuSolver not initialized"
     assert L.cs_get_positions(None, None, 0) == _abi.CS_E_NOT_INIT
-    assert L.cs_abi_version() == 1
+    assert L.cs_abi_version() == 2
 
 
 @pytest.mark.parametrize("make,sizes", [

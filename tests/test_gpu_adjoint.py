@@ -176,3 +176,24 @@ def test_one_step_gradients_with_floor_contact_match_fd():
         print(f"{A.PARAMS[j]}: adjoint {dth[j]:.9e} fd {fd:.9e} rel {rel_err(dth[j], fd):.1e} active {ap}/{am}")
         assert ap == act0 and am == act0
         assert rel_err(dth[j], fd) < REL, j
+
+
+def test_material_adjoint_at_128x128_first_steps_match_fd():
+    # The 128x128 material-adjoint workload (two pinned corners, seeded true
+    # and guessed parameters), its first 2 steps: targets from the true
+    # parameters, the trajectory loss's gradient at the guess by checkpointed
+    # step adjoints against central differences in every parameter.
+    sc, theta_true, theta_guess = A.material_problem(127, 1.0, seed=0)
+    g = A.converged_solver(sc, "fp64")
+    r = A.MaterialRollout(g, 2, H, checkpoint_every=1)
+    targets = r.run(theta_true)
+    L, grad, fd = A.fd_check(r, theta_guess, targets, rel_eps=1e-3)
+    print(f"[c3] {sc.mesh.count} vertices, loss {L:.6e}, forward |grad| up to {r.max_grad_norm:.1e}, "
+          f"adjoint iterations {[it for it, _ in r.adjoint_iters]}", flush=True)
+    for j in range(5):
+        if theta_guess[j] == 0 and abs(fd[j]) < 1e-300:
+            continue
+        print(f"[c3] {A.PARAMS[j]}: adjoint {grad[j]:.9e} fd {fd[j]:.9e} rel {rel_err(grad[j], fd[j]):.1e}",
+              flush=True)
+        assert rel_err(grad[j], fd[j]) < REL, A.PARAMS[j]
+    g.close()

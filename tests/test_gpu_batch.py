@@ -63,3 +63,45 @@ def test_batch_reports_the_failing_instance():
     assert not np.array_equal(gs[2].getPositions(), ref[2])
     for g in gs:
         g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_of_eight_uneven_drapes(precision):
+    # The first 8 instances of the 64-sheet uneven drape workload (1e3..1e5
+    # vertices, per-instance materials, each over its own sphere and a floor),
+    # stepped as one batch and one at a time: identical trajectories.
+    sb, sa = S.batch_drapes(8), S.batch_drapes(8)
+    print(f"[batch] vertices per instance: {[sc.mesh.count for sc in sb]}", flush=True)
+    gb = [HipSolver(precision, precond=_abi.CS_PRECOND_BLOCK_JACOBI) for _ in sb]
+    ga = [HipSolver(precision, precond=_abi.CS_PRECOND_BLOCK_JACOBI) for _ in sa]
+    for g, sc in zip(gb + ga, sb + sa):
+        g.initialize(sc)
+    B = HipBatch(gb, threads=8)
+    for t in range(2):
+        assert B.step(H) == [0] * len(gb)
+        for g in ga:
+            g.step(H)
+        for k, (a, b) in enumerate(zip(sa, sb)):
+            assert np.array_equal(a.positions, b.positions), (t, k)
+            assert np.array_equal(a.velocities, b.velocities), (t, k)
+    for g in gb + ga:
+        g.close()
+
+
+def test_smallest_uneven_drape_matches_oracle():
+    # FP64 first steps of the smallest of those 8 instances against the oracle.
+    import oracle_ffi as O
+    scenes = S.batch_drapes(8)
+    k = int(np.argmin([sc.mesh.count for sc in scenes]))
+    sg, so = scenes[k], S.batch_drapes(8)[k]
+    g = HipSolver("fp64", precond=_abi.CS_PRECOND_BLOCK_JACOBI)
+    g.initialize(sg)
+    o = O.OracleSolver(so, _abi.options(precond=_abi.CS_PRECOND_BLOCK_JACOBI))
+    for t in range(2):
+        g.step(H)
+        o.step(H)
+        print(f"[batch] instance {k} ({sg.mesh.count} vertices) step {t}: newton {g.lastStats['newton_used']} "
+              f"pcg {g.lastStats['pcg_iters']}", flush=True)
+        assert g.lastStats["pcg_iters"] == o.lastStats["pcg_iters"]
+        assert np.array_equal(sg.positions, so.positions) and np.array_equal(sg.velocities, so.velocities)
+    g.close()

@@ -167,3 +167,60 @@ def test_mixed_hvp_kernel_matches_oracle_analytic_hvp(make):
     print(f"{sc.mesh.count} vertices: FP32 HVP rel err {err:.2e}")
     assert err < 1e-6  # FP32 arithmetic on FP32 inputs (measured ~1.5e-7)
     g.close()
+
+
+@pytest.mark.parametrize("name", ["stacked", "plate", "floor", "crumpled", "sphere_touch"])
+def test_mixed_contact_hvp_matches_oracle_apply_hvp(name):
+    # The frozen barrier term of the mixed solve (contacts rounded to FP32,
+    # per-entry terms, in-wave running sums per vertex, added by the HVP
+    # kernel) against the oracle's applyHvp (contact-assembly.ts:371-392) of
+    # the same active set, and the whole FP32 operator against the oracle's
+    # membrane HVP + M p / h^2 + applyHvp.
+    from contact_scenes import ALL
+    sc = ALL[name]()
+    g = HipSolver("mixed")
+    g.initialize(sc)
+    o = O.OracleSolver(ALL[name]())
+    rng = np.random.default_rng(5)
+    x = sc.positions.copy()
+    p = rng.standard_normal(len(x)).astype(np.float32).astype(np.float64)
+    inv_h2 = 3600.0
+    got, k = g.probe_hvp_contact(x, x, p, inv_h2)
+    mem32 = g.probe_hvp(x, p, inv_h2)
+    want_c, ko = o.contact_hvp(x, x, p)
+    assert k == ko and k > 0
+    m = np.repeat(np.asarray(sc.mesh.masses, np.float64), 3)
+    want = o.membrane_hvp(x, p, analytic=True) + m * p * inv_h2 + want_c
+    free = np.ones(len(x), bool)
+    for v in sc.pinned:
+        free[v * 3:v * 3 + 3] = False
+    assert np.all(got[~free] == 0.0)
+    err_c = np.max(np.abs((got - mem32)[free] - want_c[free])) / np.max(np.abs(want_c[free]))
+    err = np.max(np.abs(got[free] - want[free])) / np.max(np.abs(want[free]))
+    print(f"[contact hvp] {name}: {k} active, FP32 contact term rel err {err_c:.2e}, operator {err:.2e}", flush=True)
+    assert err_c < 1e-5 and err < 1e-5
+    g.close()
+
+
+def test_mixed_hvp_fallback_kernel_on_a_70_triangle_fan():
+    # A vertex with 70 corners does not fit one 64-lane chunk, so the mixed
+    # solve uses the per-vertex gather kernel (8 lanes per vertex over the
+    # CSR); it must give the analytic HVP as the chunked kernel does.
+    from test_abi import _chunks, _fan
+    sc = _fan(70)
+    assert len(_chunks(sc)[0]) == 0  # no chunk layout: the fallback kernel runs
+    g = HipSolver("mixed")
+    g.initialize(sc)
+    o = O.OracleSolver(_fan(70))
+    rng = np.random.default_rng(70)
+    x = sc.positions * 1.02 + rng.standard_normal(len(sc.positions)) * 1e-3
+    x = x.astype(np.float32).astype(np.float64)
+    p = rng.standard_normal(len(x)).astype(np.float32).astype(np.float64)
+    inv_h2 = 3600.0
+    got = g.probe_hvp(x, p, inv_h2)
+    m = np.repeat(np.asarray(sc.mesh.masses, np.float64), 3)
+    want = o.membrane_hvp(x, p, analytic=True) + m * p * inv_h2
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"fan70: FP32 HVP (fallback kernel) rel err {err:.2e}", flush=True)
+    assert err < 1e-6
+    g.close()
