@@ -9,7 +9,7 @@ Prints one JSON line:
   value        mixed-precision HipSolver steps/s (state resident in HBM;
                host copy-out of positions is excluded and reported apart)
   fp64         the FP64 HipSolver (reference operator) on the same scene
-  roofline     HVP kernel (k_mx_hvp_ck): algorithmic (compulsory) bytes per
+  roofline     HVP kernel (k_mx_hvp_tile): algorithmic (compulsory) bytes per
                launch, 40n + 32m + 64C for n vertices, m triangles and C
                active contacts, / mean launch time of back-to-back launches on
                the operands the timed steps left resident (HIP events on the
@@ -75,9 +75,9 @@ PROFILE_NAME = {"sphere": "sphere_drape_256", "fold": "crumple_fold_512", "strip
 
 
 def profile_traffic(workload):
-    """HBM bytes per k_mx_hvp_ck launch from the committed counter summary of
+    """HBM bytes per k_mx_hvp_tile launch from the committed counter summary of
     this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if there is one."""
-    prof = os.path.join(ROOT, "profiles", f"k_mx_hvp_ck_{PROFILE_NAME.get(workload, workload)}.json")
+    prof = os.path.join(ROOT, "profiles", f"k_mx_hvp_tile_{PROFILE_NAME.get(workload, workload)}.json")
     if not os.path.exists(prof):
         return None, None
     with open(prof) as f:
@@ -200,7 +200,7 @@ def pair_section(workload, steps, warmup, h):
                 mixed=m, fp64=res["fp64"], mixed_over_fp64=res["fp64"]["ms_per_step"] / m["ms_per_step"],
                 pos_rel_err_mixed_vs_fp64=float(np.max(np.abs(pm - pd)) / np.max(np.abs(pd))),
                 roofline_hvp={"bound": "hbm", "achieved": m["hvp_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": m["hvp_gbs"] / HBM_PEAK_GBS, "kernel": "k_mx_hvp_ck",
+                              "frac": m["hvp_gbs"] / HBM_PEAK_GBS, "kernel": "k_mx_hvp_tile",
                               "bytes_per_launch": m["hvp_algorithmic_bytes"],
                               "kernel_bytes_per_launch": m["hvp_kernel_bytes"],
                               "traffic": tr[0], "traffic_source": tr[1]})
@@ -535,7 +535,8 @@ def main():
     print(f"[bench] mixed: {args.steps} steps in {t_m:.3f} s", file=sys.stderr, flush=True)
     st_m = gm.lastStats
     hvp_ms, hvp_bytes = gm.time_hvp(args.hvp_reps)
-    print(f"[bench] k_mx_hvp_ck: {1000 * hvp_ms:.2f} us x {args.hvp_reps} launches", file=sys.stderr, flush=True)
+    print(f"[bench] k_mx_hvp_tile: {1000 * hvp_ms:.2f} us x {args.hvp_reps} launches ({gm.hvp_layout()} tiles)",
+          file=sys.stderr, flush=True)
     res = {}
     if gd:
         t_d = timed_steps(gd, h, args.steps)
@@ -604,7 +605,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_mx_hvp_ck (FP32 analytic membrane HVP + mass + frozen-contact HVP)",
+                     "kernel": "k_mx_hvp_tile (FP32 analytic membrane HVP + mass + frozen-contact HVP over "
+                               "vertex-owner tiles)",
                      "bytes_per_launch": hvp_bytes, "kernel_bytes_per_launch": hvp_kernel_bytes,
                      "bytes_model": "40n + 32m + 64C (compulsory: x, p, y, mass per vertex; indices, Dm^-1, area "
                                     "per triangle; one record per active contact)",

@@ -19,6 +19,26 @@
 
 namespace cloth {
 
+// Vertex-owner tiles of the FP32 HVP (built once on the host): each tile owns
+// a compact set of vertices (Morton order of the rest positions) and lists
+// every triangle incident to them, so a work-group evaluates each of its
+// triangles once into LDS and sums the owned vertices' corners there, and
+// every vertex's product is finished by exactly one work-group.
+constexpr uint32_t kTileOwn = 256, kTileTris = 640, kTileLoc = 384;
+struct TileHdr {
+  uint32_t loc, n_own, n_loc, tri, n_tri, ent, off, pad;
+};
+struct DTiles {
+  const TileHdr* hdr;
+  uint32_t count;
+  const uint32_t* loc;    // local vertices per tile, owned first
+  const float4* inv;      // Dm^-1 per tile triangle
+  const uint2* lidx;      // local corners (a | b << 16, c)
+  const float* area;      // area per tile triangle
+  const uint16_t* off;    // per owned vertex: its first entry (n_own + 1 per tile)
+  const uint16_t* ent;    // slot * 4 + corner, ascending triangle order
+};
+
 struct DMesh {
   uint32_t n, m, nh;
   const uint32_t* idx;
@@ -35,12 +55,7 @@ struct DMesh {
   const uint32_t* vh_off;
   const uint32_t* vh_rec;
   const uint8_t* pinmask;  // 1 = vertex is pinned (makePinFilter, scene.ts:46-55)
-  // vt_corner regrouped into 64-lane chunks that each hold whole vertices
-  // (padding 0xffffffff, a vertex without triangles one 0xfffffffe lane);
-  // null when some vertex has more than 64 corners
-  const uint4* ck_corner;
-  uint32_t ck_count;   // chunks
-  uint32_t ck_maxdeg;  // largest lane count of one vertex
+  DTiles tl;           // count 0 when the mesh was not tiled
 };
 
 struct DMat {
@@ -58,8 +73,7 @@ struct DMesh32 {
   const uint32_t* vt_rec;
   const uint4* vt_corner;
   const uint8_t* pinmask;
-  const uint4* ck_corner;
-  uint32_t ck_count, ck_maxdeg;
+  DTiles tl;
 };
 
 struct Scalars;  // device scalar block (solver.hip)

@@ -58,6 +58,7 @@ struct LbNode {  // 32 B: two float4 loads
 };
 
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLocalBit = 0x80000000u;  // in a parent link: see k_lb_build
 constexpr uint32_t kEnd = 0xffffffffu;
 constexpr int kScanBlock = 1024;
 
@@ -293,13 +294,17 @@ __global__ void __launch_bounds__(kBlock) k_lb_build(const uint64_t* __restrict_
   const uint32_t right = last == gamma + 1 ? (kLeafBit | (gamma + 1)) : gamma + 1;
   nodes[i].left = left;
   nodes[i].last = last;
-  if (left & kLeafBit) lparent[gamma] = i;
-  else iparent[gamma] = i;
-  if (right & kLeafBit) lparent[gamma + 1] = i;
-  else iparent[gamma + 1] = i;
+  // the parent links carry kLocalBit when node i's leaves all belong to one
+  // refit work-group (so both of its climbers run in that work-group)
+  const uint32_t link = i | ((first / kBlock == last / kBlock) ? kLocalBit : 0u);
+  if (left & kLeafBit) lparent[gamma] = link;
+  else iparent[gamma] = link;
+  if (right & kLeafBit) lparent[gamma + 1] = link;
+  else iparent[gamma + 1] = link;
   esc[gamma] = right;
 }
 
+template <int kScope>
 __device__ __forceinline__ void child_box(const LbLeaf* __restrict__ leaves, const LbNode* nodes, uint32_t c,
                                           float lo[3], float hi[3]) {
   if (c & kLeafBit) {
@@ -310,41 +315,54 @@ __device__ __forceinline__ void child_box(const LbLeaf* __restrict__ leaves, con
     }
   } else {
     for (int k = 0; k < 3; k++) {
-      lo[k] = __hip_atomic_load(&nodes[c].lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      hi[k] = __hip_atomic_load(&nodes[c].hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      lo[k] = __hip_atomic_load(&nodes[c].lo[k], __ATOMIC_RELAXED, kScope);
+      hi[k] = __hip_atomic_load(&nodes[c].hi[k], __ATOMIC_RELAXED, kScope);
     }
   }
 }
 
-// Bottom-up refit: leaf j climbs; at each parent the first arrival stops and
-// the second (which sees the sibling's box through the counter's
-// acquire/release) writes the union.  Depth is at most 30 + log2(m) + 1.
+// Node p's box from its children's, once both have arrived: the first
+// arrival at p's counter stops, the second (which sees the sibling's box
+// through the counter's release / acquire) writes the union.  Returns whether
+// this thread continues upwards.
+template <int kScope>
+__device__ __forceinline__ bool refit_node(const LbLeaf* __restrict__ leaves, LbNode* nodes, uint32_t p,
+                                           uint32_t* flags) {
+  if (__hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, kScope) == 0) return false;
+  const uint32_t left = nodes[p].left, last = nodes[p].last;
+  const uint32_t g = left & ~kLeafBit;
+  const uint32_t right = last == g + 1 ? (kLeafBit | (g + 1)) : g + 1;
+  float a[3], b[3], c[3], d[3];
+  child_box<kScope>(leaves, nodes, left, a, b);
+  child_box<kScope>(leaves, nodes, right, c, d);
+  for (int k = 0; k < 3; k++) {
+    __hip_atomic_store(&nodes[p].lo[k], fminf(a[k], c[k]), __ATOMIC_RELAXED, kScope);
+    __hip_atomic_store(&nodes[p].hi[k], fmaxf(b[k], d[k]), __ATOMIC_RELAXED, kScope);
+  }
+  return true;
+}
+
+// Bottom-up refit: leaf j climbs from its parent.  A node whose leaves all
+// belong to this work-group synchronises at work-group scope; a node spanning
+// work-groups at agent scope, whose release also publishes the boxes this
+// thread wrote below it.  Depth is at most 30 + log2(m) + 1.
 __global__ void __launch_bounds__(kBlock) k_lb_refit(const LbLeaf* __restrict__ leaves, uint32_t m, LbNode* nodes,
                                                      const uint32_t* __restrict__ iparent,
                                                      const uint32_t* __restrict__ lparent, uint32_t* flags,
                                                      LbMisc* misc) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m || m < 2) return;
-  uint32_t p = lparent[j];
+  uint32_t link = lparent[j];
   for (int depth = 0;; depth++) {
+    const uint32_t p = link & ~kLocalBit;
     if (depth >= 128 || p + 1 >= m) {
       atomicOr(&misc->error, 2u);
       return;
     }
-    const uint32_t prev = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == 0) return;
-    const uint32_t left = nodes[p].left, last = nodes[p].last;
-    const uint32_t g = left & ~kLeafBit;
-    const uint32_t right = last == g + 1 ? (kLeafBit | (g + 1)) : g + 1;
-    float a[3], b[3], c[3], d[3];
-    child_box(leaves, nodes, left, a, b);
-    child_box(leaves, nodes, right, c, d);
-    for (int k = 0; k < 3; k++) {
-      __hip_atomic_store(&nodes[p].lo[k], fminf(a[k], c[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&nodes[p].hi[k], fmaxf(b[k], d[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (p == 0) return;
-    p = iparent[p];
+    const bool up = (link & kLocalBit) ? refit_node<__HIP_MEMORY_SCOPE_WORKGROUP>(leaves, nodes, p, flags)
+                                       : refit_node<__HIP_MEMORY_SCOPE_AGENT>(leaves, nodes, p, flags);
+    if (!up || p == 0) return;
+    link = iparent[p];
   }
 }
 

@@ -31,7 +31,6 @@ struct MxArgs {
   DMesh32 me;
   float c00, c11, c01, g, thick, inv_h2;
   const float* x;
-  const float4* tri;  // k_mx_tri_prep output (chunked HVP), 3 per triangle
   int psd;            // stress projected to PSD (stress_psd)
   const float* diag;
   const float* binv;  // null => Jacobi
@@ -135,70 +134,6 @@ __device__ __forceinline__ void hvp_corner32(const MxArgs& a, uint4 cr, const fl
   if (pown)
 #pragma unroll
     for (int k = 0; k < 3; k++) pown[k] = corner == 0 ? v0[k] : corner == 1 ? v1[k] : v2[k];
-}
-
-// Per-triangle quantities of the membrane HVP that depend only on the
-// linearisation point x, computed once per solve: the deformation gradient
-// columns F0, F1, the second Piola stress S and thickness * area, packed in
-// three float4 per triangle (F0x F0y F0z F1x | F1y F1z S00 S11 | S01 sc - -).
-__global__ void __launch_bounds__(kBlock) k_mx_tri_prep(MxArgs a, float4* __restrict__ tri) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.me.m) return;
-  const uint32_t i0 = a.me.idx[(size_t)t * 3], i1 = a.me.idx[(size_t)t * 3 + 1], i2 = a.me.idx[(size_t)t * 3 + 2];
-  float x0[3], x1[3], x2[3];
-  ld3f(a.x, i0, x0); ld3f(a.x, i1, x1); ld3f(a.x, i2, x2);
-  const float4 inv = *reinterpret_cast<const float4*>(a.me.inv_dm + (size_t)t * 4);
-  float F0[3], F1[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const float e1 = x1[k] - x0[k], e2 = x2[k] - x0[k];
-    F0[k] = inv.x * e1 + inv.z * e2;
-    F1[k] = inv.y * e1 + inv.w * e2;
-  }
-  const float E00 = 0.5f * (F0[0] * F0[0] + F0[1] * F0[1] + F0[2] * F0[2] - 1.f);
-  const float E11 = 0.5f * (F1[0] * F1[0] + F1[1] * F1[1] + F1[2] * F1[2] - 1.f);
-  const float E01 = 0.5f * (F0[0] * F1[0] + F0[1] * F1[1] + F0[2] * F1[2]);
-  float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
-  if (a.psd) stress_psd(S00, S11, S01);
-  tri[(size_t)t * 3] = make_float4(F0[0], F0[1], F0[2], F1[0]);
-  tri[(size_t)t * 3 + 1] = make_float4(F1[1], F1[2], S00, S11);
-  tri[(size_t)t * 3 + 2] = make_float4(S01, a.thick * a.me.area[t], 0.f, 0.f);
-}
-
-// hvp_corner32 from the prepared triangle quantities (same arithmetic on the
-// p-dependent terms).
-__device__ __forceinline__ void hvp_corner32_prep(const MxArgs& a, uint4 cr, const float* p, float* out,
-                                                  float* pown) {
-  const uint32_t i0 = cr.x, i1 = cr.y, i2 = cr.z, t = cr.w >> 2;
-  const int corner = cr.w & 3;
-  const float4 q0 = a.tri[(size_t)t * 3], q1 = a.tri[(size_t)t * 3 + 1], q2 = a.tri[(size_t)t * 3 + 2];
-  const float4 inv = *reinterpret_cast<const float4*>(a.me.inv_dm + (size_t)t * 4);
-  float v0[3], v1[3], v2[3];
-  ld3f(p, i0, v0); ld3f(p, i1, v1); ld3f(p, i2, v2);
-  const float ia = inv.x, ib = inv.y, ic = inv.z, id = inv.w;
-  const float F0[3] = {q0.x, q0.y, q0.z}, F1[3] = {q0.w, q1.x, q1.y};
-  const float S00 = q1.z, S11 = q1.w, S01 = q2.x, sc = q2.y;
-  float dF0[3], dF1[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const float w1 = v1[k] - v0[k], w2 = v2[k] - v0[k];
-    dF0[k] = ia * w1 + ic * w2;
-    dF1[k] = ib * w1 + id * w2;
-  }
-  const float dE00 = F0[0] * dF0[0] + F0[1] * dF0[1] + F0[2] * dF0[2];
-  const float dE11 = F1[0] * dF1[0] + F1[1] * dF1[1] + F1[2] * dF1[2];
-  const float dE01 = 0.5f * (dF0[0] * F1[0] + F0[0] * dF1[0] + dF0[1] * F1[1] + F0[1] * dF1[1] + dF0[2] * F1[2] +
-                             F0[2] * dF1[2]);
-  const float dS00 = a.c00 * dE00 + a.c01 * dE11, dS11 = a.c11 * dE11 + a.c01 * dE00, dS01 = 2.f * a.g * dE01;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const float dP0 = dF0[k] * S00 + F0[k] * dS00 + dF1[k] * S01 + F1[k] * dS01;
-    const float dP1 = dF0[k] * S01 + F0[k] * dS01 + dF1[k] * S11 + F1[k] * dS11;
-    const float g1 = sc * (dP0 * ia + dP1 * ib);
-    const float g2 = sc * (dP0 * ic + dP1 * id);
-    out[k] = corner == 0 ? -(g1 + g2) : corner == 1 ? g1 : g2;
-    pown[k] = corner == 0 ? v0[k] : corner == 1 ? v1[k] : v2[k];
-  }
 }
 
 // Frozen barrier HVP term of one incidence entry: coeff (J.p) times the
@@ -421,97 +356,101 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
   block_partial_f(pap, mpart(a, MX_PAP));
 }
 
-// The same product over the chunked corner records: one lane per corner,
-// whole vertices per wave, so no lane waits on the vertex offsets before its
-// record load.  A segmented inclusive scan over the lanes of each vertex
-// (fixed order) leaves the vertex's sum in its last lane, which adds the
-// contact sums and the mass term and writes Ap.
-struct CkLane {
-  uint32_t vid;
-  bool pad, self, last;
-  float mh;
-  bool pinned;
-  uint32_t cb, ce;
-};
-
-__device__ __forceinline__ CkLane ck_lane(const MxArgs& a, uint4 cr, int lane) {
-  CkLane L;
-  L.pad = cr.w == 0xffffffffu;
-  L.self = cr.w == 0xfffffffeu;
-  const int corner = cr.w & 3;
-  L.vid = L.pad ? 0xffffffffu : (L.self || corner == 0) ? cr.x : corner == 1 ? cr.y : cr.z;
-  const uint32_t next = __shfl_down(L.vid, 1, kWave);
-  L.last = !L.pad && (lane == kWave - 1 || next != L.vid);
-  L.mh = 0.f;
-  L.pinned = false;
-  L.cb = L.ce = 0;
-  // the last lane's per-vertex operands, issued before the corner gathers
-  if (L.last) {
-    L.mh = a.me.mass[L.vid] * a.inv_h2;
-    L.pinned = a.me.pinmask[L.vid] != 0;
-    if (a.scan) {
-      L.cb = a.v_off[L.vid];
-      L.ce = a.v_off[L.vid + 1];
-    }
-  }
-  return L;
-}
-
-__device__ __forceinline__ float ck_finish(const MxArgs& a, const CkLane& L, int lane, float* t, const float* pv) {
-  float cs[3] = {0.f, 0.f, 0.f};
-  if (L.cb < L.ce)
-    for (uint32_t k = L.cb / kWave; k <= (L.ce - 1) / kWave; k++) {
-      const size_t e = (size_t)min(L.ce, (k + 1) * (uint32_t)kWave) - 1;
-      cs[0] += a.scan[e * 3];
-      cs[1] += a.scan[e * 3 + 1];
-      cs[2] += a.scan[e * 3 + 2];
-    }
-  for (uint32_t d = 1; d < a.me.ck_maxdeg; d <<= 1) {
-    const uint32_t uu = __shfl_up(L.vid, d, kWave);
-    float y[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) y[i] = __shfl_up(t[i], d, kWave);
-    if (lane >= (int)d && uu == L.vid) {
-#pragma unroll
-      for (int i = 0; i < 3; i++) t[i] += y[i];
-    }
-  }
-  float pap = 0.f;
-  if (L.last) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const float ap = L.pinned ? 0.f : L.mh * pv[i] + (t[i] + cs[i]);
-      a.Ap[(size_t)L.vid * 3 + i] = ap;
-      pap += pv[i] * ap;
-    }
-  }
-  return pap;
-}
-
-// Work-groups are dispatched to the 8 XCDs round-robin (block b on XCD
-// b % 8), each XCD with its own L2.  Block b takes the contiguous chunk range
-// of logical block (b % 8) * (G / 8) + b / 8, so each XCD walks one eighth of
-// the vertex order and its x/p gathers stay within that eighth (and its
-// halo) in its own L2 (measured on the 512x512 fold: L2 hits 50k -> 211k,
-// misses 498k -> 234k per launch).  G is a multiple of 8.
-__global__ void __launch_bounds__(kBlock) k_mx_hvp_ck(MxArgs a) {
-  // the PCG latch, read by every wave without a barrier: an earlier kernel
-  // set it, so every wave of the grid sees the same value
+// The same product over vertex-owner tiles: the work-group stages its local
+// vertices' x and p in LDS, evaluates every tile triangle once (all three
+// corners, membrane-hvp.ts:18-75 from x, no prepared records), then each
+// owned vertex sums its corners in ascending triangle order, adds its contact
+// sums and the mass term and writes Ap.  Tiles are in Morton order; each XCD
+// walks a contiguous eighth of them.
+__global__ void __launch_bounds__(kBlock) k_mx_hvp_tile(MxArgs a) {
   if (__hip_atomic_load(&a.s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-  const int lane = threadIdx.x % kWave;
-  const uint32_t wpb = kBlock / kWave;
+  __shared__ float xs[kTileLoc * 3], ps[kTileLoc * 3], out[kTileTris * 9];
+  const DTiles& T = a.me.tl;
   const uint32_t G = gridDim.x;
   const uint32_t lb = (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds;
-  const uint32_t cpb = (a.me.ck_count + G - 1) / G;
-  const uint32_t c_end = min(a.me.ck_count, (lb + 1) * cpb);
   float pap = 0;
-  for (uint32_t c = lb * cpb + threadIdx.x / kWave; c < c_end; c += wpb) {
-    const uint4 cr = a.me.ck_corner[(size_t)c * kWave + lane];
-    const CkLane L = ck_lane(a, cr, lane);
-    float t[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f};
-    if (L.self) ld3f(a.p, L.vid, pv);
-    else if (!L.pad) hvp_corner32_prep(a, cr, a.p, t, pv);
-    pap += ck_finish(a, L, lane, t, pv);
+  for (uint32_t tile = lb; tile < T.count; tile += G) {
+    const TileHdr h = T.hdr[tile];
+    __syncthreads();  // the previous tile's LDS reads are done
+    for (uint32_t l = threadIdx.x; l < h.n_loc; l += kBlock) {
+      const uint32_t v = T.loc[h.loc + l];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        xs[l * 3 + k] = a.x[(size_t)v * 3 + k];
+        ps[l * 3 + k] = a.p[(size_t)v * 3 + k];
+      }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < h.n_tri; t += kBlock) {
+      const uint2 li = T.lidx[h.tri + t];
+      const float4 inv = T.inv[h.tri + t];
+      const float sc = a.thick * T.area[h.tri + t];
+      const uint32_t i0 = li.x & 0xffffu, i1 = li.x >> 16, i2 = li.y;
+      const float ia = inv.x, ib = inv.y, ic = inv.z, id = inv.w;
+      float F0[3], F1[3], dF0[3], dF1[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float e1 = xs[i1 * 3 + k] - xs[i0 * 3 + k], e2 = xs[i2 * 3 + k] - xs[i0 * 3 + k];
+        const float w1 = ps[i1 * 3 + k] - ps[i0 * 3 + k], w2 = ps[i2 * 3 + k] - ps[i0 * 3 + k];
+        F0[k] = ia * e1 + ic * e2;
+        F1[k] = ib * e1 + id * e2;
+        dF0[k] = ia * w1 + ic * w2;
+        dF1[k] = ib * w1 + id * w2;
+      }
+      const float E00 = 0.5f * (F0[0] * F0[0] + F0[1] * F0[1] + F0[2] * F0[2] - 1.f);
+      const float E11 = 0.5f * (F1[0] * F1[0] + F1[1] * F1[1] + F1[2] * F1[2] - 1.f);
+      const float E01 = 0.5f * (F0[0] * F1[0] + F0[1] * F1[1] + F0[2] * F1[2]);
+      float S00 = a.c00 * E00 + a.c01 * E11, S11 = a.c11 * E11 + a.c01 * E00, S01 = 2.f * a.g * E01;
+      if (a.psd) stress_psd(S00, S11, S01);
+      const float dE00 = F0[0] * dF0[0] + F0[1] * dF0[1] + F0[2] * dF0[2];
+      const float dE11 = F1[0] * dF1[0] + F1[1] * dF1[1] + F1[2] * dF1[2];
+      const float dE01 = 0.5f * (dF0[0] * F1[0] + F0[0] * dF1[0] + dF0[1] * F1[1] + F0[1] * dF1[1] +
+                                 dF0[2] * F1[2] + F0[2] * dF1[2]);
+      const float dS00 = a.c00 * dE00 + a.c01 * dE11, dS11 = a.c11 * dE11 + a.c01 * dE00, dS01 = 2.f * a.g * dE01;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float dP0 = dF0[k] * S00 + F0[k] * dS00 + dF1[k] * S01 + F1[k] * dS01;
+        const float dP1 = dF0[k] * S01 + F0[k] * dS01 + dF1[k] * S11 + F1[k] * dS11;
+        const float g1 = sc * (dP0 * ia + dP1 * ib);
+        const float g2 = sc * (dP0 * ic + dP1 * id);
+        out[t * 9 + k] = -(g1 + g2);
+        out[t * 9 + 3 + k] = g1;
+        out[t * 9 + 6 + k] = g2;
+      }
+    }
+    __syncthreads();
+    for (uint32_t o = threadIdx.x; o < h.n_own; o += kBlock) {
+      const uint32_t v = T.loc[h.loc + o];
+      float acc[3] = {0.f, 0.f, 0.f};
+      const uint32_t e1 = T.off[h.off + o + 1];
+      for (uint32_t e = T.off[h.off + o]; e < e1; e++) {
+        const uint32_t q = T.ent[h.ent + e];
+        const float* g = out + (q >> 2) * 9 + (q & 3) * 3;
+        acc[0] += g[0];
+        acc[1] += g[1];
+        acc[2] += g[2];
+      }
+      float cs[3] = {0.f, 0.f, 0.f};
+      if (a.scan) {
+        const uint32_t cb = a.v_off[v], ce = a.v_off[v + 1];
+        if (cb < ce)
+          for (uint32_t k = cb / kWave; k <= (ce - 1) / kWave; k++) {
+            const size_t e = (size_t)min(ce, (k + 1) * (uint32_t)kWave) - 1;
+            cs[0] += a.scan[e * 3];
+            cs[1] += a.scan[e * 3 + 1];
+            cs[2] += a.scan[e * 3 + 2];
+          }
+      }
+      const float mh = a.me.mass[v] * a.inv_h2;
+      const bool pinned = a.me.pinmask[v] != 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float pv = ps[o * 3 + k];
+        const float ap = pinned ? 0.f : mh * pv + (acc[k] + cs[k]);
+        a.Ap[(size_t)v * 3 + k] = ap;
+        pap += pv * ap;
+      }
+    }
   }
   block_partial_f(pap, mpart(a, MX_PAP));
 }
@@ -596,7 +535,6 @@ struct MixedImpl {
   uint32_t n;
   hipGraphExec_t exec = nullptr;  // the last captured chunk of kGraphIters iterations
   DevBuf<float> x32, d32, binv32, sol, r, z, p, Ap, partials, scan;
-  DevBuf<float4> tri;
   DevBuf<Frozen32> act32;
   DevBuf<MxScalars> s;
   MxArgs last{};
@@ -606,9 +544,8 @@ struct MixedImpl {
 static unsigned mx_grid(uint32_t n) { return red_grid(n); }
 constexpr int kGraphIters = 8;  // PCG iterations per captured chunk (even)
 static unsigned mx_hvp_grid(const DMesh32& me) {
-  if (me.ck_corner) {
-    const size_t per_block = kBlock / kWave;
-    const size_t g = std::min<size_t>(kMaxHvpBlocks, (me.ck_count + per_block - 1) / per_block);
+  if (me.tl.count) {
+    const size_t g = std::min<size_t>(kMaxHvpBlocks, me.tl.count);
     return (unsigned)((g + kXcds - 1) / kXcds * kXcds);  // a multiple of the XCD count
   }
   return (unsigned)std::min<size_t>(kMaxHvpBlocks, ((size_t)me.n + kMxVertsPerBlock - 1) / kMxVertsPerBlock);
@@ -616,7 +553,7 @@ static unsigned mx_hvp_grid(const DMesh32& me) {
 
 static void launch_mx_hvp(const MxArgs& a, hipStream_t st) {
   const dim3 gh(mx_hvp_grid(a.me)), b(kBlock);
-  if (a.me.ck_corner) hipLaunchKernelGGL(k_mx_hvp_ck, gh, b, 0, st, a);
+  if (a.me.tl.count) hipLaunchKernelGGL(k_mx_hvp_tile, gh, b, 0, st, a);
   else hipLaunchKernelGGL(k_mx_hvp, gh, b, 0, st, a, 0);
 }
 
@@ -654,12 +591,11 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   MxArgs a;
   a.me = DMesh32{in.me.n,         in.me.m,       in.me.idx,       in.me.inv_dm,   in.me.area,
                  in.me.mass,      in.me.vt_off,  in.me.vt_rec,    in.me.vt_corner, in.me.pinmask,
-                 in.me.ck_corner, in.me.ck_count, in.me.ck_maxdeg};
+                 in.me.tl};
   a.c00 = (float)in.mat.c00; a.c11 = (float)in.mat.c11; a.c01 = (float)in.mat.c01; a.g = (float)in.mat.g;
   a.thick = (float)in.mat.thick;
   a.inv_h2 = (float)in.inv_h2;
   a.x = M.x32.ptr; a.diag = M.d32.ptr; a.binv = in.binv ? M.binv32.ptr : nullptr;
-  a.tri = nullptr;
   a.psd = in.psd ? 1 : 0;
   a.sol = M.sol.ptr; a.r = M.r.ptr; a.z = M.z.ptr; a.p = M.p.ptr; a.Ap = M.Ap.ptr;
   a.tol = (float)in.tol;
@@ -673,13 +609,6 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.act = C ? M.act32.ptr : nullptr;
   a.v_off = in.contact.v_off;
   a.v_ent = in.contact.v_ent;
-  if (a.me.ck_corner) {
-    M.tri.alloc((size_t)std::max<uint32_t>(a.me.m, 1) * 3);
-    a.tri = M.tri.ptr;
-    hipLaunchKernelGGL(k_mx_tri_prep, dim3(grid_for(std::max<uint32_t>(a.me.m, 1))), dim3(kBlock), 0, st, a,
-                       M.tri.ptr);
-    CS_LAUNCH_CHECK();
-  }
   M.last = a;
   M.has_last = true;
   const dim3 g(mx_grid(n)), b(kBlock);
@@ -779,7 +708,7 @@ void MixedSolve::probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, cons
   MxArgs a{};
   a.me = DMesh32{me.n,         me.m,        me.idx,       me.inv_dm,   me.area,
                  me.mass,      me.vt_off,   me.vt_rec,    me.vt_corner, me.pinmask,
-                 me.ck_corner, me.ck_count, me.ck_maxdeg};
+                 me.tl};
   a.c00 = (float)mat.c00; a.c11 = (float)mat.c11; a.c01 = (float)mat.c01; a.g = (float)mat.g;
   a.thick = (float)mat.thick;
   a.inv_h2 = (float)inv_h2;
@@ -787,12 +716,6 @@ void MixedSolve::probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, cons
   a.s = M.s.ptr;
   a.partials = M.partials.ptr;
   CS_HIP(hipMemsetAsync(&M.s.ptr->done, 0, sizeof(int32_t), st));
-  if (a.me.ck_corner) {
-    M.tri.alloc((size_t)std::max<uint32_t>(a.me.m, 1) * 3);
-    a.tri = M.tri.ptr;
-    hipLaunchKernelGGL(k_mx_tri_prep, dim3(grid_for(std::max<uint32_t>(a.me.m, 1))), dim3(kBlock), 0, st, a,
-                       M.tri.ptr);
-  }
   const uint32_t C = contact && contact->act ? contact->C : 0;
   const uint32_t E = C ? contact->E : 0;
   if (C) {

@@ -11,6 +11,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cloth_hip.h"
@@ -169,36 +170,138 @@ __global__ void k_adj_out(const double* lam, const double* gv, const float* mass
   dvp[i] = lam[i] * m / h;
 }
 
-// The vertex-sorted corner records regrouped into 64-lane chunks that each
-// hold whole vertices: a vertex's records go to the current chunk when they
-// fit and start a new one (the rest of the current padded with 0xffffffff)
-// otherwise; a vertex without triangles takes one 0xfffffffe lane; the last
-// chunk is padded.  False when some vertex has more than 64 corners (the
-// chunked HVP is then not used).  `maxdeg` is the largest lane count of one
-// vertex.
-static bool pack_chunks(uint32_t n, const std::vector<uint32_t>& off, const std::vector<uint4>& cor,
-                        std::vector<uint4>& ck, uint32_t& maxdeg) {
-  ck.clear();
-  ck.reserve(cor.size() + cor.size() / 8 + kWave);
-  uint32_t fill = 0;
-  maxdeg = 1;
+// Vertex-owner tiles of the FP32 HVP (DTiles): vertices in Morton order of
+// the rest positions are taken greedily into a tile until its owned vertices,
+// its triangles (all incident to an owned vertex) or its local vertices would
+// exceed kTileOwn / kTileTris / kTileLoc.  Within a tile the triangles are in
+// ascending order, the local vertices are the owned ones followed by the
+// others in order of appearance, and each owned vertex lists its corners in
+// ascending triangle order.  Returns false (no tiles) when one vertex alone
+// exceeds a tile.
+struct TileHost {
+  std::vector<TileHdr> hdr;
+  std::vector<uint32_t> loc;
+  std::vector<float4> inv;
+  std::vector<uint2> lidx;
+  std::vector<float> area;
+  std::vector<uint16_t> off, ent;
+};
+
+static bool build_tiles(const cs_scene_desc& d, const std::vector<uint32_t>& voff, const std::vector<uint32_t>& vrec,
+                        TileHost& T) {
+  const uint32_t n = d.n_verts, m = d.n_tris;
+  T = TileHost{};
+  if (!n) return false;
+  // Morton order of the rest positions (10 bits per axis over their box)
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  auto rp = [&](uint32_t v, int k) -> double {
+    return d.rest_positions ? (double)d.rest_positions[(size_t)v * 3 + k] : d.positions[(size_t)v * 3 + k];
+  };
+  for (uint32_t v = 0; v < n; v++)
+    for (int k = 0; k < 3; k++) {
+      const double c = rp(v, k);
+      if (c == c) { lo[k] = std::min(lo[k], c); hi[k] = std::max(hi[k], c); }
+    }
+  std::vector<std::pair<uint32_t, uint32_t>> key(n);
   for (uint32_t v = 0; v < n; v++) {
-    const uint32_t deg = off[v + 1] - off[v], need = std::max<uint32_t>(deg, 1);
-    if (need > kWave) {
-      ck.clear();
-      return false;
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+      const double span = hi[k] - lo[k];
+      double f = span > 0 ? (rp(v, k) - lo[k]) / span : 0.0;
+      f = f == f ? std::min(std::max(f, 0.0), 1.0) : 0.0;
+      q[k] = std::min<uint32_t>(1023u, (uint32_t)(f * 1024.0));
     }
-    if (fill + need > kWave) {
-      for (; fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
-      fill = 0;
-    }
-    if (deg == 0) ck.push_back(make_uint4(v, v, v, 0xfffffffeu));
-    else
-      for (uint32_t q = off[v]; q < off[v + 1]; q++) ck.push_back(cor[q]);
-    fill += need;
-    maxdeg = std::max(maxdeg, need);
+    uint32_t code = 0;
+    for (int b = 9; b >= 0; b--) code = (code << 3) | (((q[0] >> b) & 1) << 2) | (((q[1] >> b) & 1) << 1) | ((q[2] >> b) & 1);
+    key[v] = {code, v};
   }
-  for (; fill > 0 && fill < kWave; fill++) ck.push_back(make_uint4(0, 0, 0, 0xffffffffu));
+  std::sort(key.begin(), key.end());
+  std::vector<int64_t> tstamp(m, -1), vstamp(n, -1);
+  std::vector<uint32_t> own, tris;
+  uint32_t nloc = 0;
+  int64_t tile = 0;
+  auto close = [&]() {
+    if (own.empty()) return;
+    std::sort(tris.begin(), tris.end());
+    TileHdr h{};
+    h.loc = (uint32_t)T.loc.size();
+    h.n_own = (uint32_t)own.size();
+    h.tri = (uint32_t)T.inv.size();
+    h.n_tri = (uint32_t)tris.size();
+    h.ent = (uint32_t)T.ent.size();
+    h.off = (uint32_t)T.off.size();
+    std::unordered_map<uint32_t, uint32_t> lid;
+    for (uint32_t v : own) {
+      lid.emplace(v, (uint32_t)lid.size());
+      T.loc.push_back(v);
+    }
+    for (uint32_t t : tris)
+      for (int c = 0; c < 3; c++) {
+        const uint32_t u = d.indices[(size_t)t * 3 + c];
+        if (lid.emplace(u, (uint32_t)lid.size()).second) T.loc.push_back(u);
+      }
+    h.n_loc = (uint32_t)lid.size();
+    for (uint32_t t : tris) {
+      const uint32_t a = lid[d.indices[(size_t)t * 3]], b = lid[d.indices[(size_t)t * 3 + 1]],
+                     c = lid[d.indices[(size_t)t * 3 + 2]];
+      T.lidx.push_back(make_uint2(a | (b << 16), c));
+      T.inv.push_back(make_float4(d.inv_dm[(size_t)t * 4], d.inv_dm[(size_t)t * 4 + 1], d.inv_dm[(size_t)t * 4 + 2],
+                                  d.inv_dm[(size_t)t * 4 + 3]));
+      T.area.push_back(d.areas[t]);
+    }
+    uint32_t e = 0;
+    for (uint32_t v : own) {
+      T.off.push_back((uint16_t)e);
+      for (uint32_t q = voff[v]; q < voff[v + 1]; q++) {
+        const uint32_t t = vrec[q] >> 2, c = vrec[q] & 3;
+        const uint32_t slot = (uint32_t)(std::lower_bound(tris.begin(), tris.end(), t) - tris.begin());
+        T.ent.push_back((uint16_t)(slot * 4 + c));
+        e++;
+      }
+    }
+    T.off.push_back((uint16_t)e);
+    T.hdr.push_back(h);
+    own.clear();
+    tris.clear();
+    nloc = 0;
+    tile++;
+  };
+  for (const auto& kv : key) {
+    const uint32_t v = kv.second;
+    for (int attempt = 0; attempt < 2; attempt++) {
+      // what v would add to the current tile
+      uint32_t add_t = 0, add_l = vstamp[v] == tile ? 0 : 1;
+      std::vector<uint32_t> nl;
+      for (uint32_t q = voff[v]; q < voff[v + 1]; q++) {
+        const uint32_t t = vrec[q] >> 2;
+        if (tstamp[t] == tile) continue;
+        add_t++;
+        for (int c = 0; c < 3; c++) {
+          const uint32_t u = d.indices[(size_t)t * 3 + c];
+          if (u != v && vstamp[u] != tile && std::find(nl.begin(), nl.end(), u) == nl.end()) nl.push_back(u);
+        }
+      }
+      add_l += (uint32_t)nl.size();
+      if (own.size() + 1 <= kTileOwn && tris.size() + add_t <= kTileTris && nloc + add_l <= kTileLoc) {
+        own.push_back(v);
+        if (vstamp[v] != tile) { vstamp[v] = tile; nloc++; }
+        for (uint32_t q = voff[v]; q < voff[v + 1]; q++) {
+          const uint32_t t = vrec[q] >> 2;
+          if (tstamp[t] == tile) continue;
+          tstamp[t] = tile;
+          tris.push_back(t);
+        }
+        for (uint32_t u : nl) { vstamp[u] = tile; nloc++; }
+        break;
+      }
+      if (attempt == 1 || own.empty()) {
+        T = TileHost{};
+        return false;  // v alone does not fit a tile
+      }
+      close();
+    }
+  }
+  close();
   return true;
 }
 
@@ -215,8 +318,16 @@ struct Solver {
 
   // mesh
   DevBuf<uint32_t> idx, vt_off, vt_rec, hv, vh_off, vh_rec;
-  DevBuf<uint4> vt_corner, ck_corner;
-  uint32_t ck_count = 0, ck_maxdeg = 0;
+  DevBuf<uint4> vt_corner;
+  // vertex-owner tiles of the FP32 HVP
+  DevBuf<TileHdr> tl_hdr;
+  DevBuf<uint32_t> tl_loc;
+  DevBuf<float4> tl_inv;
+  DevBuf<uint2> tl_lidx;
+  DevBuf<float> tl_area;
+  DevBuf<uint16_t> tl_off, tl_ent;
+  uint32_t tl_count = 0;
+  uint64_t tl_tris = 0, tl_locs = 0, tl_ents = 0;
   DevBuf<float> inv_dm, area, mass;
   DevBuf<double> hrest, hlen, hasum;
   // state
@@ -264,9 +375,7 @@ struct Solver {
     d.hv = hv.ptr; d.hrest = hrest.ptr; d.hlen = hlen.ptr; d.hasum = hasum.ptr;
     d.vh_off = vh_off.ptr; d.vh_rec = vh_rec.ptr;
     d.pinmask = pinmask.ptr;
-    d.ck_corner = ck_count ? ck_corner.ptr : nullptr;
-    d.ck_count = ck_count;
-    d.ck_maxdeg = ck_maxdeg;
+    d.tl = DTiles{tl_hdr.ptr, tl_count, tl_loc.ptr, tl_inv.ptr, tl_lidx.ptr, tl_area.ptr, tl_off.ptr, tl_ent.ptr};
     return d;
   }
   DMat dmat() const {
@@ -358,13 +467,20 @@ struct Solver {
         cor[q] = make_uint4(d->indices[t * 3], d->indices[t * 3 + 1], d->indices[t * 3 + 2], rec[q]);
       }
       vt_corner.upload(cor.data(), cor.size(), st);
-      // the same records in 64-lane chunks of whole vertices (the mixed HVP)
-      std::vector<uint4> ck;
-      uint32_t maxdeg = 0;
-      if (pack_chunks(n, off, cor, ck, maxdeg) && n) {
-        ck_corner.upload(ck.data(), ck.size(), st);
-        ck_count = (uint32_t)(ck.size() / kWave);
-        ck_maxdeg = maxdeg;
+      TileHost T;
+      if (o->precision == CS_PREC_MIXED && build_tiles(*d, off, rec, T)) {
+        tl_hdr.upload(T.hdr.data(), T.hdr.size(), st);
+        tl_loc.upload(T.loc.data(), T.loc.size(), st);
+        tl_inv.upload(T.inv.data(), std::max<size_t>(T.inv.size(), 1), st);
+        tl_lidx.upload(T.lidx.data(), std::max<size_t>(T.lidx.size(), 1), st);
+        tl_area.upload(T.area.data(), std::max<size_t>(T.area.size(), 1), st);
+        tl_off.upload(T.off.data(), T.off.size(), st);
+        tl_ent.upload(T.ent.data(), std::max<size_t>(T.ent.size(), 1), st);
+        tl_count = (uint32_t)T.hdr.size();
+        tl_tris = T.inv.size();
+        tl_locs = T.loc.size();
+        tl_ents = T.ent.size();
+        CS_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
       }
     }
     if (nh) {
@@ -965,7 +1081,12 @@ struct Solver {
     const bool mixed_p = opt.precision == CS_PREC_MIXED;
     const double fs = mixed_p ? 4.0 : 8.0;
     double b = (double)n * (9 * fs + 4 + 1 + 4) + (double)m * (16 + 4 + 3 * 16);
-    if (mixed_p && ck_count) b = (double)n * (6 * fs + 4 + 1) + (double)m * (16 + 48 + 3 * 16);
+    // tiles: per tile triangle Dm^-1, area and local corners; per local vertex
+    // its id, x and p; per owned corner a 2 B entry; per vertex mass, pin,
+    // entry offset and Ap; 32 B per tile header
+    if (mixed_p && tl_count)
+      b = (double)tl_tris * 28 + (double)tl_locs * (4 + 6 * fs) + (double)tl_ents * 2 + (double)n * (4 + 1 + 2 + 3 * fs) +
+          32.0 * tl_count;
     if (contact && contact->active_count())
       b += mixed_p ? 4.0 * n + 3 * fs * (double)contact->incidence_vertices() : 3 * fs * n;
     return b;
@@ -1368,43 +1489,46 @@ int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, i
   });
 }
 
-int cs_hvp_chunks(const cs_scene_desc* scene, uint32_t* records, uint64_t cap, uint64_t* lanes,
-                  uint32_t* maxdeg) {
-  if (!scene || !scene->indices || !lanes || !maxdeg) return fail(CS_E_INVALID, "null argument");
+int cs_hvp_tiles(const cs_scene_desc* scene, uint64_t* sizes, uint32_t* hdr, uint32_t* loc, uint32_t* lidx,
+                 uint16_t* off, uint16_t* ent) {
+  if (!scene || !scene->indices || !scene->inv_dm || !scene->areas || !sizes) return fail(CS_E_INVALID, "null argument");
+  if (!scene->positions && !scene->rest_positions) return fail(CS_E_INVALID, "null argument");
   for (size_t k = 0; k < (size_t)scene->n_tris * 3; k++)
     if (scene->indices[k] >= scene->n_verts) return fail(CS_E_INVALID, "triangle index out of range");
   return guarded([&]() -> int {
     const uint32_t n = scene->n_verts, m = scene->n_tris;
-    std::vector<uint32_t> off(n + 1, 0), rec((size_t)m * 3);
+    std::vector<uint32_t> voff(n + 1, 0), vrec((size_t)m * 3);
     for (uint32_t e = 0; e < m; e++)
-      for (int c = 0; c < 3; c++) off[scene->indices[(size_t)e * 3 + c] + 1]++;
-    for (uint32_t v = 0; v < n; v++) off[v + 1] += off[v];
-    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+      for (int c = 0; c < 3; c++) voff[scene->indices[(size_t)e * 3 + c] + 1]++;
+    for (uint32_t v = 0; v < n; v++) voff[v + 1] += voff[v];
+    std::vector<uint32_t> cur(voff.begin(), voff.end() - 1);
     for (uint32_t e = 0; e < m; e++)
-      for (int c = 0; c < 3; c++) rec[cur[scene->indices[(size_t)e * 3 + c]]++] = (e << 2) | (uint32_t)c;
-    std::vector<uint4> cor(rec.size());
-    for (size_t q = 0; q < rec.size(); q++) {
-      const size_t t = rec[q] >> 2;
-      cor[q] = make_uint4(scene->indices[t * 3], scene->indices[t * 3 + 1], scene->indices[t * 3 + 2], rec[q]);
+      for (int c = 0; c < 3; c++) vrec[cur[scene->indices[(size_t)e * 3 + c]]++] = (e << 2) | (uint32_t)c;
+    cloth::TileHost T;
+    const bool ok = cloth::build_tiles(*scene, voff, vrec, T);
+    const uint64_t want[5] = {T.hdr.size(), T.loc.size(), T.lidx.size(), T.off.size(), T.ent.size()};
+    const bool fill = hdr && loc && lidx && off && ent;
+    if (fill)
+      for (int k = 0; k < 5; k++)
+        if (sizes[k] < want[k]) return fail(CS_E_OVERFLOW, "tile buffers are too small (sizes holds the sizes needed)");
+    for (int k = 0; k < 5; k++) sizes[k] = ok ? want[k] : 0;
+    if (!ok || !fill) return CS_OK;
+    for (size_t t = 0; t < T.hdr.size(); t++) std::memcpy(hdr + t * 8, &T.hdr[t], sizeof(cloth::TileHdr));
+    std::copy(T.loc.begin(), T.loc.end(), loc);
+    for (size_t t = 0; t < T.lidx.size(); t++) {
+      lidx[t * 2] = T.lidx[t].x;
+      lidx[t * 2 + 1] = T.lidx[t].y;
     }
-    std::vector<uint4> ck;
-    uint32_t md = 0;
-    if (!cloth::pack_chunks(n, off, cor, ck, md)) {
-      *lanes = 0;
-      *maxdeg = 0;
-      return CS_OK;
-    }
-    *lanes = ck.size();
-    *maxdeg = md;
-    if (records)
-      for (size_t k = 0; k < ck.size() && k < cap; k++) {
-        records[k * 4] = ck[k].x;
-        records[k * 4 + 1] = ck[k].y;
-        records[k * 4 + 2] = ck[k].z;
-        records[k * 4 + 3] = ck[k].w;
-      }
+    std::copy(T.off.begin(), T.off.end(), off);
+    std::copy(T.ent.begin(), T.ent.end(), ent);
     return CS_OK;
   });
+}
+
+int cs_hvp_layout(cs_solver* s, uint32_t* tiles) {
+  if (!s) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+  if (tiles) *tiles = s->s.tl_count;
+  return CS_OK;
 }
 
 int cs_probe_hvp(cs_solver* s, const double* x, const double* p, double inv_h2, double* out) {

@@ -570,31 +570,52 @@ static napi_value SchwarzPartition(napi_env env, napi_callback_info info) {
   return o;
 }
 
-/* hvpChunks(scene) -> {lanes, maxDeg, records} (the mixed HVP's chunk layout;
- * no device needed) */
-static napi_value HvpChunks(napi_env env, napi_callback_info info) {
+/* hvpTiles(scene) -> {tiles, hdr, loc, lidx, off, ent} (the mixed HVP's
+ * vertex-owner tiles, cs_hvp_tiles; no device needed; tiles 0 when the mesh
+ * is not tiled) */
+static napi_value HvpTiles(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   args(env, info, 1, argv);
   cs_scene_desc d;
   if (scene_of(env, argv[0], &d)) return NULL;
-  uint64_t lanes = 0;
-  uint32_t md = 0;
-  if (cs_hvp_chunks(&d, NULL, 0, &lanes, &md) != CS_OK) return throw_status(env);
-  uint32_t* rec = (uint32_t*)malloc((size_t)(lanes ? lanes : 1) * 16);
-  if (!rec) {
+  uint64_t sz[5] = {0, 0, 0, 0, 0};
+  if (cs_hvp_tiles(&d, sz, NULL, NULL, NULL, NULL, NULL) != CS_OK) return throw_status(env);
+  uint32_t* hdr = (uint32_t*)malloc((size_t)(sz[0] ? sz[0] : 1) * 32);
+  uint32_t* loc = (uint32_t*)malloc((size_t)(sz[1] ? sz[1] : 1) * 4);
+  uint32_t* lidx = (uint32_t*)malloc((size_t)(sz[2] ? sz[2] : 1) * 8);
+  uint16_t* off = (uint16_t*)malloc((size_t)(sz[3] ? sz[3] : 1) * 2);
+  uint16_t* ent = (uint16_t*)malloc((size_t)(sz[4] ? sz[4] : 1) * 2);
+  int rc = CS_OK;
+  if (!hdr || !loc || !lidx || !off || !ent) {
+    free(hdr); free(loc); free(lidx); free(off); free(ent);
     napi_throw_error(env, NULL, "out of host memory");
     return NULL;
   }
-  const int rc = cs_hvp_chunks(&d, rec, lanes, &lanes, &md);
-  napi_value o = NULL, r, l, m;
-  if (rc == CS_OK && napi_create_object(env, &o) == napi_ok && (r = u32_array(env, rec, (size_t)lanes * 4)) &&
-      napi_create_double(env, (double)lanes, &l) == napi_ok && napi_create_uint32(env, md, &m) == napi_ok) {
-    napi_set_named_property(env, o, "records", r);
-    napi_set_named_property(env, o, "lanes", l);
-    napi_set_named_property(env, o, "maxDeg", m);
+  if (sz[0]) rc = cs_hvp_tiles(&d, sz, hdr, loc, lidx, off, ent);
+  napi_value o = NULL, t, a, b, c, e, f;
+  uint32_t* off32 = (uint32_t*)malloc((size_t)(sz[3] ? sz[3] : 1) * 4);
+  uint32_t* ent32 = (uint32_t*)malloc((size_t)(sz[4] ? sz[4] : 1) * 4);
+  if (off32 && ent32 && rc == CS_OK) {
+    for (uint64_t k = 0; k < sz[3]; k++) off32[k] = off[k];
+    for (uint64_t k = 0; k < sz[4]; k++) ent32[k] = ent[k];
+    if (napi_create_object(env, &o) == napi_ok && napi_create_double(env, (double)sz[0], &t) == napi_ok &&
+        (a = u32_array(env, hdr, (size_t)sz[0] * 8)) && (b = u32_array(env, loc, (size_t)sz[1])) &&
+        (c = u32_array(env, lidx, (size_t)sz[2] * 2)) && (e = u32_array(env, off32, (size_t)sz[3])) &&
+        (f = u32_array(env, ent32, (size_t)sz[4]))) {
+      napi_set_named_property(env, o, "tiles", t);
+      napi_set_named_property(env, o, "hdr", a);
+      napi_set_named_property(env, o, "loc", b);
+      napi_set_named_property(env, o, "lidx", c);
+      napi_set_named_property(env, o, "off", e);
+      napi_set_named_property(env, o, "ent", f);
+    }
   }
-  free(rec);
+  free(hdr); free(loc); free(lidx); free(off); free(ent); free(off32); free(ent32);
   if (rc != CS_OK) return throw_status(env);
+  if (!o) {
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
   return o;
 }
 
@@ -631,7 +652,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"setState", NULL, SetState, NULL, NULL, NULL, napi_enumerable, NULL},
       {"lastStats", NULL, LastStats, NULL, NULL, NULL, napi_enumerable, NULL},
       {"schwarzPartition", NULL, SchwarzPartition, NULL, NULL, NULL, napi_enumerable, NULL},
-      {"hvpChunks", NULL, HvpChunks, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"hvpTiles", NULL, HvpTiles, NULL, NULL, NULL, napi_enumerable, NULL},
       {"deviceInfo", NULL, DeviceInfo, NULL, NULL, NULL, napi_enumerable, NULL},
       {"abiVersion", NULL, AbiVersion, NULL, NULL, NULL, napi_enumerable, NULL},
   };

@@ -71,8 +71,9 @@ def load_library(path=LIB_PATH):
         "cs_coarse_pattern": (C.c_int, [C.c_void_p, _up, _up, C.c_uint64, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint32)]),
         "cs_probe_two_level": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
-        "cs_hvp_chunks": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_uint64),
-                                    C.POINTER(C.c_uint32)]),
+        "cs_hvp_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), _up, _up, _up, C.POINTER(C.c_uint16),
+                                   C.POINTER(C.c_uint16)]),
+        "cs_hvp_layout": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
         "cs_probe_schwarz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8),
                                        C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)]),
         "cs_probe_active_set": (C.c_int, [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), _dp, C.c_uint64,
@@ -189,6 +190,28 @@ def expand_pairs(pairs, idx):
     ids = np.concatenate(vt + ee).astype(np.int32)
     kinds = np.concatenate([np.zeros(6 * len(pairs), np.int32), np.ones(9 * len(pairs), np.int32)])
     return kinds, np.ascontiguousarray(ids)
+
+
+def hvp_tiles(scene):
+    """The vertex-owner tile layout of the mixed HVP for a scene (cs_hvp_tiles,
+    host only): dict of hdr (tiles x 8), loc, lidx (tile triangles x 2), off,
+    ent; None when the mesh is not tiled."""
+    L = load_library()
+    desc, keep = to_desc(scene)
+    sizes = (C.c_uint64 * 5)()
+    _check(L.cs_hvp_tiles(C.byref(desc), sizes, None, None, None, None, None))
+    if sizes[0] == 0:
+        return None
+    hdr = np.zeros((sizes[0], 8), np.uint32)
+    loc = np.zeros(sizes[1], np.uint32)
+    lidx = np.zeros((max(sizes[2], 1), 2), np.uint32)
+    off = np.zeros(sizes[3], np.uint16)
+    ent = np.zeros(max(sizes[4], 1), np.uint16)
+    _check(L.cs_hvp_tiles(C.byref(desc), sizes, hdr.ctypes.data_as(_up), loc.ctypes.data_as(_up),
+                          lidx.ctypes.data_as(_up), off.ctypes.data_as(C.POINTER(C.c_uint16)),
+                          ent.ctypes.data_as(C.POINTER(C.c_uint16))))
+    del keep
+    return dict(hdr=hdr, loc=loc, lidx=lidx[:sizes[2]], off=off, ent=ent[:sizes[4]])
 
 
 def ccd_queries(x0, x1, kinds, ids4, thickness, device=0, timing=False):
@@ -377,6 +400,14 @@ class HipSolver:
         b = C.c_double()
         _check(self._lib.cs_time_hvp(self._h, int(reps), C.byref(ms), C.byref(b)))
         return ms.value, b.value
+
+    def hvp_layout(self):
+        """Tile count of this solver's FP32 HVP (0: the per-vertex gather
+        kernel runs), cs_hvp_layout."""
+        self._need()
+        t = C.c_uint32()
+        _check(self._lib.cs_hvp_layout(self._h, C.byref(t)))
+        return t.value
 
     def tuned_launch(self):
         """Grid caps the launch autotuner chose for the contact kernels."""

@@ -321,15 +321,23 @@ int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double*
 int cs_schwarz_partition(const cs_scene_desc* scene, int target, int min_size, int max_size, uint32_t* domain_of,
                          uint32_t* local_of, uint32_t* domains);
 
-/* (no reference counterpart; no device needed) The mixed HVP's chunk layout
- * for a mesh: the vertex->corner records of incidence.ts:41-93 (ascending
- * element, each as i0, i1, i2, tri<<2|corner) regrouped into 64-lane chunks
- * of whole vertices, padding lanes 0xffffffff in the last word, a vertex
- * without triangles one lane (v, v, v, 0xfffffffe).  `lanes` receives the
- * lane count (a multiple of 64; 0 when a vertex has more than 64 corners and
- * the chunked kernel is not used), `maxdeg` the largest lane count of one
- * vertex; up to `cap` records (4 words each) go to `records` when non-null. */
-int cs_hvp_chunks(const cs_scene_desc* scene, uint32_t* records, uint64_t cap, uint64_t* lanes, uint32_t* maxdeg);
+/* Layout probe (host only, no device): the vertex-owner tiles of the mixed
+ * solve's HVP.  Vertices in Morton order of the rest positions are taken
+ * greedily into tiles of at most 256 owned vertices, 640 triangles (every
+ * triangle with an owned corner, ascending) and 384 local vertices (the owned
+ * ones first, then the others in order of appearance).  sizes[5] receives
+ * (tiles, local vertices, tile triangles, owned-vertex offsets, entries), all
+ * 0 when some vertex alone exceeds a tile; when all five buffers are given
+ * they receive, per tile, 8 u32 (local begin, owned count, local count,
+ * triangle begin, triangle count, entry begin, offset begin, 0), the local
+ * vertex ids, per tile triangle 2 u32 (a | b << 16, c) of local corners, and
+ * per owned vertex its first entry (one extra per tile) and the entries
+ * slot * 4 + corner. */
+int cs_hvp_tiles(const cs_scene_desc* scene, uint64_t* sizes, uint32_t* hdr, uint32_t* loc, uint32_t* lidx,
+                 uint16_t* off, uint16_t* ent);
+/* The HVP layout a mixed solver uses: its tile count (0: not tiled, the
+ * per-vertex gather kernel runs). */
+int cs_hvp_layout(cs_solver* s, uint32_t* tiles);
 
 /* Component probe (membraneHvp with the analytic Hessian,
  * membrane-hvp.ts:18-108, plus the inertia term): one launch of the mixed

@@ -2734,6 +2734,18 @@ int64_t orc_contact_hvp(orc_solver* o, const double* x0, const double* x, const 
   return (int64_t)s.contact.active.size();
 }
 
+// The contact gradient (energyAndGradient, contact-assembly.ts:262-367) at y
+// of the active set updateActiveSet froze at x after beginStep(x0).
+int64_t orc_contact_grad_frozen(orc_solver* o, const double* x0, const double* x, const double* y, double* grad) {
+  Solver& s = o->s;
+  if (!s.hasContact) return -1;
+  const size_t n3 = (size_t)s.mesh.n * 3;
+  s.contact.begin_step(std::vector<double>(x0, x0 + n3));
+  s.contact.update_active_set(x, s.mesh.n);
+  s.contact.energy_grad(y, grad, n3);
+  return (int64_t)s.contact.active.size();
+}
+
 // buildCandidates (contact-assembly.ts:87-121) at x with the given pad after beginStep(x0).
 int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap) {
   Solver& s = o->s;

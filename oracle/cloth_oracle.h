@@ -90,6 +90,7 @@ int orc_coarse_solve(orc_solver* o, int target, int minsz, int maxsz, const doub
 int orc_two_level_apply(orc_solver* o, int target, int minsz, int maxsz, const double* x, const double* diag,
                         int mode, int iters, double omega, const double* r, double* z);
 int64_t orc_contact_hvp(orc_solver* o, const double* x0, const double* x, const double* p, double* out);
+int64_t orc_contact_grad_frozen(orc_solver* o, const double* x0, const double* x, const double* y, double* grad);
 int64_t orc_candidates(orc_solver* o, const double* x0, const double* x, int pad_is_dmin, int32_t* ids6, int64_t cap);
 int64_t orc_contact_probe(orc_solver* s, const double* x0, const double* x, double pad_is_dmin,
                           int32_t* ids6, double* vals7, int64_t cap, int64_t* n_candidates);

@@ -73,6 +73,7 @@ def lib():
             "orc_check_trial": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
             "orc_candidates": (C.c_int64, [C.c_void_p, _dp, _dp, C.c_int, _ip, C.c_int64]),
             "orc_contact_hvp": (C.c_int64, [C.c_void_p, _dp, _dp, _dp, _dp]),
+            "orc_contact_grad_frozen": (C.c_int64, [C.c_void_p, _dp, _dp, _dp, _dp]),
             "orc_coarse_pattern": (C.c_int64, [C.c_void_p, C.c_int, C.c_int, C.c_int, _up, _up, C.c_int64]),
             "orc_coarse_values": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]),
             "orc_coarse_transfer": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]),
@@ -280,6 +281,15 @@ class OracleSolver:
         z = np.zeros(self.n3)
         lib().orc_two_level_apply(self.h, *dom, P(d3(x)), P(d3(diag)), mode, iters, omega, P(d3(r)), P(z))
         return z
+
+    def contact_grad_frozen(self, x0, x, y):
+        """(contact gradient at y of the active set frozen at x after
+        beginStep(x0), active count)."""
+        out = np.zeros(self.n3)
+        k = lib().orc_contact_grad_frozen(self.h, P(d3(x0)), P(d3(x)), P(d3(y)), P(out))
+        if k < 0:
+            raise RuntimeError("scene has no contact system")
+        return out, int(k)
 
     def candidates(self, x0, x, dmin=False, cap=None):
         """buildCandidates (contact-assembly.ts:87-121) at x after beginStep(x0),

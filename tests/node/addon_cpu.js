@@ -50,8 +50,7 @@ const part = addon.schwarzPartition(packScene(scene), 16, 8, 32);
 assert.strictEqual(part.count, expected.schwarz.count);
 assert.deepStrictEqual(Array.from(part.domainOf), expected.schwarz.domainOf);
 assert.deepStrictEqual(Array.from(part.localOf), expected.schwarz.localOf);
-const ck = addon.hvpChunks(packScene(scene));
-assert.strictEqual(ck.lanes, expected.chunks.lanes);
-assert.strictEqual(ck.maxDeg, expected.chunks.maxDeg);
-assert.deepStrictEqual(Array.from(ck.records), expected.chunks.records);
+const tl = addon.hvpTiles(packScene(scene));
+assert.strictEqual(tl.tiles, expected.tiles.tiles);
+for (const k of ["hdr", "loc", "lidx", "off", "ent"]) assert.deepStrictEqual(Array.from(tl[k]), expected.tiles[k]);
 console.log("addon cpu checks OK");

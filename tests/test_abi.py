@@ -143,19 +143,6 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in r.stderr
 
 
-def _chunks(scene):
-    from closim_amd.scene import to_desc
-    from closim_amd.solver import load_library
-    L = load_library()
-    desc, keep = to_desc(scene)
-    lanes, md = C.c_uint64(), C.c_uint32()
-    assert L.cs_hvp_chunks(C.byref(desc), None, 0, C.byref(lanes), C.byref(md)) == _abi.CS_OK
-    rec = np.zeros((max(lanes.value, 1), 4), np.uint32)
-    assert L.cs_hvp_chunks(C.byref(desc), rec.ctypes.data_as(C.POINTER(C.c_uint32)), lanes.value,
-                           C.byref(lanes), C.byref(md)) == _abi.CS_OK
-    return rec[:lanes.value], md.value
-
-
 def _fan(rim):
     """A disk: one centre vertex shared by `rim` triangles."""
     from closim_amd.mesh import preprocess
@@ -166,48 +153,6 @@ def _fan(rim):
     uv = np.concatenate([[0.0, 0.0], np.stack([np.cos(a), np.sin(a)], 1).ravel()])
     idx = np.array([[0, 1 + k, 1 + (k + 1) % rim] for k in range(rim)], np.uint32).ravel()
     return create_scene(preprocess(pos, uv, idx, STRIP_MATERIAL.arealDensityKgM2), STRIP_MATERIAL)
-
-
-@pytest.mark.parametrize("make", ["small_strip", "strip_pull", "sphere_drape", "fan40"])
-def test_hvp_chunk_layout(make):
-    """The mixed HVP's corner chunks (no device needed): every CSR corner
-    record exactly once, each vertex's records contiguous, in CSR order and
-    inside one 64-lane chunk, padding only at chunk ends."""
-    from closim_amd import scenes as S
-    sc = _fan(40) if make == "fan40" else getattr(S, make)()
-    rec, md = _chunks(sc)
-    n, idx = sc.mesh.count, np.asarray(sc.mesh.indices, np.int64).reshape(-1, 3)
-    assert len(rec) % 64 == 0 and len(rec) > 0
-    # the CSR of incidence.ts: records of vertex v in ascending element order
-    csr = [[] for _ in range(n)]
-    for t, tri in enumerate(idx):
-        for c in range(3):
-            csr[tri[c]].append((t << 2) | c)
-    assert md == max(max(len(c), 1) for c in csr)
-    r64 = rec.astype(np.int64)
-    pad = r64[:, 3] == 0xFFFFFFFF
-    own = np.where(pad, -1, np.where(r64[:, 3] == 0xFFFFFFFE, r64[:, 0],
-                                     np.choose(np.minimum(r64[:, 3] & 3, 2), [r64[:, 0], r64[:, 1], r64[:, 2]])))
-    seen = 0
-    for ch in range(len(rec) // 64):
-        lanes = own[ch * 64:(ch + 1) * 64]
-        real = np.nonzero(lanes >= 0)[0]
-        assert np.all(lanes[real[-1] + 1:] == -1) if len(real) else True  # padding only at the end
-        for v in np.unique(lanes[real]):
-            at = np.nonzero(lanes == v)[0]
-            assert np.all(np.diff(at) == 1)  # contiguous
-            recs = [int(r) for r in rec[ch * 64 + at, 3]]
-            assert recs == (csr[v] if csr[v] else [0xFFFFFFFE])
-            for r in rec[ch * 64 + at]:
-                if r[3] != 0xFFFFFFFE:
-                    assert list(r[:3]) == list(idx[r[3] >> 2])
-            seen += 1
-    assert seen == n  # every vertex in exactly one chunk
-
-
-def test_hvp_chunks_refuse_a_vertex_of_more_than_64_corners():
-    rec, md = _chunks(_fan(70))
-    assert len(rec) == 0 and md == 0
 
 
 def test_broadphase_arguments_are_rejected_without_a_device():

@@ -142,14 +142,15 @@ def test_graph_replay_matches_eager_launches(make, steps, precond):
 
 @pytest.mark.parametrize("make", [S.small_strip, S.floor_patch, crumpled_sheet, S.sphere_drape])
 def test_mixed_hvp_kernel_matches_oracle_analytic_hvp(make):
-    # The mixed solve's HVP kernel (chunked corner records, prepared
-    # per-triangle F/S, segmented per-vertex sums) at a stretched state
+    # The mixed solve's HVP kernel (vertex-owner tiles: each triangle once
+    # into LDS, per-vertex sums in triangle order) at a stretched state
     # against the oracle's analytic membrane HVP plus M p / h^2.  Inputs are
     # rounded to FP32 first, so the bar measures the kernel's FP32 arithmetic,
     # not the rounding of the positions.
     sc = make()
     g = HipSolver("mixed")
     g.initialize(sc)
+    assert g.hvp_layout() > 0
     o = O.OracleSolver(make())
     rng = np.random.default_rng(21)
     x = sc.positions * 1.01 + rng.standard_normal(len(sc.positions)) * 1e-4
@@ -202,17 +203,20 @@ def test_mixed_contact_hvp_matches_oracle_apply_hvp(name):
     g.close()
 
 
-def test_mixed_hvp_fallback_kernel_on_a_70_triangle_fan():
-    # A vertex with 70 corners does not fit one 64-lane chunk, so the mixed
-    # solve uses the per-vertex gather kernel (8 lanes per vertex over the
-    # CSR); it must give the analytic HVP as the chunked kernel does.
-    from test_abi import _chunks, _fan
-    sc = _fan(70)
-    assert len(_chunks(sc)[0]) == 0  # no chunk layout: the fallback kernel runs
+@pytest.mark.parametrize("rim", [70, 650])
+def test_mixed_hvp_on_fans(rim):
+    # A vertex with 70 corners fits a tile (the tile kernel runs); one with
+    # 650 does not, and the mixed solve uses the per-vertex gather kernel (8
+    # lanes per vertex over the CSR).
+    # Both must give the analytic HVP.
+    from test_abi import _fan
+    sc = _fan(rim)
     g = HipSolver("mixed")
     g.initialize(sc)
-    o = O.OracleSolver(_fan(70))
-    rng = np.random.default_rng(70)
+    tiles = g.hvp_layout()
+    assert (tiles > 0) == (rim == 70)
+    o = O.OracleSolver(_fan(rim))
+    rng = np.random.default_rng(rim)
     x = sc.positions * 1.02 + rng.standard_normal(len(sc.positions)) * 1e-3
     x = x.astype(np.float32).astype(np.float64)
     p = rng.standard_normal(len(x)).astype(np.float32).astype(np.float64)
@@ -221,6 +225,7 @@ def test_mixed_hvp_fallback_kernel_on_a_70_triangle_fan():
     m = np.repeat(np.asarray(sc.mesh.masses, np.float64), 3)
     want = o.membrane_hvp(x, p, analytic=True) + m * p * inv_h2
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
-    print(f"fan70: FP32 HVP (fallback kernel) rel err {err:.2e}", flush=True)
-    assert err < 1e-6
+    print(f"fan{rim}: FP32 HVP ({'tiles' if tiles else 'per-vertex gather'}) rel err {err:.2e}", flush=True)
+    # FP32 sums of `rim` corner terms at the centre vertex
+    assert err < (1e-6 if rim == 70 else 1e-5)
     g.close()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from closim_amd import scenes as S
-from closim_amd.solver import load_library, schwarz_partition
+from closim_amd.solver import hvp_tiles, load_library, schwarz_partition
 from node_scene import write_scene
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -20,29 +20,16 @@ ADDON = os.path.join(ROOT, "closim_amd", "_build", "hip_solver_addon.node")
 pytestmark = pytest.mark.skipif(shutil.which("node") is None, reason="no node in this image")
 
 
-def _chunks(scene):
-    import ctypes as C
-    from closim_amd import _abi
-    from closim_amd.solver import to_desc
-    L = load_library()
-    desc, keep = to_desc(scene)
-    lanes, md = C.c_uint64(), C.c_uint32()
-    assert L.cs_hvp_chunks(C.byref(desc), None, 0, C.byref(lanes), C.byref(md)) == _abi.CS_OK
-    rec = np.zeros((max(lanes.value, 1), 4), np.uint32)
-    assert L.cs_hvp_chunks(C.byref(desc), rec.ctypes.data_as(C.POINTER(C.c_uint32)), lanes.value,
-                           C.byref(lanes), C.byref(md)) == _abi.CS_OK
-    return rec[:lanes.value], md.value
-
-
 def test_addon_checks_on_the_cpu(tmp_path):
     assert os.path.exists(ADDON), "addon not built (closim_amd/node/build_addon.sh)"
     sc = S.strip_pull(9, 29, 0.1, 0.3)
     write_scene(sc, tmp_path / "scene.json")
     dom, loc, cnt = schwarz_partition(sc, 16, 8, 32)
-    rec, md = _chunks(sc)
+    tl = hvp_tiles(sc)
     with open(tmp_path / "expected.json", "w") as f:
         json.dump(dict(schwarz=dict(count=int(cnt), domainOf=[int(v) for v in dom], localOf=[int(v) for v in loc]),
-                       chunks=dict(lanes=int(len(rec)), maxDeg=int(md), records=[int(v) for v in rec.ravel()])), f)
+                       tiles=dict(tiles=int(len(tl["hdr"])), **{k: [int(v) for v in tl[k].ravel()]
+                                                                 for k in ("hdr", "loc", "lidx", "off", "ent")})), f)
     env = dict(os.environ, HIP_VISIBLE_DEVICES="")  # no device even on a GPU host
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "node", "addon_cpu.js"), str(tmp_path / "scene.json"),
                         str(tmp_path / "expected.json")], capture_output=True, text=True, env=env, timeout=120)

@@ -220,3 +220,38 @@ def test_self_pairs_equal_brute_force(seed):
         got = O.self_pairs(x0, x1, idx, pad)
         ref = brute_pairs(x0, x1, idx, pad)
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("name", ["stacked", "floor", "crumpled"])
+def test_frozen_contact_hvp_is_the_derivative_of_the_frozen_contact_gradient(name):
+    # applyHvp (contact-assembly.ts:371-392) is kappa b''(d) (J.p) J^T for the
+    # frozen contacts, and the frozen distance is linear in x (frozen normal
+    # and weights), so without friction it is exactly the derivative of the
+    # frozen contact gradient: central differences along p agree to
+    # truncation (b is smooth inside dHat) and the planted sign is caught
+    from dataclasses import replace
+    from closim_amd import _abi
+    from closim_amd.scene import ContactSystem
+    import contact_scenes as CS
+    sc = {"stacked": lambda: CS.stacked_patches(mu=0.0), "floor": lambda: CS.floor_close(),
+          "crumpled": lambda: CS.crumpled_sheet()}[name]()
+    c = sc.contact
+    params = replace(c.params, frictionMu=0.0)
+    nc = ContactSystem(params)
+    if c.floorY is not None:
+        nc.setFloor(c.floorY)
+    sc.contact = nc
+    o = O.OracleSolver(sc, _abi.options())
+    x = sc.positions.copy()
+    rng = np.random.default_rng(11)
+    p = rng.standard_normal(len(x))
+    hvp, k = o.contact_hvp(x, x, p)
+    assert k > 0 and np.any(hvp != 0)
+    eps = 1e-9
+    gp, _ = o.contact_grad_frozen(x, x, x + eps * p)
+    gm, _ = o.contact_grad_frozen(x, x, x - eps * p)
+    fd = (gp - gm) / (2 * eps)
+    err = np.max(np.abs(fd - hvp)) / np.max(np.abs(hvp))
+    print(f"{name}: {k} active, FD rel err {err:.1e}")
+    assert err < 1e-5
+    assert np.max(np.abs(fd + hvp)) / np.max(np.abs(hvp)) > 1.0  # a flipped sign fails
