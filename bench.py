@@ -248,8 +248,10 @@ def batch_section(steps, h, threads=8, count=64):
     """The batched uneven drape (`scenes.batch_drapes`: 64 sheets of 1e3..1e5
     vertices, each on its own sphere and floor): all instances stepped
     together in mixed precision (cs_batch_step, `threads` host threads, one
-    HIP stream per instance) against one instance at a time in FP64, both
-    from the same initial states after one warmup step, block-Jacobi."""
+    HIP stream per instance), the same with the inner solves packed
+    (cs_batch_step_packed, identical trajectories), and one instance at a time
+    in FP64, all from the same initial states after one warmup step,
+    block-Jacobi."""
     from closim_amd import _abi, scenes as S
     from closim_amd.solver import HipBatch
     popts = dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI)
@@ -269,6 +271,21 @@ def batch_section(steps, h, threads=8, count=64):
     hip_sync()
     t_b = time.perf_counter() - t0
     print(f"[bench] batch mixed: {steps} steps of {count} instances in {t_b:.3f} s", file=sys.stderr, flush=True)
+    scenes_p = S.batch_drapes(count)
+    gp = [solver_for(sc, "mixed", popts, timing=False) for sc in scenes_p]
+    P = HipBatch(gp, packed=True)
+    P.step(h)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.step(h)
+    hip_sync()
+    t_p = time.perf_counter() - t0
+    same = all(np.array_equal(a.positions, b.positions) for a, b in zip(scenes_p, scenes_m))
+    print(f"[bench] batch mixed packed: {steps} steps in {t_p:.3f} s (trajectories identical: {same})",
+          file=sys.stderr, flush=True)
+    for g in gp:
+        g.close()
     for g in gd:
         raw_step(g, h)
     hip_sync()
@@ -289,6 +306,8 @@ def batch_section(steps, h, threads=8, count=64):
                 preconditioner="block-jacobi", warmup=1, steps=steps,
                 mixed_batched=dict(batch_steps_per_s=steps / t_b, instance_steps_per_s=steps * count / t_b,
                                    vertex_steps_per_s=steps * verts / t_b),
+                mixed_packed=dict(batch_steps_per_s=steps / t_p, instance_steps_per_s=steps * count / t_p,
+                                  identical_to_batched=same),
                 fp64_one_at_a_time=dict(batch_steps_per_s=steps / t_s, instance_steps_per_s=steps * count / t_s),
                 speedup=t_s / t_b, pos_rel_err_mixed_vs_fp64_max=err)
 

@@ -3,10 +3,22 @@
 // (the Newton gradient, energies, CCD and the Armijo test stay FP64).
 #pragma once
 
+#include <vector>
+
 #include "../../include/cloth_hip.h"
 #include "pcg.h"
 
 namespace cloth {
+
+// Device tables of the packed multi-instance solve (kept across solves).
+struct PackedScratch {
+  void* args = nullptr;  // MxArgs per instance
+  void* tabs = nullptr;  // block tables
+  int* caps = nullptr;   // iteration cap per instance
+  int2* flags = nullptr; // (done, iters) per instance
+  size_t args_bytes = 0, tabs_bytes = 0, n_inst = 0;
+  ~PackedScratch();
+};
 
 struct MixedInputs {
   DMesh me;
@@ -26,6 +38,17 @@ struct MixedInputs {
 };
 
 struct MixedImpl;
+struct PackedScratch;
+class MixedSolve;
+
+// One instance's system in a packed solve (solve_packed): its solver, its
+// inputs (Jacobi or block-Jacobi preconditioner; no Schwarz) and, on return,
+// its PCG iteration count.
+struct PackedJob {
+  MixedSolve* solver;
+  MixedInputs in;
+  int iters = 0;
+};
 
 class MixedSolve {
  public:
@@ -45,6 +68,12 @@ class MixedSolve {
   // barrier term of its active set in FP32 as the solve applies it.
   void probe_hvp(const DMesh& me, const DMat& mat, double inv_h2, const double* x, const double* p, double* out,
                  const DContactHvp* contact = nullptr);
+  // Solves several instances' systems together on `st`: every PCG stage is
+  // one launch over all instances' blocks (a block table maps each block to
+  // its instance, local block and local grid), each instance with its own
+  // arithmetic, partial slots and reduction trees, so every result is the
+  // one solve_plain gives; `scratch` holds the tables between calls.
+  static void solve_packed(std::vector<PackedJob>& jobs, hipStream_t st, PackedScratch& scratch);
   void set_material(const cs_material&) {}
 
  private:

@@ -75,9 +75,9 @@ __device__ __forceinline__ float block_sum_f(float v) {
   return r;
 }
 
-__device__ __forceinline__ void block_partial_f(float v, float* partials) {
+__device__ __forceinline__ void block_partial_f(float v, float* partials, uint32_t bid) {
   const float b = block_sum_f(v);
-  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+  if (threadIdx.x == 0) partials[bid] = b;
 }
 
 __device__ __forceinline__ float merge_partials_f(const float* partials, unsigned count) {
@@ -224,9 +224,9 @@ __global__ void k_mx_contacts(const Frozen* __restrict__ act, const uint8_t* __r
   out[i] = o;
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_init(MxArgs a) {
+__device__ __forceinline__ void k_mx_init_body(const MxArgs& a, uint32_t bid, uint32_t nb) {
   float rz = 0, bb = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = bid * blockDim.x + threadIdx.x; v < a.me.n; v += nb * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
     float r3[3], z3[3];
     ld3f(a.r, v, r3);
@@ -244,13 +244,16 @@ __global__ void __launch_bounds__(kBlock) k_mx_init(MxArgs a) {
 #pragma unroll
     for (int i = 0; i < 3; i++) bb += r3[i] * r3[i];
   }
-  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ));
-  block_partial_f(bb, mpart(a, MX_BB));
+  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ), bid);
+  block_partial_f(bb, mpart(a, MX_BB), bid);
 }
+
+__global__ void __launch_bounds__(kBlock) k_mx_init(MxArgs a) { k_mx_init_body(a, blockIdx.x, gridDim.x); }
 
 // z from the Schwarz apply: pin filter, p = z (init), r.z partials.
 __global__ void __launch_bounds__(kBlock) k_mx_ext_z(MxArgs a, int init) {
   if (!init && block_latched(&a.s->done)) return;
+  const uint32_t bid = blockIdx.x;
   float rz = 0;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
@@ -263,10 +266,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_ext_z(MxArgs a, int init) {
       rz += a.r[j] * zi;
     }
   }
-  block_partial_f(rz, mpart(a, MX_RZ));
+  block_partial_f(rz, mpart(a, MX_RZ), bid);
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) {
+__device__ __forceinline__ void k_mx_decide_body(const MxArgs& a, unsigned count, uint32_t bid, uint32_t nb) {
   const float rz = merge_partials_f(mpart(a, MX_RZ), count);
   const float bb = merge_partials_f(mpart(a, MX_BB), count);
   if (threadIdx.x == 0) {
@@ -279,13 +282,17 @@ __global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) 
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_mx_decide(MxArgs a, unsigned count) {
+  k_mx_decide_body(a, count, blockIdx.x, gridDim.x);
+}
+
 // Contact HVP terms of all incidence entries (one thread each) and, within
 // each wave's 64 entries, their running sums per vertex (entries are sorted
 // by vertex): the sum of a vertex's entries is then the last running sum in
 // each 64-entry chunk its range touches.
-__global__ void __launch_bounds__(kBlock) k_mx_cscan(MxArgs a) {
+__device__ __forceinline__ void k_mx_cscan_body(const MxArgs& a, uint32_t bid, uint32_t nb) {
   if (block_latched(&a.s->done)) return;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t e = bid * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x % kWave;
   float t[3] = {0.f, 0.f, 0.f};
   uint32_t u = 0xffffffffu;
@@ -304,18 +311,20 @@ __global__ void __launch_bounds__(kBlock) k_mx_cscan(MxArgs a) {
     for (int i = 0; i < 3; i++) a.scan[(size_t)e * 3 + i] = t[i];
 }
 
+__global__ void __launch_bounds__(kBlock) k_mx_cscan(MxArgs a) { k_mx_cscan_body(a, blockIdx.x, gridDim.x); }
+
 constexpr int kMxLanes = 8;  // lanes per vertex in the HVP gather
 constexpr int kMxVertsPerBlock = kBlock / kMxLanes;
 
 // Ap = M p / h^2 + H_membrane p + contact, pins zeroed.  The 8 lanes of a
 // vertex group evaluate the incident corners (and contact entries) in
 // parallel; an xor-butterfly sums them (fixed order, deterministic).
-__global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
+__device__ __forceinline__ void k_mx_hvp_body(const MxArgs& a, int k, uint32_t bid, uint32_t nb) {
   if (block_latched(&a.s->done)) return;
   float pap = 0;
   const int sub = threadIdx.x % kMxLanes;
-  const uint32_t stride = gridDim.x * kMxVertsPerBlock;
-  for (uint32_t v0 = blockIdx.x * kMxVertsPerBlock; v0 < a.me.n; v0 += stride) {
+  const uint32_t stride = nb * kMxVertsPerBlock;
+  for (uint32_t v0 = bid * kMxVertsPerBlock; v0 < a.me.n; v0 += stride) {
     const uint32_t v = v0 + threadIdx.x / kMxLanes;
     const bool valid = v < a.me.n;
     float acc[3] = {0.f, 0.f, 0.f};
@@ -353,8 +362,10 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
       }
     }
   }
-  block_partial_f(pap, mpart(a, MX_PAP));
+  block_partial_f(pap, mpart(a, MX_PAP), bid);
 }
+
+__global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) { k_mx_hvp_body(a, k, blockIdx.x, gridDim.x); }
 
 // The same product over vertex-owner tiles: the work-group stages its local
 // vertices' x and p in LDS, evaluates every tile triangle once (all three
@@ -362,12 +373,12 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp(MxArgs a, int k) {
 // owned vertex sums its corners in ascending triangle order, adds its contact
 // sums and the mass term and writes Ap.  Tiles are in Morton order; each XCD
 // walks a contiguous eighth of them.
-__global__ void __launch_bounds__(kBlock) k_mx_hvp_tile(MxArgs a) {
+__device__ __forceinline__ void k_mx_hvp_tile_body(const MxArgs& a, uint32_t bid, uint32_t nb) {
   if (__hip_atomic_load(&a.s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
   __shared__ float xs[kTileLoc * 3], ps[kTileLoc * 3], out[kTileTris * 9];
   const DTiles& T = a.me.tl;
-  const uint32_t G = gridDim.x;
-  const uint32_t lb = (blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds;
+  const uint32_t G = nb;
+  const uint32_t lb = (bid % kXcds) * (G / kXcds) + bid / kXcds;
   float pap = 0;
   for (uint32_t tile = lb; tile < T.count; tile += G) {
     const TileHdr h = T.hdr[tile];
@@ -452,15 +463,19 @@ __global__ void __launch_bounds__(kBlock) k_mx_hvp_tile(MxArgs a) {
       }
     }
   }
-  block_partial_f(pap, mpart(a, MX_PAP));
+  block_partial_f(pap, mpart(a, MX_PAP), bid);
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned pap_count) {
+__global__ void __launch_bounds__(kBlock) k_mx_hvp_tile(MxArgs a) {
+  k_mx_hvp_tile_body(a, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void k_mx_update_body(const MxArgs& a, int k, unsigned pap_count, uint32_t bid, uint32_t nb) {
   MxScalars* s = a.s;
   if (block_latched(&s->done)) return;
   const float pAp = merge_partials_f(mpart(a, MX_PAP), pap_count);
   if (!(pAp > 1e-30f)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       s->pAp = pAp;
       s->done = 1;
       s->iters = k;
@@ -469,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned 
   }
   const float alpha = s->rz[k & 1] / pAp;
   float rr = 0, rz = 0;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = bid * blockDim.x + threadIdx.x; v < a.me.n; v += nb * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
     float r3[3], z3[3];
 #pragma unroll
@@ -493,25 +508,29 @@ __global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned 
       }
     }
   }
-  block_partial_f(rr, mpart(a, MX_RR));
-  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ));
+  block_partial_f(rr, mpart(a, MX_RR), bid);
+  if (!a.ext) block_partial_f(rz, mpart(a, MX_RZ), bid);
 }
 
-__global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a, int k) {
+__global__ void __launch_bounds__(kBlock) k_mx_update(MxArgs a, int k, unsigned pap_count) {
+  k_mx_update_body(a, k, pap_count, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void k_mx_dir_body(const MxArgs& a, int k, uint32_t bid, uint32_t nb) {
   MxScalars* s = a.s;
   if (block_latched(&s->done)) return;
-  const float residual = sqrtf(merge_partials_f(mpart(a, MX_RR), gridDim.x));
+  const float residual = sqrtf(merge_partials_f(mpart(a, MX_RR), nb));
   if (residual < s->abs_tol) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       s->residual = residual;
       s->done = 2;
       s->iters = k + 1;
     }
     return;
   }
-  const float rznew = merge_partials_f(mpart(a, MX_RZ), gridDim.x);
+  const float rznew = merge_partials_f(mpart(a, MX_RZ), nb);
   const float beta = rznew / s->rz[k & 1];
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.me.n; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = bid * blockDim.x + threadIdx.x; v < a.me.n; v += nb * blockDim.x) {
     const bool pinned = a.me.pinmask[v] != 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -519,11 +538,13 @@ __global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a, int k) {
       a.p[j] = pinned ? 0.f : a.z[j] + beta * a.p[j];
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     s->rz[(k + 1) & 1] = rznew;
     s->residual = residual;
   }
 }
+
+__global__ void __launch_bounds__(kBlock) k_mx_dir(MxArgs a, int k) { k_mx_dir_body(a, k, blockIdx.x, gridDim.x); }
 
 __global__ void k_mx_out(const float* sol, double* dx, size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -572,9 +593,9 @@ MixedSolve::~MixedSolve() {
   delete im_;
 }
 
-int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, bool), void* ctx) {
-  MixedImpl& M = *im_;
-  hipStream_t st = M.st;
+// The per-solve set-up of solve_plain: FP32 copies of x, r = b, the diagonal
+// and the block inverses, the FP32 contact records, and the kernel arguments.
+static MxArgs prepare_args(MixedImpl& M, const MixedInputs& in, hipStream_t st) {
   const uint32_t n = in.me.n;
   if (in.binv) M.binv32.alloc((size_t)n * 9);
   const uint32_t C = in.contact.act ? in.contact.C : 0;
@@ -611,6 +632,15 @@ int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, boo
   a.v_ent = in.contact.v_ent;
   M.last = a;
   M.has_last = true;
+  return a;
+}
+
+int MixedSolve::solve_plain(const MixedInputs& in, void (*timer)(void*, int, bool), void* ctx) {
+  MixedImpl& M = *im_;
+  hipStream_t st = M.st;
+  const uint32_t n = in.me.n;
+  const MxArgs a = prepare_args(M, in, st);
+  const uint32_t E = a.E;
   const dim3 g(mx_grid(n)), b(kBlock);
   hipLaunchKernelGGL(k_mx_init, g, b, 0, st, a);
   if (in.sw) {
@@ -762,6 +792,148 @@ double MixedSolve::time_hvp(int reps) {
   CS_HIP(hipMemcpyAsync(&M.s.ptr->done, &one, sizeof(one), hipMemcpyHostToDevice, st));
   CS_HIP(hipStreamSynchronize(st));
   return (double)ms / reps;
+}
+
+// ---- packed multi-instance solve --------------------------------------------
+// Block table entry: instance, local block, local grid, and a count the stage
+// needs (the partial count a merge reads).
+struct PkEntry {
+  uint32_t inst, bid, nb, aux;
+};
+
+__device__ __forceinline__ bool pk_capped(const int* caps, uint32_t inst, int k) { return k >= caps[inst]; }
+
+__global__ void __launch_bounds__(kBlock) k_pk_init(const MxArgs* A, const PkEntry* tab) {
+  const PkEntry e = tab[blockIdx.x];
+  k_mx_init_body(A[e.inst], e.bid, e.nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pk_decide(const MxArgs* A, const PkEntry* tab) {
+  const PkEntry e = tab[blockIdx.x];
+  k_mx_decide_body(A[e.inst], e.aux, e.bid, e.nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pk_cscan(const MxArgs* A, const PkEntry* tab, const int* caps, int k) {
+  const PkEntry e = tab[blockIdx.x];
+  if (pk_capped(caps, e.inst, k)) return;
+  k_mx_cscan_body(A[e.inst], e.bid, e.nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pk_hvp(const MxArgs* A, const PkEntry* tab, const int* caps, int k) {
+  const PkEntry e = tab[blockIdx.x];
+  if (pk_capped(caps, e.inst, k)) return;
+  const MxArgs& a = A[e.inst];
+  if (a.me.tl.count) k_mx_hvp_tile_body(a, e.bid, e.nb);
+  else k_mx_hvp_body(a, 0, e.bid, e.nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pk_update(const MxArgs* A, const PkEntry* tab, const int* caps, int k) {
+  const PkEntry e = tab[blockIdx.x];
+  if (pk_capped(caps, e.inst, k)) return;
+  k_mx_update_body(A[e.inst], k, e.aux, e.bid, e.nb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pk_dir(const MxArgs* A, const PkEntry* tab, const int* caps, int k) {
+  const PkEntry e = tab[blockIdx.x];
+  if (pk_capped(caps, e.inst, k)) return;
+  k_mx_dir_body(A[e.inst], k, e.bid, e.nb);
+}
+
+__global__ void k_pk_flags(const MxArgs* A, uint32_t count, int2* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = make_int2(A[i].s->done, A[i].s->iters);
+}
+
+PackedScratch::~PackedScratch() {
+  if (args) (void)hipFree(args);
+  if (tabs) (void)hipFree(tabs);
+  if (caps) (void)hipFree(caps);
+  if (flags) (void)hipFree(flags);
+}
+
+void MixedSolve::solve_packed(std::vector<PackedJob>& jobs, hipStream_t st, PackedScratch& ps) {
+  const size_t ni = jobs.size();
+  if (!ni) return;
+  std::vector<MxArgs> A(ni);
+  std::vector<int> caps(ni);
+  int kmax = 0;
+  for (size_t i = 0; i < ni; i++) {
+    const MixedInputs& in = jobs[i].in;
+    if (in.sw) throw std::invalid_argument("a packed solve takes Jacobi or block-Jacobi systems");
+    A[i] = prepare_args(*jobs[i].solver->im_, in, st);
+    caps[i] = in.max_iters;
+    kmax = std::max(kmax, in.max_iters);
+  }
+  // block tables: vector stages, one block per instance, contact scans, HVP
+  std::vector<PkEntry> vec, one, scan, hvp, upd;
+  for (uint32_t i = 0; i < (uint32_t)ni; i++) {
+    const uint32_t gv = mx_grid(A[i].me.n), gh = mx_hvp_grid(A[i].me), gs = A[i].E ? grid_for(A[i].E) : 0;
+    for (uint32_t b = 0; b < gv; b++) {
+      vec.push_back(PkEntry{i, b, gv, 0});
+      upd.push_back(PkEntry{i, b, gv, gh});
+    }
+    one.push_back(PkEntry{i, 0, 1, gv});
+    for (uint32_t b = 0; b < gs; b++) scan.push_back(PkEntry{i, b, gs, 0});
+    for (uint32_t b = 0; b < gh; b++) hvp.push_back(PkEntry{i, b, gh, 0});
+  }
+  const size_t off_vec = 0, off_one = off_vec + vec.size(), off_scan = off_one + one.size(),
+               off_hvp = off_scan + scan.size(), off_upd = off_hvp + hvp.size(), n_tab = off_upd + upd.size();
+  std::vector<PkEntry> tab;
+  tab.reserve(n_tab);
+  for (auto* t : {&vec, &one, &scan, &hvp, &upd}) tab.insert(tab.end(), t->begin(), t->end());
+  const size_t abytes = ni * sizeof(MxArgs), tbytes = n_tab * sizeof(PkEntry);
+  if (abytes > ps.args_bytes) {
+    if (ps.args) CS_HIP(hipFree(ps.args));
+    CS_HIP(hipMalloc(&ps.args, abytes));
+    ps.args_bytes = abytes;
+  }
+  if (tbytes > ps.tabs_bytes) {
+    if (ps.tabs) CS_HIP(hipFree(ps.tabs));
+    CS_HIP(hipMalloc(&ps.tabs, tbytes));
+    ps.tabs_bytes = tbytes;
+  }
+  if (ni > ps.n_inst) {
+    if (ps.caps) CS_HIP(hipFree(ps.caps));
+    if (ps.flags) CS_HIP(hipFree(ps.flags));
+    CS_HIP(hipMalloc(&ps.caps, ni * sizeof(int)));
+    CS_HIP(hipMalloc(&ps.flags, ni * sizeof(int2)));
+    ps.n_inst = ni;
+  }
+  CS_HIP(hipMemcpyAsync(ps.args, A.data(), abytes, hipMemcpyHostToDevice, st));
+  CS_HIP(hipMemcpyAsync(ps.tabs, tab.data(), tbytes, hipMemcpyHostToDevice, st));
+  CS_HIP(hipMemcpyAsync(ps.caps, caps.data(), ni * sizeof(int), hipMemcpyHostToDevice, st));
+  const MxArgs* dA = static_cast<const MxArgs*>(ps.args);
+  const PkEntry* T = static_cast<const PkEntry*>(ps.tabs);
+  const dim3 b(kBlock);
+  hipLaunchKernelGGL(k_pk_init, dim3((unsigned)vec.size()), b, 0, st, dA, T + off_vec);
+  hipLaunchKernelGGL(k_pk_decide, dim3((unsigned)one.size()), b, 0, st, dA, T + off_one);
+  CS_LAUNCH_CHECK();
+  std::vector<int2> hf(ni);
+  for (int k = 0; k < kmax; k++) {
+    if (!scan.empty())
+      hipLaunchKernelGGL(k_pk_cscan, dim3((unsigned)scan.size()), b, 0, st, dA, T + off_scan, ps.caps, k);
+    hipLaunchKernelGGL(k_pk_hvp, dim3((unsigned)hvp.size()), b, 0, st, dA, T + off_hvp, ps.caps, k);
+    hipLaunchKernelGGL(k_pk_update, dim3((unsigned)upd.size()), b, 0, st, dA, T + off_upd, ps.caps, k);
+    hipLaunchKernelGGL(k_pk_dir, dim3((unsigned)vec.size()), b, 0, st, dA, T + off_vec, ps.caps, k);
+    CS_LAUNCH_CHECK();
+    if ((k + 1) % kGraphIters == 0 || k + 1 == kmax) {
+      hipLaunchKernelGGL(k_pk_flags, dim3(grid_for(ni)), b, 0, st, dA, (uint32_t)ni, ps.flags);
+      CS_HIP(hipMemcpyAsync(hf.data(), ps.flags, ni * sizeof(int2), hipMemcpyDeviceToHost, st));
+      CS_HIP(hipStreamSynchronize(st));
+      bool all = true;
+      for (size_t i = 0; i < ni; i++) all = all && (hf[i].x != 0 || k + 1 >= caps[i]);
+      if (all) break;
+    }
+  }
+  for (size_t i = 0; i < ni; i++) {
+    const uint32_t n3 = jobs[i].in.me.n * 3;
+    hipLaunchKernelGGL(k_mx_out, dim3(grid_for(n3)), b, 0, st, A[i].sol, jobs[i].in.dx, (size_t)n3);
+  }
+  CS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pk_flags, dim3(grid_for(ni)), b, 0, st, dA, (uint32_t)ni, ps.flags);
+  CS_HIP(hipMemcpyAsync(hf.data(), ps.flags, ni * sizeof(int2), hipMemcpyDeviceToHost, st));
+  CS_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < ni; i++) jobs[i].iters = hf[i].x ? hf[i].y : caps[i];
 }
 
 }  // namespace cloth

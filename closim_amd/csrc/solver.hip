@@ -6,6 +6,9 @@
 // completion, g.dx, max |dx|, trial energies, contact counts/validity).
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <exception>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -305,6 +308,65 @@ static bool build_tiles(const cs_scene_desc& d, const std::vector<uint32_t>& vof
   return true;
 }
 
+// Packed batch (cs_batch_step_packed): every instance steps in its own host
+// thread; at each Newton iteration its PCG system is handed over here, and
+// when every instance still stepping has handed one over, the last to arrive
+// solves them all together (MixedSolve::solve_packed) and releases the others.
+// An instance that finishes its step leaves, so the rest never wait for it.
+struct PackedBatch {
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0;    // instances still inside their step
+  uint64_t gen = 0;  // completed packed solves
+  std::vector<PackedJob> pending;
+  std::vector<int*> outs;
+  std::vector<std::exception_ptr*> errs;
+  hipStream_t st = nullptr;
+  PackedScratch scratch;
+
+  int solve(MixedSolve* ms, const MixedInputs& in) {
+    std::unique_lock<std::mutex> lk(mu);
+    int out = -1;
+    std::exception_ptr err;
+    pending.push_back(PackedJob{ms, in, 0});
+    outs.push_back(&out);
+    errs.push_back(&err);
+    const uint64_t mine = gen;
+    if ((int)pending.size() == active) run(lk);
+    else cv.wait(lk, [&] { return gen != mine; });
+    if (err) std::rethrow_exception(err);
+    return out;
+  }
+  void leave() {
+    std::unique_lock<std::mutex> lk(mu);
+    active--;
+    if (!pending.empty() && (int)pending.size() == active) run(lk);
+  }
+  // with lk held; every instance still stepping is waiting in solve()
+  void run(std::unique_lock<std::mutex>& lk) {
+    std::vector<PackedJob> jobs;
+    std::vector<int*> o;
+    std::vector<std::exception_ptr*> e;
+    jobs.swap(pending);
+    o.swap(outs);
+    e.swap(errs);
+    lk.unlock();
+    std::exception_ptr err;
+    try {
+      MixedSolve::solve_packed(jobs, st, scratch);
+    } catch (...) {
+      err = std::current_exception();
+    }
+    lk.lock();
+    for (size_t i = 0; i < jobs.size(); i++) {
+      *o[i] = jobs[i].iters;
+      if (err) *e[i] = err;
+    }
+    gen++;
+    cv.notify_all();
+  }
+};
+
 struct Solver {
   int device = 0;
   hipStream_t st = nullptr;
@@ -313,6 +375,7 @@ struct Solver {
   cs_material mat;
   double gravity[3];
   bool has_contact = false;
+  PackedBatch* packer = nullptr;  // set for the duration of a packed batch step
   cs_step_stats last;
   bool has_last = false;
 
@@ -1025,6 +1088,20 @@ struct Solver {
   }
 
   int mixed_solve(double invH2, int maxIters) {
+    if (packer) {
+      MixedInputs in = mixed_inputs(invH2, maxIters);
+      CS_HIP(hipStreamSynchronize(st));  // b, the diagonal and the block inverses are ready
+      return packer->solve(mixed.get(), in);
+    }
+    MixedInputs in = mixed_inputs(invH2, maxIters);
+    int mark = -1;
+    return mixed->solve(in, [&](int, bool begin) {
+      if (begin) mark = ev_mark();
+      else ev_span(CS_KERNEL_HVP, mark, ev_mark());
+    });
+  }
+
+  MixedInputs mixed_inputs(double invH2, int maxIters) {
     MixedInputs in;
     in.me = dmesh();
     in.mat = dmat();
@@ -1048,13 +1125,9 @@ struct Solver {
     if (contact) in.contact = contact->hvp_hook();
     in.max_iters = maxIters;
     in.tol = 1e-3;
-    int mark = -1;
     in.graph = (opt.flags & (CS_FLAG_TIMING | CS_FLAG_EAGER)) == 0;
     in.psd = (opt.flags & CS_FLAG_PSD_NEWTON) != 0;
-    return mixed->solve(in, [&](int, bool begin) {
-      if (begin) mark = ev_mark();
-      else ev_span(CS_KERNEL_HVP, mark, ev_mark());
-    });
+    return in;
   }
 
   PcgArgs last_pcg{};
@@ -1322,6 +1395,56 @@ int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int thre
     }
   }
   return first;
+}
+
+int cs_batch_step_packed(cs_solver* const* solvers, uint32_t count, double dt, int32_t* status) {
+  if (!solvers && count) return fail(CS_E_INVALID, "null solver array");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(CS_E_INVALID, "dt must be positive and finite");
+  for (uint32_t i = 0; i < count; i++) {
+    if (!solvers[i]) return fail(CS_E_NOT_INIT, "CpuSolver not initialized");
+    const cloth::Solver& S = solvers[i]->s;
+    if (!S.mixed || S.schwarz) return fail(CS_E_INVALID, "a packed batch takes mixed-precision solvers with the "
+                                                         "Jacobi or block-Jacobi preconditioner");
+    if (S.device != solvers[0]->s.device) return fail(CS_E_INVALID, "a packed batch runs on one device");
+  }
+  {
+    std::vector<cs_solver*> seen(solvers, solvers + count);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+      return fail(CS_E_INVALID, "a solver appears twice in the batch");
+  }
+  if (!count) return CS_OK;
+  return guarded([&]() -> int {
+    CS_HIP(hipSetDevice(solvers[0]->s.device));
+    cloth::PackedBatch pb;
+    pb.active = (int)count;
+    CS_HIP(hipStreamCreateWithFlags(&pb.st, hipStreamNonBlocking));
+    for (uint32_t i = 0; i < count; i++) solvers[i]->s.packer = &pb;
+    std::vector<int32_t> st(count, CS_OK);
+    std::vector<std::string> msg(count);
+    auto worker = [&](uint32_t i) {
+      st[i] = cs_step(solvers[i], dt);
+      if (st[i] != CS_OK) msg[i] = g_last_error;
+      pb.leave();
+    };
+    // one thread per instance: an instance waits for the others at every
+    // packed solve, so none may wait for a free thread
+    std::vector<std::thread> pool;
+    pool.reserve(count);
+    for (uint32_t i = 0; i < count; i++) pool.emplace_back(worker, i);
+    for (auto& th : pool) th.join();
+    for (uint32_t i = 0; i < count; i++) solvers[i]->s.packer = nullptr;
+    (void)hipStreamDestroy(pb.st);
+    int first = CS_OK;
+    for (uint32_t i = 0; i < count; i++) {
+      if (status) status[i] = st[i];
+      if (st[i] != CS_OK && first == CS_OK) {
+        first = st[i];
+        g_last_error = "instance " + std::to_string(i) + ": " + msg[i];
+      }
+    }
+    return first;
+  });
 }
 
 int cs_step_adjoint(cs_solver* s, double dt, const double* x_prev, const double* v_prev, const double* x_next,

@@ -64,6 +64,7 @@ def load_library(path=LIB_PATH):
         "cs_hvp_bytes": (C.c_int, [C.c_void_p, _dp, _dp]),
         "cs_tuned_launch": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
         "cs_batch_step": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.c_int, C.POINTER(C.c_int32)]),
+        "cs_batch_step_packed": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_double, C.POINTER(C.c_int32)]),
         "cs_step_adjoint": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_int, _dp,
                                       _dp, _dp, C.POINTER(C.c_int), _dp]),
         "cs_probe_hvp": (C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp]),
@@ -577,10 +578,14 @@ class HipSolver:
 class HipBatch:
     """Many independent HipSolvers stepped together (cs_batch_step): host
     threads take the instances largest first and step each on its own HIP
-    stream, so small instances' kernels overlap on the GPU."""
+    stream, so small instances' kernels overlap on the GPU.  With packed=True
+    (cs_batch_step_packed; mixed precision, Jacobi or block-Jacobi) the
+    instances' inner PCG solves run together, one launch per PCG stage over
+    all instances."""
 
-    def __init__(self, solvers, threads=None):
+    def __init__(self, solvers, threads=None, packed=False):
         self.solvers = list(solvers)
+        self.packed = bool(packed)
         if len({id(s) for s in self.solvers}) != len(self.solvers):
             raise ValueError("a solver appears twice in the batch")
         # host threads: 16 measured fastest on the 64-sheet batch (4: 0.43 s,
@@ -594,7 +599,10 @@ class HipBatch:
         n = len(self.solvers)
         hs = (C.c_void_p * n)(*[s._h.value for s in self.solvers])
         status = (C.c_int32 * n)()
-        rc = self._lib.cs_batch_step(hs, n, float(dt), self.threads, status)
+        if self.packed:
+            rc = self._lib.cs_batch_step_packed(hs, n, float(dt), status)
+        else:
+            rc = self._lib.cs_batch_step(hs, n, float(dt), self.threads, status)
         # the instances that stepped are copied out even when another failed
         for s, st in zip(self.solvers, status):
             if st == _abi.CS_OK:

@@ -294,6 +294,16 @@ int cs_tuned_launch(const cs_solver* s, char* buf, size_t cap);
  * return value is the first failure (cs_last_error names the instance) or
  * CS_OK.  A solver may appear only once in `solvers`. */
 int cs_batch_step(cs_solver* const* solvers, uint32_t count, double dt, int threads, int32_t* status);
+/* The same batch step with the instances' inner solves packed (no reference
+ * counterpart): each instance steps in its own host thread, and at every
+ * Newton iteration the PCG systems of all instances still stepping are
+ * solved together, each PCG stage one launch over every instance's blocks
+ * (a block table gives each block its instance, local block and local grid,
+ * so each instance keeps its own material, partial sums and reduction trees:
+ * the trajectories are those of cs_step).  Mixed-precision solvers with the
+ * Jacobi or block-Jacobi preconditioner on one device (CS_E_INVALID
+ * otherwise). */
+int cs_batch_step_packed(cs_solver* const* solvers, uint32_t count, double dt, int32_t* status);
 
 /* Adjoint of one step (no reference counterpart: the reference has no
  * differentiation).  Given the state before the step (x_prev, v_prev), the

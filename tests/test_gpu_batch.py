@@ -105,3 +105,74 @@ def test_smallest_uneven_drape_matches_oracle():
         assert g.lastStats["pcg_iters"] == o.lastStats["pcg_iters"]
         assert np.array_equal(sg.positions, so.positions) and np.array_equal(sg.velocities, so.velocities)
     g.close()
+
+
+def test_packed_batch_matches_instances_stepped_alone():
+    # cs_batch_step_packed: the instances' PCG systems solved together at every
+    # Newton iteration (one launch per PCG stage over all instances), each with
+    # its own arithmetic and reduction trees: trajectories and PCG counts equal
+    # to stepping every instance alone, bit for bit
+    kw = dict(precond=_abi.CS_PRECOND_BLOCK_JACOBI)
+    sb, sa = instances(), instances()
+    gb = [HipSolver("mixed", **kw) for _ in sb]
+    ga = [HipSolver("mixed", **kw) for _ in sa]
+    for g, sc in zip(gb + ga, sb + sa):
+        g.initialize(sc)
+    B = HipBatch(gb, packed=True)
+    for t in range(6):
+        assert B.step(H) == [0] * len(gb)
+        for g in ga:
+            g.step(H)
+        for k, (a, b) in enumerate(zip(sa, sb)):
+            assert gb[k].lastStats["pcg_iters"] == ga[k].lastStats["pcg_iters"], (t, k)
+            assert np.array_equal(a.positions, b.positions), (t, k)
+            assert np.array_equal(a.velocities, b.velocities), (t, k)
+    for g in gb + ga:
+        g.close()
+
+
+def test_packed_batch_of_eight_uneven_drapes():
+    sb, sa = S.batch_drapes(8), S.batch_drapes(8)
+    gb = [HipSolver("mixed", precond=_abi.CS_PRECOND_BLOCK_JACOBI) for _ in sb]
+    ga = [HipSolver("mixed", precond=_abi.CS_PRECOND_BLOCK_JACOBI) for _ in sa]
+    for g, sc in zip(gb + ga, sb + sa):
+        g.initialize(sc)
+    B = HipBatch(gb, packed=True)
+    for t in range(2):
+        assert B.step(H) == [0] * len(gb)
+        for g in ga:
+            g.step(H)
+        its = [g.lastStats["pcg_iters"] for g in gb]
+        print(f"[packed] step {t}: PCG iterations {its}", flush=True)
+        for k, (a, b) in enumerate(zip(sa, sb)):
+            assert np.array_equal(a.positions, b.positions), (t, k)
+    for g in gb + ga:
+        g.close()
+
+
+def test_packed_batch_rejects_fp64_and_schwarz_and_survives_a_failing_instance():
+    for prec, kw in (("fp64", {}), ("mixed", dict(precond=_abi.CS_PRECOND_SCHWARZ))):
+        g = HipSolver(prec, **kw)
+        g.initialize(S.small_strip())
+        with pytest.raises(ClothError) as e:
+            HipBatch([g], packed=True).step(H)
+        assert e.value.code == _abi.CS_E_INVALID
+        g.close()
+    # instance 1 fails its first Newton check (non-finite gradient) and leaves
+    # the packed solves; the others keep stepping
+    scenes = [S.floor_patch(nx=4), S.small_strip(), S.floor_patch(nx=6)]
+    gs = [HipSolver("mixed", precond=_abi.CS_PRECOND_BLOCK_JACOBI) for _ in scenes]
+    for g, sc in zip(gs, scenes):
+        g.initialize(sc)
+    ref = [sc.positions.copy() for sc in scenes]
+    bad = scenes[1].positions.copy()
+    free = next(v for v in range(scenes[1].mesh.count) if v not in scenes[1].pinned)
+    bad[free * 3] = np.nan
+    gs[1].set_state(bad, scenes[1].velocities)
+    with pytest.raises(ClothError) as e:
+        HipBatch(gs, packed=True).step(H)
+    assert e.value.code == _abi.CS_E_NONFINITE and "instance 1" in str(e.value)
+    assert not np.array_equal(gs[0].getPositions(), ref[0])
+    assert not np.array_equal(gs[2].getPositions(), ref[2])
+    for g in gs:
+        g.close()
