@@ -1989,6 +1989,7 @@ struct ContactImpl {
     kill_lds_max = kill_lds_limit();
     cverts.alloc(std::max<uint32_t>(n, 1));
     hvsum.alloc((size_t)std::max<uint32_t>(n, 1) * 3);
+    hvsum.zero(st);
     lag_count.alloc(1);
     ensure_sort(m);
     grow_pairs(std::max<uint32_t>(1024, m * 4), std::max<uint32_t>(256, ms * 8));
@@ -2262,6 +2263,7 @@ struct ContactImpl {
   void build_incidence() {
     CS_HIP(hipMemsetAsync(v_off.ptr, 0, (size_t)(n + 1) * sizeof(uint32_t), st));
     E = 0;
+    nv = 0;  // no contact vertex until the entries are rebuilt below
     if (!C) {
       sync_cs();
       return;

@@ -150,9 +150,10 @@ __device__ __forceinline__ void hvp_vertex8(const PcgArgs& a, uint32_t v, bool v
   const double m = valid ? (double)me.mass[v] : 0.0;
 #pragma unroll
   for (int i = 0; i < 3; i++) out[i] = m * (valid ? p[(size_t)v * 3 + i] : 0.0) * a.inv_h2 + Hp[i];
-  if (a.contact.act && valid) {
+  if (a.contact.act && a.contact.nv && valid) {
     // applyHvp (contact-assembly.ts:371-392): cHp summed per vertex in list
-    // order by k_contact_vsum, then out += cHp
+    // order by k_contact_vsum, then out += cHp (vsum holds the sums of this
+    // HVP only while the active set has incidence entries: nv > 0)
     out[0] += a.contact.vsum[(size_t)v * 3];
     out[1] += a.contact.vsum[(size_t)v * 3 + 1];
     out[2] += a.contact.vsum[(size_t)v * 3 + 2];

@@ -107,6 +107,37 @@ def test_smallest_uneven_drape_matches_oracle():
     g.close()
 
 
+def test_contact_free_step_after_a_freed_contact_solver():
+    # A solver freed after steps with active contacts hands its device memory
+    # back to the runtime, and the next solver's buffers may reuse it.  The
+    # FP64 PCG's per-vertex contact sums must not enter a solve whose active
+    # set is empty: the smallest uneven drape's first step (no contact yet)
+    # still matches the oracle bit for bit.
+    import oracle_ffi as O
+    scenes = S.batch_drapes(8)
+    k = int(np.argmin([sc.mesh.count for sc in scenes]))
+    ga = HipSolver("fp64", precond=_abi.CS_PRECOND_BLOCK_JACOBI)
+    ga.initialize(scenes[k])
+    active = 0
+    for _ in range(12):
+        ga.step(H)
+        active = max(active, ga.lastStats["contact"]["active_pairs"])
+    ga.close()
+    assert active > 0
+    sg, so = S.batch_drapes(8)[k], S.batch_drapes(8)[k]
+    g = HipSolver("fp64", precond=_abi.CS_PRECOND_BLOCK_JACOBI)
+    g.initialize(sg)
+    o = O.OracleSolver(so, _abi.options(precond=_abi.CS_PRECOND_BLOCK_JACOBI))
+    g.step(H)
+    o.step(H)
+    print(f"[batch] after a freed solver with {active} active contacts: pcg {g.lastStats['pcg_iters']} "
+          f"(oracle {o.lastStats['pcg_iters']})", flush=True)
+    assert g.lastStats["contact"]["active_pairs"] == 0
+    assert g.lastStats["pcg_iters"] == o.lastStats["pcg_iters"]
+    assert np.array_equal(sg.positions, so.positions) and np.array_equal(sg.velocities, so.velocities)
+    g.close()
+
+
 def test_packed_batch_matches_instances_stepped_alone():
     # cs_batch_step_packed: the instances' PCG systems solved together at every
     # Newton iteration (one launch per PCG stage over all instances), each with

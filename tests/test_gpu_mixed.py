@@ -23,10 +23,14 @@ TOL = 1e-5          # relative position error, converged Newton
 # With the reference's iteration caps (10 Newton x 60 PCG) and frictional
 # contact Newton stops unconverged, and the trajectory depends on the rounding
 # of the inner solve: replacing the oracle's FD Hessian-vector product by the
-# analytic one, in FP64, already moves it by up to percents.  The bar here is that
-# mixed precision moves it no more than that change of operator does (a
-# factor 5 of slack: same order); the precision claim is the converged test above.
+# analytic one, in FP64, already moves it by up to percents, and so does
+# starting the oracle's own FP64 operator from initial positions scaled by
+# 1 + 6e-8·U(-1, 1) (the FP32 unit round-off).  The bar here is that mixed
+# precision moves the trajectory no more than the larger of those changes do
+# (a factor 5 of slack: same order); the precision claim is the converged test
+# above.
 DRIFT_FACTOR = 5.0
+PERTURB_SEEDS = (1, 2, 3)
 
 CONVERGED = dict(newton_iters=60, pcg_max_iters=400, grad_tol=1e-11)
 
@@ -90,21 +94,33 @@ def test_mixed_default_budget_tracks_oracle(make, steps, precond):
     ga = HipSolver("fp64", precond=precond, hvp_mode=_abi.CS_HVP_ANALYTIC)
     ga.initialize(sa)
     o = O.OracleSolver(so, _abi.options(precond=precond))
-    worst = worst_a = 0.0
+    gp = []
+    for seed in PERTURB_SEEDS:
+        sp = make()
+        sp.positions *= 1.0 + 6e-8 * np.random.default_rng(seed).uniform(-1.0, 1.0, sp.positions.shape)
+        gp.append(HipSolver("fp64", precond=precond))
+        gp[-1].initialize(sp)
+    worst = worst_a = worst_p = 0.0
     for k in range(steps):
-        for g in (gm, gd, ga):
+        for g in [gm, gd, ga] + gp:
             g.step(H)
         o.step(H)
         e_fp64 = rel(gd.getPositions(), o.getPositions())
         e_mixed = rel(gm.getPositions(), o.getPositions())
         e_an = rel(ga.getPositions(), o.getPositions())
+        e_p = max(rel(g.getPositions(), o.getPositions()) for g in gp)
         worst = max(worst, e_mixed)
         worst_a = max(worst_a, e_an)
+        worst_p = max(worst_p, e_p)
         print(f"step {k}: fp64 vs oracle {e_fp64:.2e}  mixed vs oracle {e_mixed:.2e}  fp64-analytic vs oracle "
-              f"{e_an:.2e}  pcg mixed {gm.lastStats['pcg_iters']} fp64 {gd.lastStats['pcg_iters']}", flush=True)
+              f"{e_an:.2e}  perturbed fp64 vs oracle {e_p:.2e}  pcg mixed {gm.lastStats['pcg_iters']} "
+              f"fp64 {gd.lastStats['pcg_iters']}", flush=True)
         assert e_fp64 <= 1e-9
-    print(f"worst rel err (default budget): mixed {worst:.2e}, fp64-analytic {worst_a:.2e}")
-    assert worst <= DRIFT_FACTOR * worst_a + 1e-6, (worst, worst_a)
+    print(f"worst rel err (default budget): mixed {worst:.2e}, fp64-analytic {worst_a:.2e}, "
+          f"perturbed fp64 {worst_p:.2e}")
+    assert worst <= DRIFT_FACTOR * max(worst_a, worst_p) + 1e-6, (worst, worst_a, worst_p)
+    for g in [gm, gd, ga] + gp:
+        g.close()
 
 
 GRAPH_SCENES = [
