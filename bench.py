@@ -206,6 +206,28 @@ def pair_section(workload, steps, warmup, h):
                               "traffic": tr[0], "traffic_source": tr[1]})
 
 
+def gate_section(workload, steps, warmup, h, base_flags):
+    """The workload with CpuSolver's descent gate (only a non-descent PCG
+    direction falls back to steepest descent; newton.ts:154-160), both
+    precisions from the same initial state after `warmup` steps: steps/s and
+    the timed Newton / PCG / line-search work, with the steps whose line
+    search failed and those in which the sheet did not move."""
+    popts = precond_options(workload)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        g = solver_for(make_scene(workload), prec, popts, timing=False, flags=base_flags)
+        for _ in range(warmup):
+            raw_step(g, h)
+        work = {}
+        t = timed_steps(g, h, steps, work)
+        res[prec] = dict(steps_per_s=steps / t, ms_per_step=1000 * t / steps, timed_work=work)
+        print(f"[bench] {workload} {prec}, CpuSolver descent gate: {steps} steps in {t:.3f} s, {work}",
+              file=sys.stderr, flush=True)
+        g.close()
+    res.update(steps=steps, warmup=warmup, mixed_over_fp64=res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"])
+    return res
+
+
 def lbvh_ccd_section(scene, h, reps=5):
     """The standalone broadphase (cs_broadphase_self_pairs: Morton codes,
     radix sort, Karras hierarchy, refit; then the pair query) over the swept
@@ -384,13 +406,35 @@ def raw_step(g, h):
     _check(g._lib.cs_step(g._h, h))
 
 
-def timed_steps(g, h, steps):
+def timed_steps(g, h, steps, work=None):
+    """Wall time of `steps` steps.  With a dict `work`, also the Newton, PCG
+    and line-search totals of those steps (cs_last_stats after each step, a
+    host-side read), the steps whose Newton loop stopped before its cap
+    without converging (a failed line search), and the steps whose energy
+    equals the previous step's (the sheet did not move)."""
     hip_sync()
+    stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
         raw_step(g, h)
+        if work is not None:
+            stats.append(last_stats_raw(g))
     hip_sync()
-    return time.perf_counter() - t0
+    t = time.perf_counter() - t0
+    if work is not None:
+        work.update(newton_iters=sum(s.newton_used for s in stats), pcg_iters=sum(s.pcg_iters for s in stats),
+                    line_search_trials=sum(s.line_search_trials for s in stats),
+                    steps_stopped_early=sum(1 for s in stats if s.newton_used < s.newton_iters and not s.converged),
+                    steps_not_moved=sum(1 for a, b in zip(stats, stats[1:]) if a.energy == b.energy))
+    return t
+
+
+def last_stats_raw(g):
+    import ctypes as C
+    from closim_amd import _abi
+    st = _abi.CsStepStats()
+    g._lib.cs_last_stats(g._h, C.byref(st))
+    return st
 
 
 def cpu_baseline(name, popts, budget_s, newton_per_step, h):
@@ -506,6 +550,12 @@ def main():
     ap.add_argument("--host-newton", action="store_true",
                     help="decide the descent safeguard, trust region and each Armijo trial on the host "
                          "(default: CS_FLAG_DEVICE_NEWTON, the same results with one host read per trial batch)")
+    ap.add_argument("--descent-gate", choices=["webgpu", "cpusolver"], default="webgpu",
+                    help="Newton's descent gate: the reference WebGPU Newton's (steepest descent also after a PCG "
+                         "breakdown, CS_FLAG_BREAKDOWN_DESCENT) or CpuSolver's (only for a non-descent direction)")
+    ap.add_argument("--cpusolver-gate-steps", type=int, default=None,
+                    help="timed steps of the fold with CpuSolver's descent gate, both precisions (default --steps, "
+                         "0 = skip)")
     ap.add_argument("--eager", action="store_true",
                     help="launch the mixed PCG kernels one by one instead of replaying captured graphs")
     ap.add_argument("--traffic", type=float, default=None,
@@ -533,7 +583,8 @@ def main():
     from closim_amd.solver import device_info
     h = 1.0 / 60.0
     popts = precond_options(args.workload)
-    flags = (_abi.CS_FLAG_EAGER if args.eager else 0) | (0 if args.host_newton else _abi.CS_FLAG_DEVICE_NEWTON)
+    base_flags = (_abi.CS_FLAG_EAGER if args.eager else 0) | (0 if args.host_newton else _abi.CS_FLAG_DEVICE_NEWTON)
+    flags = base_flags | (_abi.CS_FLAG_BREAKDOWN_DESCENT if args.descent_gate == "webgpu" else 0)
     print(f"[bench] device: {device_info(0)}", file=sys.stderr, flush=True)
     sc_m = make_scene(args.workload)
     mesh = sc_m.mesh
@@ -550,7 +601,8 @@ def main():
         if gd:
             raw_step(gd, h)
         print(f"[bench] warmup step {k} done", file=sys.stderr, flush=True)
-    t_m = timed_steps(gm, h, args.steps)
+    work_m = {}
+    t_m = timed_steps(gm, h, args.steps, work_m)
     print(f"[bench] mixed: {args.steps} steps in {t_m:.3f} s", file=sys.stderr, flush=True)
     st_m = gm.lastStats
     hvp_ms, hvp_bytes = gm.time_hvp(args.hvp_reps)
@@ -558,7 +610,8 @@ def main():
           file=sys.stderr, flush=True)
     res = {}
     if gd:
-        t_d = timed_steps(gd, h, args.steps)
+        work_d = {}
+        t_d = timed_steps(gd, h, args.steps, work_d)
         print(f"[bench] fp64: {args.steps} steps in {t_d:.3f} s", file=sys.stderr, flush=True)
         st_d = gd.lastStats
         hvp_ms_d, hvp_bytes_d = gd.time_hvp(args.hvp_reps)
@@ -569,14 +622,15 @@ def main():
         res["fp64"] = dict(steps_per_s=args.steps / t_d, ms_per_step=1000 * t_d / args.steps,
                            hvp_kernel="k_pcg_hvp (FP64 finite-difference membrane HVP, the CpuSolver operator)",
                            hvp_mean_us=1000 * hvp_ms_d, hvp_gbs=hvp_bytes_d / (hvp_ms_d / 1000) / 1e9,
-                           newton_used_last_step=st_d["newton_used"], pcg_iters_last_step=st_d["pcg_iters"])
+                           newton_used_last_step=st_d["newton_used"], pcg_iters_last_step=st_d["pcg_iters"],
+                           timed_work=work_d)
         res["mixed_over_fp64"] = t_d / t_m
         res["pos_rel_err_mixed_vs_fp64"] = err
         # sensitivity of the same capped solve to the Hessian operator alone:
         # FP64 with the analytic HVP instead of the FD one
         sc_a = make_scene(args.workload)
         from closim_amd.solver import HipSolver
-        ga = HipSolver("fp64", **popts, hvp_mode=_abi.CS_HVP_ANALYTIC)
+        ga = HipSolver("fp64", **popts, hvp_mode=_abi.CS_HVP_ANALYTIC, flags=flags)
         ga.initialize(sc_a)
         for _ in range(args.warmup + args.steps):
             raw_step(ga, h)
@@ -621,6 +675,9 @@ def main():
                    "preconditioner": WORKLOADS[args.workload]["precond"],
                    "pcg_launch": "eager" if args.eager else "graph replay",
                    "newton_control": "host" if args.host_newton else "device (batched Armijo trials)",
+                   "descent_gate": ("reference WebGPU Newton: steepest descent after a non-descent direction or a "
+                                    "PCG breakdown" if args.descent_gate == "webgpu" else
+                                    "CpuSolver: steepest descent after a non-descent direction"),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -631,6 +688,7 @@ def main():
                                     "per triangle; one record per active contact)",
                      "mean_launch_us": mean_s * 1e6, "launches": args.hvp_reps,
                      "traffic_source": traffic_src},
+        "timed_work": work_m,
         "newton_used_last_step": st_m["newton_used"],
         "pcg_iters_last_step": st_m["pcg_iters"],
         "active_contacts_last_step": st_m["contact"]["active_pairs"] if st_m["contact"] else 0,
@@ -646,6 +704,9 @@ def main():
     gm.close()
     if gd:
         gd.close()
+    gate_steps = args.steps if args.cpusolver_gate_steps is None else args.cpusolver_gate_steps
+    if args.workload == "fold" and args.descent_gate == "webgpu" and gate_steps > 0 and gd:
+        out["cpusolver_gate"] = gate_section(args.workload, gate_steps, args.warmup, h, base_flags)
     if args.coarse_steps > 0:
         out["coarse"] = coarse_section(args.workload, args.coarse_steps, 2, h)
     if args.workload == "fold" and args.drape_steps > 0 and gd:

@@ -74,6 +74,9 @@ class MixedSolve {
   // arithmetic, partial slots and reduction trees, so every result is the
   // one solve_plain gives; `scratch` holds the tables between calls.
   static void solve_packed(std::vector<PackedJob>& jobs, hipStream_t st, PackedScratch& scratch);
+  // The device latch of the last solve: 0 iteration cap, 1 stopped on
+  // non-positive curvature (p.Ap <= 1e-30), 2 converged.
+  const int32_t* done_flag() const;
   void set_material(const cs_material&) {}
 
  private:

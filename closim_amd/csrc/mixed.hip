@@ -278,7 +278,7 @@ __device__ __forceinline__ void k_mx_decide_body(const MxArgs& a, unsigned count
     s->residual = sqrtf(bb);
     s->abs_tol = fmaxf(a.tol * fmaxf(s->residual, 1e-12f), 1e-12f);
     s->iters = 0;
-    s->done = s->residual < s->abs_tol ? 1 : 0;
+    s->done = s->residual < s->abs_tol ? 2 : 0;  // 2 converged, 1 (update) breakdown
   }
 }
 
@@ -587,6 +587,8 @@ MixedSolve::MixedSolve(const cs_scene_desc& d, const cs_options&, hipStream_t st
   im_->s.alloc(1);
   CS_HIP(hipMemsetAsync(im_->s.ptr, 0, sizeof(MxScalars), st));
 }
+
+const int32_t* MixedSolve::done_flag() const { return &im_->s.ptr->done; }
 
 MixedSolve::~MixedSolve() {
   if (im_->exec) (void)hipGraphExecDestroy(im_->exec);

@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init_decide(PcgArgs a, unsigned 
     s->residual = jsm::sqrt_(rr);
     s->rz[0] = rz;
     s->iters = 0;
-    s->done = (s->residual < s->abs_tol) ? 1 : 0;
+    s->done = (s->residual < s->abs_tol) ? 2 : 0;  // 2 converged, 1 (k_pcg_update) breakdown
   }
 }
 

@@ -39,6 +39,8 @@ struct SchwarzImpl {
   DevBuf<double> L;      // factors, D * T (T = DW (DW + 1) / 2), packed lower triangle by columns
   DevBuf<float> L32;     // FP32 copy of L for the mixed-precision apply (reciprocal diagonal)
   DevBuf<float> L32T;    // the same packed by rows, for its back substitution
+  DevBuf<float> T32;     // FP32 factor as 16x16 tiles (domains of at most 12 blocks), see k_sw_apply32t
+  uint32_t nbt = 0;      // 16-row blocks of the widest domain
   DevBuf<double> W;      // factorisation workspace when the triangle does not fit in LDS
   DevBuf<uint8_t> flag;  // 1 = Cholesky, 0 = Jacobi fallback
   size_t lds_max = 0;    // dynamic LDS the factor kernel may use
@@ -177,9 +179,22 @@ struct SwArgs {
   double* L;
   float* L32;
   float* L32T;
+  float* T32;
+  uint32_t nbt;
   double* W;
   uint8_t* flag;
 };
+
+// The FP32 factor of a domain of at most 12 blocks of 16 rows as 16x16 tiles
+// (a, b), a >= b, ordered by block columns: tile (a, b) starts at
+// tile_index(nbt, a, b) * 256 floats of the domain's nbt (nbt + 1) / 2 tiles.
+// An off-diagonal tile holds L_ab by columns (entry (i, j) at j * 16 + i);
+// a diagonal tile holds the inverse of the lower triangle L_aa by rows
+// (entry (i, j) at i * 16 + j).  Rows beyond the domain are zero.
+__host__ __device__ __forceinline__ uint32_t tile_count(uint32_t nbt) { return nbt * (nbt + 1) / 2; }
+__host__ __device__ __forceinline__ uint32_t tile_index(uint32_t nbt, uint32_t a, uint32_t b) {
+  return b * nbt - b * (b - 1) / 2 + (a - b);
+}
 
 __device__ __forceinline__ double* dom_mat(const SwArgs& s, uint32_t d) { return s.A + (size_t)d * s.DW * s.DW; }
 __host__ __device__ __forceinline__ size_t tri_size(uint32_t dw) { return (size_t)dw * (dw + 1) / 2; }
@@ -457,6 +472,35 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
     }
   }
   __syncthreads();
+  if constexpr (kF32) {
+    // the tiles of k_sw_apply32t: tile (a, b) is this work-group's e[a][b],
+    // one entry per thread, stored by columns (thread tc * 16 + tr writes
+    // entry tc * 16 + tr: coalesced); a diagonal tile keeps the reciprocal
+    // pivots on its diagonal until k_sw_tile_inv inverts it.  The Jacobi
+    // fallback becomes the diagonal tiles diag(1 / sqrt(d)) (the two sweeps
+    // then scale by 1 / d) with zero off-diagonal tiles.
+    const uint32_t nbt = s.nbt;
+    float* Tb = s.T32 + (size_t)d * tile_count(nbt) * 256;
+    const bool good = ok != 0;
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+      for (int b = 0; b <= a; b++) {
+        if ((uint32_t)a >= nbt) continue;
+        const uint32_t i = 16 * a + tr, k = 16 * b + tc;
+        float v = 0.f;
+        if (good) {
+          if (i < dw && k <= i) v = e[a][b];
+        } else if (a == b && tr == tc && i < dw) {
+          const uint32_t mv = s.members[(size_t)d * s.dmax + i / 3];
+          const double dd = diag[(size_t)mv * 3 + i % 3];
+          v = dd > 1e-12 ? (float)sqrt(1.0 / dd) : 0.f;
+        }
+        Tb[(size_t)tile_index(nbt, (uint32_t)a, (uint32_t)b) * 256 + tc * 16 + tr] = v;
+      }
+    if (threadIdx.x == 0) s.flag[d] = good ? 1 : 0;
+    return;
+  }
   const size_t base = (size_t)d * tri_size(s.DW);
   if (ok) {
 #pragma unroll
@@ -465,13 +509,8 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
       for (int b = 0; b <= a; b++) {
         const uint32_t i = 16 * a + tr, k = 16 * b + tc;
         if (i < dw && k <= i) {
-          if constexpr (kF32) {
-            s.L32[base + col_off(dw, k) + (i - k)] = e[a][b];
-            s.L32T[base + row_off(i) + k] = e[a][b];
-          } else {
-            s.L[base + col_off(dw, k) + (i - k)] = e[a][b];
-            if (want32) store32(s, d, dw, i, k, e[a][b]);
-          }
+          s.L[base + col_off(dw, k) + (i - k)] = e[a][b];
+          if (want32) store32(s, d, dw, i, k, e[a][b]);
         }
       }
   } else {
@@ -480,14 +519,45 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
       const uint32_t v = s.members[(size_t)d * s.dmax + i / 3];
       const double dd = diag[(size_t)v * 3 + i % 3];
       const double inv = dd > 1e-12 ? 1 / dd : 0;
-      if (!kF32) s.L[base + col_off(dw, i)] = inv;
-      if (kF32 || want32) {
+      s.L[base + col_off(dw, i)] = inv;
+      if (want32) {
         s.L32[base + col_off(dw, i)] = (float)inv;
         s.L32T[base + row_off(i) + i] = (float)inv;
       }
     }
   }
   if (threadIdx.x == 0) s.flag[d] = ok ? 1 : 0;
+}
+
+// Each diagonal tile of the FP32 factor (reciprocal pivots on the diagonal,
+// L's strictly lower entries below, by columns) becomes the inverse of its
+// lower triangle, by rows: column k of the inverse by forward substitution
+// in FP64 (x_k = 1 / L_kk, x_i = -(sum_{k<=j<i} L_ij x_j) / L_ii), one
+// thread per (block, column).  Zero pivots (rows beyond the domain) give
+// zero rows and columns.
+__global__ void __launch_bounds__(192) k_sw_tile_inv(SwArgs s) {
+  __shared__ float t[12][256];
+  const uint32_t d = blockIdx.x, nbt = s.nbt;
+  float* Tb = s.T32 + (size_t)d * tile_count(nbt) * 256;
+  for (uint32_t e = threadIdx.x; e < nbt * 256; e += blockDim.x)
+    t[e / 256][e % 256] = Tb[(size_t)tile_index(nbt, e / 256, e / 256) * 256 + e % 256];
+  __syncthreads();
+  const uint32_t a = threadIdx.x / 16, k = threadIdx.x % 16;
+  if (a >= nbt) return;
+  const float* L = t[a];
+  double x[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    double sum = 0;
+#pragma unroll
+    for (int j = 0; j < i; j++)
+      if ((uint32_t)j >= k) sum += (double)L[j * 16 + i] * x[j];
+    const double p = L[i * 16 + i];
+    x[i] = (uint32_t)i < k ? 0.0 : (uint32_t)i == k ? p : -sum * p;
+  }
+  float* D = Tb + (size_t)tile_index(nbt, a, a) * 256;
+#pragma unroll
+  for (int i = 0; i < 16; i++) D[i * 16 + k] = (float)x[i];
 }
 
 // The substitutions keep row i of a domain in lane i % 64, register slot
@@ -762,6 +832,81 @@ __global__ void __launch_bounds__(kWave) k_sw_apply32(SwArgs s, const float* __r
   }
 }
 
+// The mixed solve's Schwarz apply over the tiles (one wave per domain):
+// both substitutions by 16-row blocks, each block a small product with the
+// inverted diagonal tile and every off-diagonal tile one 16x16
+// matrix-vector product, so the only dependency chain is the block order.
+// Lane l holds, of every tile it reads, the float4 at 4 l: column c = l / 4
+// and rows 4 q .. 4 q + 3 (q = l % 4) of an off-diagonal tile, row c and
+// columns 4 q .. 4 q + 3 of a diagonal tile.  A block vector is kept either
+// by columns (every lane holds entry c) or by rows (every lane holds entries
+// 4 q .. 4 q + 3); a sum over the lanes that share c takes two butterfly
+// steps (xor 1, 2), over those that share q four (xor 4 .. 32), and the
+// off-diagonal contributions of a block are summed per lane first and
+// reduced once, when the block is reached.
+__device__ __forceinline__ float xsum(float v, int m) { return v + __shfl_xor(v, m, kWave); }
+__device__ __forceinline__ float sum_rows(float v) { return xsum(xsum(v, 1), 2); }
+__device__ __forceinline__ float sum_cols(float v) { return xsum(xsum(xsum(xsum(v, 4), 8), 16), 32); }
+
+template <int NBT>
+__global__ void __launch_bounds__(kWave) k_sw_apply32t(SwArgs s, const float* __restrict__ r, float* z,
+                                                       const int32_t* done) {
+  if (done && *done) return;
+  const uint32_t d = blockIdx.x;
+  const uint32_t dw = s.size[d] * 3;
+  const uint32_t l = threadIdx.x, q = l & 3, c = l >> 2;
+  const float4* __restrict__ T = reinterpret_cast<const float4*>(s.T32) + (size_t)d * tile_count(NBT) * 64 + l;
+  const uint32_t* mem = s.members + (size_t)d * s.dmax;
+  // forward: y = L^-1 r, block column by block column
+  float acc[NBT][4], y[NBT];
+#pragma unroll
+  for (int a = 0; a < NBT; a++) acc[a][0] = acc[a][1] = acc[a][2] = acc[a][3] = 0.f;
+#pragma unroll
+  for (int J = 0; J < NBT; J++) {
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t i = 16 * J + 4 * q + k;
+      const float ri = i < dw ? r[(size_t)mem[i / 3] * 3 + i % 3] : 0.f;
+      v[k] = ri - sum_cols(acc[J][k]);  // by rows
+    }
+    const float4 D = T[(size_t)tile_index(NBT, J, J) * 64];
+    const float yj = sum_rows(D.x * v[0] + D.y * v[1] + D.z * v[2] + D.w * v[3]);  // y_J[c]
+    y[J] = yj;
+#pragma unroll
+    for (int I = J + 1; I < NBT; I++) {
+      const float4 t = T[(size_t)tile_index(NBT, I, J) * 64];
+      acc[I][0] += t.x * yj;
+      acc[I][1] += t.y * yj;
+      acc[I][2] += t.z * yj;
+      acc[I][3] += t.w * yj;
+    }
+  }
+  // back: x = L^-T y, block row by block row from the last
+  float bcc[NBT];
+#pragma unroll
+  for (int a = 0; a < NBT; a++) bcc[a] = 0.f;
+#pragma unroll
+  for (int J = NBT - 1; J >= 0; J--) {
+    const float v = y[J] - sum_rows(bcc[J]);  // by columns
+    const float4 D = T[(size_t)tile_index(NBT, J, J) * 64];
+    const float x0 = sum_cols(D.x * v), x1 = sum_cols(D.y * v), x2 = sum_cols(D.z * v), x3 = sum_cols(D.w * v);
+    if (c == 0) {
+      const float xs[4] = {x0, x1, x2, x3};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint32_t i = 16 * J + 4 * q + k;
+        if (i < dw) z[(size_t)mem[i / 3] * 3 + i % 3] = xs[k];
+      }
+    }
+#pragma unroll
+    for (int I = 0; I < J; I++) {
+      const float4 t = T[(size_t)tile_index(NBT, J, I) * 64];
+      bcc[I] += t.x * x0 + t.y * x1 + t.z * x2 + t.w * x3;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- facade
 
 Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size, hipStream_t st,
@@ -817,6 +962,7 @@ static SwArgs sw_args(SchwarzImpl& S) {
   a.D = S.D; a.dmax = S.dmax; a.DW = S.DW;
   a.members = S.members.ptr; a.size = S.size.ptr; a.domain_of = S.domain_of.ptr; a.local_of = S.local_of.ptr;
   a.A = S.A.ptr; a.L = S.L.ptr; a.L32 = S.L32.ptr; a.L32T = S.L32T.ptr; a.W = S.W.ptr; a.flag = S.flag.ptr;
+  a.T32 = S.T32.ptr; a.nbt = S.nbt;
   return a;
 }
 
@@ -824,13 +970,16 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
   SchwarzImpl& S = *im_;
   if (!S.D) return;
   if (S.coarse) S.coarse->assemble(me, mat, x, diag);
-  if (fp32 && !S.L32.ptr) {
+  const uint32_t nb = (S.DW + 15) / 16;
+  if (fp32 && nb <= 12 && !S.T32.ptr) {
+    S.nbt = nb;
+    S.T32.alloc((size_t)S.D * tile_count(nb) * 256);
+  } else if (fp32 && nb > 12 && !S.L32.ptr) {
     S.L32.alloc((size_t)S.D * tri_size(S.DW));
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
   }
   const SwArgs a = sw_args(S);
   hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwBlock), 0, S.st, a, me, mat, x, diag);
-  const uint32_t nb = (S.DW + 15) / 16;
   if (fp32 && nb <= 12) {
 #define SW_RL32(C, NB)                                                                                       \
   case C:                                                                                                    \
@@ -841,6 +990,7 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
       SW_RL32(8, 9) SW_RL32(9, 9) SW_RL32(10, 10) SW_RL32(11, 11) default: SW_RL32(12, 12)
     }
 #undef SW_RL32
+    hipLaunchKernelGGL(k_sw_tile_inv, dim3(S.D), dim3(192), 0, S.st, a);
     CS_LAUNCH_CHECK();
     return;
   }
@@ -892,6 +1042,26 @@ void Schwarz::apply32(const float* r, float* z, const int32_t* done) {
   SchwarzImpl& S = *im_;
   if (!S.D) return;
   const SwArgs a = sw_args(S);
+  if (S.T32.ptr) {
+    const dim3 g(S.D), b(kWave);
+    switch (S.nbt) {
+      case 1: hipLaunchKernelGGL(k_sw_apply32t<1>, g, b, 0, S.st, a, r, z, done); break;
+      case 2: hipLaunchKernelGGL(k_sw_apply32t<2>, g, b, 0, S.st, a, r, z, done); break;
+      case 3: hipLaunchKernelGGL(k_sw_apply32t<3>, g, b, 0, S.st, a, r, z, done); break;
+      case 4: hipLaunchKernelGGL(k_sw_apply32t<4>, g, b, 0, S.st, a, r, z, done); break;
+      case 5: hipLaunchKernelGGL(k_sw_apply32t<5>, g, b, 0, S.st, a, r, z, done); break;
+      case 6: hipLaunchKernelGGL(k_sw_apply32t<6>, g, b, 0, S.st, a, r, z, done); break;
+      case 7: hipLaunchKernelGGL(k_sw_apply32t<7>, g, b, 0, S.st, a, r, z, done); break;
+      case 8: hipLaunchKernelGGL(k_sw_apply32t<8>, g, b, 0, S.st, a, r, z, done); break;
+      case 9: hipLaunchKernelGGL(k_sw_apply32t<9>, g, b, 0, S.st, a, r, z, done); break;
+      case 10: hipLaunchKernelGGL(k_sw_apply32t<10>, g, b, 0, S.st, a, r, z, done); break;
+      case 11: hipLaunchKernelGGL(k_sw_apply32t<11>, g, b, 0, S.st, a, r, z, done); break;
+      default: hipLaunchKernelGGL(k_sw_apply32t<12>, g, b, 0, S.st, a, r, z, done); break;
+    }
+    CS_LAUNCH_CHECK();
+    if (S.coarse) S.coarse->apply32(r, z, done);
+    return;
+  }
   switch ((S.DW + kWave - 1) / kWave) {
     case 1: hipLaunchKernelGGL(k_sw_apply32<1>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;
     case 2: hipLaunchKernelGGL(k_sw_apply32<2>, dim3(S.D), dim3(kWave), 0, S.st, a, r, z, done); break;

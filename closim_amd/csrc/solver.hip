@@ -50,9 +50,9 @@ struct NewtonScalars {
 // Device Newton control: the descent safeguard (newton.ts:154-160) and the
 // trust region (:167-179) decided on the device from the reductions the
 // preceding kernels wrote, so dx and g.dx are final without a host read.
-__global__ void k_ctl_descent(NewtonScalars* s) {
+__global__ void k_ctl_descent(NewtonScalars* s, const int32_t* pcg_done, int gate_breakdown) {
   const double g = s->gtdx;
-  s->fallback = (!isfinite(g) || g >= 0) ? 1 : 0;
+  s->fallback = (!isfinite(g) || g >= 0 || (gate_breakdown && *pcg_done == 1)) ? 1 : 0;
 }
 
 __global__ void k_fallback_pred(const double* __restrict__ g, const double* __restrict__ diag, double* dx, size_t n,
@@ -832,6 +832,10 @@ struct Solver {
     const double gradTol = opt.grad_tol;
     const bool polish = (opt.flags & CS_FLAG_POLISH_ROOT) != 0;
     const bool devctl = (opt.flags & CS_FLAG_DEVICE_NEWTON) != 0;
+    // the WebGPU Newton's descent gate (gpu-solver.ts:1168-1188,
+    // newton-control.wgsl:58-73): a PCG stopped on non-positive curvature
+    // also selects the Jacobi-scaled steepest descent
+    const bool gate_breakdown = (opt.flags & CS_FLAG_BREAKDOWN_DESCENT) != 0;
     CS_HIP(hipMemcpyAsync(x0.ptr, pos.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_predictor, dim3(grid_for(n)), dim3(kBlock), 0, st, x0.ptr, vel.ptr, h, gravity[0],
                        gravity[1], gravity[2], yhat.ptr, x.ptr, n);
@@ -903,7 +907,7 @@ struct Solver {
         // descent safeguard and trust region decided on the device; g.dx is
         // read with the first line-search batch
         launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
-        hipLaunchKernelGGL(k_ctl_descent, dim3(1), dim3(1), 0, st, ds());
+        hipLaunchKernelGGL(k_ctl_descent, dim3(1), dim3(1), 0, st, ds(), pcg_done_flag(), gate_breakdown ? 1 : 0);
         hipLaunchKernelGGL(k_fallback_pred, dim3(grid_for(n3)), dim3(kBlock), 0, st, g.ptr, diag.ptr, dx.ptr, n3,
                            ds());
         launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
@@ -915,7 +919,8 @@ struct Solver {
         // descent safeguard (newton.ts:154-160)
         launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
         gtdx = read(&ds()->gtdx);
-        if (!std::isfinite(gtdx) || gtdx >= 0) {
+        const bool breakdown = gate_breakdown && read(pcg_done_flag()) == 1;
+        if (!std::isfinite(gtdx) || gtdx >= 0 || breakdown) {
           launch_fallback_dir(st, g.ptr, diag.ptr, dx.ptr, n3);
           launch_dot(st, g.ptr, dx.ptr, n3, partials.ptr, &ds()->gtdx);
           gtdx = read(&ds()->gtdx);
@@ -1086,6 +1091,9 @@ struct Solver {
     if (contact) a.contact = contact->hvp_hook();
     return a;
   }
+
+  // the latch of the last inner solve (0 cap, 1 non-positive curvature, 2 converged)
+  const int32_t* pcg_done_flag() { return mixed ? mixed->done_flag() : &ds()->pcg.done; }
 
   int mixed_solve(double invH2, int maxIters) {
     if (packer) {

@@ -93,6 +93,13 @@ enum { CS_COARSE_NONE = 0, CS_COARSE_JACOBI = 1, CS_COARSE_BLOCK = 2, CS_COARSE_
  * reference's order and rules, so positions, energies, trial counts and
  * contact diagnostics are those of the sequential line search. */
 #define CS_FLAG_DEVICE_NEWTON 32u
+/* The descent gate of the reference's WebGPU Newton (src/backend/webgpu/
+ * gpu-solver.ts:1168-1188, shaders/newton-control.wgsl:58-73) instead of
+ * CpuSolver's (src/solver/newton.ts:154-160): the Jacobi-scaled steepest
+ * descent -g / diag replaces the PCG direction also when the PCG stopped on
+ * non-positive curvature (p.Ap <= 1e-30), not only when the direction is
+ * not a descent direction.  Off by default (CpuSolver's trajectories). */
+#define CS_FLAG_BREAKDOWN_DESCENT 64u
 
 /* ClothMaterial (src/physics/types.ts:1-14). */
 typedef struct cs_material {

@@ -68,3 +68,34 @@ def test_device_control_fp64_matches_oracle():
             assert st[key] == ref[key], (k, key)
         for key in ("active_pairs", "rejected_line_search_steps", "ccd_failures", "min_distance", "min_toi"):
             assert st["contact"][key] == ref["contact"][key], (k, key)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_breakdown_descent_gate_host_equals_device(precision):
+    # CS_FLAG_BREAKDOWN_DESCENT, the reference WebGPU Newton's gate
+    # (gpu-solver.ts:1168-1188, newton-control.wgsl:58-73): a PCG stopped on
+    # non-positive curvature also takes the Jacobi-scaled steepest descent.
+    # The host loop reads the PCG latch, the device control kernel reads it
+    # on the device: the same trajectory bit for bit.  On the 64x64 fold the
+    # compressed membrane makes PCG break down, so the gated trajectory leaves
+    # CpuSolver's.
+    kw = dict(precond=_abi.CS_PRECOND_SCHWARZ, schwarz=(64, 1, 64))
+    gate = _abi.CS_FLAG_BREAKDOWN_DESCENT
+    scs = [S.crumple_fold(63) for _ in range(3)]
+    a = HipSolver(precision, **kw, flags=gate)
+    b = HipSolver(precision, **kw, flags=gate | _abi.CS_FLAG_DEVICE_NEWTON)
+    c = HipSolver(precision, **kw)
+    for g, sc in zip((a, b, c), scs):
+        g.initialize(sc)
+    moved = 0.0
+    for k in range(12):
+        for g in (a, b, c):
+            g.step(H)
+        assert np.array_equal(scs[0].positions, scs[1].positions), k
+        assert np.array_equal(scs[0].velocities, scs[1].velocities), k
+        assert a.lastStats == b.lastStats, k
+        moved = max(moved, float(np.max(np.abs(scs[0].positions - scs[2].positions))))
+    print(f"fold64 {precision}: gated vs CpuSolver gate max position difference {moved:.3e} m", flush=True)
+    assert moved > 0
+    for g in (a, b, c):
+        g.close()
