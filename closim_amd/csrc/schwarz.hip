@@ -211,9 +211,11 @@ __global__ void __launch_bounds__(kSwBlock) k_sw_assemble(SwArgs s, DMesh me, DM
   const uint32_t d = blockIdx.x;
   const uint32_t dv = s.size[d], dw = dv * 3;
   double* A = dom_mat(s, d);
+  // clear the matrix with consecutive threads on consecutive entries; each
+  // thread then owns the three rows of its vertices
+  for (uint32_t e = threadIdx.x; e < dw * dw; e += blockDim.x) A[e] = 0;
+  __syncthreads();
   for (uint32_t l = threadIdx.x; l < dv; l += blockDim.x) {
-    for (uint32_t r = l * 3; r < l * 3 + 3; r++)
-      for (uint32_t c = 0; c < dw; c++) A[(size_t)r * dw + c] = 0;
     const uint32_t v = s.members[(size_t)d * s.dmax + l];
     for (uint32_t q = me.vt_off[v]; q < me.vt_off[v + 1]; q++) {
       const uint4 cr = me.vt_corner[q];
