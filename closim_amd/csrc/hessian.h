@@ -9,8 +9,12 @@ namespace cloth {
 
 // Rows A = ci*3 + ra (ra = 0..2), all columns B, of triangleHessian9
 // (membrane-blocks.ts:21-98), same loop nest and accumulation order.
-__device__ __forceinline__ void tri_hessian_rows(const double* x0, const double* x1, const double* x2, const float* inv,
-                                                 double area, const DMat& mat, int ci, double* h27) {
+// Rows ra0 .. ra1-1 of corner ci's three rows into h[(ra - ra0) * 9 + B];
+// every entry is computed on its own, so any subset of rows gives the same
+// values as the full block.
+__device__ __forceinline__ void tri_hessian_rows_range(const double* x0, const double* x1, const double* x2,
+                                                       const float* inv, double area, const DMat& mat, int ci,
+                                                       int ra0, int ra1, double* h) {
   const double a = inv[0], b = inv[1], c = inv[2], d = inv[3];
   const double* X[3] = {x0, x1, x2};
   const double K[3][2] = {{-(a + c), -(b + d)}, {a, b}, {c, d}};
@@ -35,7 +39,7 @@ __device__ __forceinline__ void tri_hessian_rows(const double* x0, const double*
   D[0][1][0][1] = mat.g; D[0][1][1][0] = mat.g; D[1][0][0][1] = mat.g; D[1][0][1][0] = mat.g;
   const double s = mat.thick * area;
   // M[A][i][j] = (i == A % 3) ? K[A / 3][j] : 0
-  for (int ra = 0; ra < 3; ra++) {
+  for (int ra = ra0; ra < ra1; ra++) {
     const int A = ci * 3 + ra, av = A / 3, ai = A % 3;
     for (int B = 0; B < 9; B++) {
       const int bv = B / 3, bi = B % 3;
@@ -67,9 +71,14 @@ __device__ __forceinline__ void tri_hessian_rows(const double* x0, const double*
           sum += mA * dP;
         }
       }
-      h27[ra * 9 + B] = s * sum;
+      h[(ra - ra0) * 9 + B] = s * sum;
     }
   }
+}
+
+__device__ __forceinline__ void tri_hessian_rows(const double* x0, const double* x1, const double* x2, const float* inv,
+                                                 double area, const DMat& mat, int ci, double* h27) {
+  tri_hessian_rows_range(x0, x1, x2, inv, area, mat, ci, 0, 3, h27);
 }
 
 }  // namespace cloth

@@ -206,17 +206,25 @@ __device__ __forceinline__ size_t row_off(uint32_t i) { return (size_t)i * (i + 
 // Local Hessian of domain d: membrane 3x3 blocks of intra-domain corner
 // pairs and the Jacobi diagonal (schwarz.ts:170-203); the symmetrisation
 // (:204-210) happens as the factor kernel loads it.
-__global__ void __launch_bounds__(kSwBlock) k_sw_assemble(SwArgs s, DMesh me, DMat mat, const double* __restrict__ x,
-                                                          const double* __restrict__ diag) {
+constexpr int kSwAsmBlock = 256;
+
+__global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me, DMat mat,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ diag) {
   const uint32_t d = blockIdx.x;
   const uint32_t dv = s.size[d], dw = dv * 3;
   double* A = dom_mat(s, d);
-  // clear the matrix with consecutive threads on consecutive entries; each
-  // thread then owns the three rows of its vertices
+  // clear the matrix with consecutive threads on consecutive entries
   for (uint32_t e = threadIdx.x; e < dw * dw; e += blockDim.x) A[e] = 0;
   __syncthreads();
-  for (uint32_t l = threadIdx.x; l < dv; l += blockDim.x) {
+  // one thread per row (vertex l = r / 3, component ra = r % 3): it evaluates
+  // its row of every corner Hessian of the vertex and adds it in corner
+  // order, then the diagonal
+  for (uint32_t r = threadIdx.x; r < dw; r += blockDim.x) {
+    const uint32_t l = r / 3;
+    const int ra = (int)(r % 3);
     const uint32_t v = s.members[(size_t)d * s.dmax + l];
+    double* Ar = A + (size_t)r * dw;
     for (uint32_t q = me.vt_off[v]; q < me.vt_off[v + 1]; q++) {
       const uint4 cr = me.vt_corner[q];
       const uint32_t t = cr.w >> 2;
@@ -224,17 +232,16 @@ __global__ void __launch_bounds__(kSwBlock) k_sw_assemble(SwArgs s, DMesh me, DM
       const uint32_t cvs[3] = {cr.x, cr.y, cr.z};
       double X[3][3];
       for (int k = 0; k < 3; k++) ld3(x, cvs[k], X[k]);
-      double h27[27];
-      tri_hessian_rows(X[0], X[1], X[2], me.inv_dm + (size_t)t * 4, me.area[t], mat, ci, h27);
+      double h9[9];
+      tri_hessian_rows_range(X[0], X[1], X[2], me.inv_dm + (size_t)t * 4, me.area[t], mat, ci, ra, ra + 1, h9);
       for (int cj = 0; cj < 3; cj++) {
         const uint32_t w = cvs[cj];
         if (s.domain_of[w] != d) continue;
         const uint32_t lw = s.local_of[w];
-        for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++) A[(size_t)(l * 3 + a) * dw + (lw * 3 + b)] += h27[a * 9 + cj * 3 + b];
+        for (int b = 0; b < 3; b++) Ar[lw * 3 + b] += h9[cj * 3 + b];
       }
     }
-    for (int a = 0; a < 3; a++) A[(size_t)(l * 3 + a) * dw + (l * 3 + a)] += diag[(size_t)v * 3 + a];
+    Ar[r] += diag[(size_t)v * 3 + ra];
   }
 }
 
@@ -981,7 +988,7 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
   }
   const SwArgs a = sw_args(S);
-  hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwBlock), 0, S.st, a, me, mat, x, diag);
+  hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag);
   if (fp32 && nb <= 12) {
 #define SW_RL32(C, NB)                                                                                       \
   case C:                                                                                                    \
