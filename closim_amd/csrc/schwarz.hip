@@ -406,8 +406,11 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
       const uint32_t i = 16 * a + tr, k = 16 * b + tc;
       double v = 0;
       if (i < dw && k <= i) {
-        // symmetrised (schwarz.ts:204-210): 0.5 (A_ki + A_ik)
-        v = i == k ? A[(size_t)i * dw + i] : 0.5 * (A[(size_t)k * dw + i] + A[(size_t)i * dw + k]);
+        // symmetrised (schwarz.ts:204-210): 0.5 (A_ki + A_ik) in FP64; the
+        // FP32 factor reads A_ki alone (the two halves differ by FP64
+        // rounding, far below FP32's), one coalesced read instead of two
+        if constexpr (kF32) v = A[(size_t)k * dw + i];
+        else v = i == k ? A[(size_t)i * dw + i] : 0.5 * (A[(size_t)k * dw + i] + A[(size_t)i * dw + k]);
         if (i == k) dmax = jsm::jmax(dmax, jsm::fabs_(v));
       }
       e[a][b] = (T)v;
