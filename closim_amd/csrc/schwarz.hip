@@ -36,6 +36,7 @@ struct SchwarzImpl {
   std::vector<uint32_t> h_domain_of, h_local_of;
   DevBuf<uint32_t> members, size, domain_of, local_of;
   DevBuf<double> A;      // assembled local Hessians, D * DW * DW, row stride dw of each domain
+  DevBuf<float> A32;     // the same in FP32 for the mixed solve's factor (domains of at most 12 blocks)
   DevBuf<double> L;      // factors, D * T (T = DW (DW + 1) / 2), packed lower triangle by columns
   DevBuf<float> L32;     // FP32 copy of L for the mixed-precision apply (reciprocal diagonal)
   DevBuf<float> L32T;    // the same packed by rows, for its back substitution
@@ -180,6 +181,7 @@ struct SwArgs {
   float* L32;
   float* L32T;
   float* T32;
+  float* A32;
   uint32_t nbt;
   double* W;
   uint8_t* flag;
@@ -208,12 +210,15 @@ __device__ __forceinline__ size_t row_off(uint32_t i) { return (size_t)i * (i + 
 // (:204-210) happens as the factor kernel loads it.
 constexpr int kSwAsmBlock = 256;
 
+// The local Hessian of domain d into Ad (FP64, or FP32 for the mixed
+// solve's factor, which accumulates in FP32).
+template <typename AT>
 __global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me, DMat mat,
                                                              const double* __restrict__ x,
-                                                             const double* __restrict__ diag) {
+                                                             const double* __restrict__ diag, AT* Abase) {
   const uint32_t d = blockIdx.x;
   const uint32_t dv = s.size[d], dw = dv * 3;
-  double* A = dom_mat(s, d);
+  AT* A = Abase + (size_t)d * s.DW * s.DW;
   // clear the matrix with consecutive threads on consecutive entries
   for (uint32_t e = threadIdx.x; e < dw * dw; e += blockDim.x) A[e] = 0;
   __syncthreads();
@@ -224,7 +229,7 @@ __global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me,
     const uint32_t l = r / 3;
     const int ra = (int)(r % 3);
     const uint32_t v = s.members[(size_t)d * s.dmax + l];
-    double* Ar = A + (size_t)r * dw;
+    AT* Ar = A + (size_t)r * dw;
     for (uint32_t q = me.vt_off[v]; q < me.vt_off[v + 1]; q++) {
       const uint4 cr = me.vt_corner[q];
       const uint32_t t = cr.w >> 2;
@@ -238,10 +243,10 @@ __global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me,
         const uint32_t w = cvs[cj];
         if (s.domain_of[w] != d) continue;
         const uint32_t lw = s.local_of[w];
-        for (int b = 0; b < 3; b++) Ar[lw * 3 + b] += h9[cj * 3 + b];
+        for (int b = 0; b < 3; b++) Ar[lw * 3 + b] += (AT)h9[cj * 3 + b];
       }
     }
-    Ar[r] += diag[(size_t)v * 3 + ra];
+    Ar[r] += (AT)diag[(size_t)v * 3 + ra];
   }
 }
 
@@ -397,6 +402,7 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
   const uint32_t dv = s.size[d], dw = dv * 3;
   const uint32_t tr = threadIdx.x % 16, tc = threadIdx.x / 16;
   const double* A = dom_mat(s, d);
+  const float* A32 = kF32 ? s.A32 + (size_t)d * s.DW * s.DW : nullptr;  // the FP32 assembly
   T e[NB][NB];
   double dmax = 0;
 #pragma unroll
@@ -409,7 +415,7 @@ __global__ void __launch_bounds__(256) k_sw_factor_rl(SwArgs s, const double* __
         // symmetrised (schwarz.ts:204-210): 0.5 (A_ki + A_ik) in FP64; the
         // FP32 factor reads A_ki alone (the two halves differ by FP64
         // rounding, far below FP32's), one coalesced read instead of two
-        if constexpr (kF32) v = A[(size_t)k * dw + i];
+        if constexpr (kF32) v = A32[(size_t)k * dw + i];
         else v = i == k ? A[(size_t)i * dw + i] : 0.5 * (A[(size_t)k * dw + i] + A[(size_t)i * dw + k]);
         if (i == k) dmax = jsm::jmax(dmax, jsm::fabs_(v));
       }
@@ -974,7 +980,7 @@ static SwArgs sw_args(SchwarzImpl& S) {
   a.D = S.D; a.dmax = S.dmax; a.DW = S.DW;
   a.members = S.members.ptr; a.size = S.size.ptr; a.domain_of = S.domain_of.ptr; a.local_of = S.local_of.ptr;
   a.A = S.A.ptr; a.L = S.L.ptr; a.L32 = S.L32.ptr; a.L32T = S.L32T.ptr; a.W = S.W.ptr; a.flag = S.flag.ptr;
-  a.T32 = S.T32.ptr; a.nbt = S.nbt;
+  a.T32 = S.T32.ptr; a.nbt = S.nbt; a.A32 = S.A32.ptr;
   return a;
 }
 
@@ -986,12 +992,16 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
   if (fp32 && nb <= 12 && !S.T32.ptr) {
     S.nbt = nb;
     S.T32.alloc((size_t)S.D * tile_count(nb) * 256);
+    S.A32.alloc((size_t)S.D * S.DW * S.DW);
   } else if (fp32 && nb > 12 && !S.L32.ptr) {
     S.L32.alloc((size_t)S.D * tri_size(S.DW));
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
   }
   const SwArgs a = sw_args(S);
-  hipLaunchKernelGGL(k_sw_assemble, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag);
+  if (fp32 && nb <= 12)
+    hipLaunchKernelGGL(k_sw_assemble<float>, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A32.ptr);
+  else
+    hipLaunchKernelGGL(k_sw_assemble<double>, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A.ptr);
   if (fp32 && nb <= 12) {
 #define SW_RL32(C, NB)                                                                                       \
   case C:                                                                                                    \
