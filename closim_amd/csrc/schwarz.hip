@@ -952,7 +952,6 @@ Schwarz::Schwarz(const cs_scene_desc& d, int target, int min_size, int max_size,
   S.size.upload(sz.data(), sz.size(), st);
   S.domain_of.upload(S.h_domain_of.data(), S.n, st);
   S.local_of.upload(S.h_local_of.data(), S.n, st);
-  S.A.alloc((size_t)std::max<uint32_t>(S.D, 1) * S.DW * S.DW);
   S.L.alloc((size_t)std::max<uint32_t>(S.D, 1) * tri_size(S.DW));
   S.flag.alloc(std::max<uint32_t>(S.D, 1));
   // the triangle being factored stays in LDS when it fits the device's
@@ -997,6 +996,8 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
     S.L32.alloc((size_t)S.D * tri_size(S.DW));
     S.L32T.alloc((size_t)S.D * tri_size(S.DW));
   }
+  // the FP64 domain matrices only for the FP64 factor (allocated on first use)
+  if (!(fp32 && nb <= 12) && !S.A.ptr) S.A.alloc((size_t)S.D * S.DW * S.DW);
   const SwArgs a = sw_args(S);
   if (fp32 && nb <= 12)
     hipLaunchKernelGGL(k_sw_assemble<float>, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A32.ptr);
