@@ -211,8 +211,9 @@ __device__ __forceinline__ size_t row_off(uint32_t i) { return (size_t)i * (i + 
 constexpr int kSwAsmBlock = 256;
 
 // The local Hessian of domain d into Ad (FP64, or FP32 for the mixed
-// solve's factor, which accumulates in FP32).
-template <typename AT>
+// solve's factor, which accumulates in FP32 and, with kUpper, only the
+// entries on or right of the diagonal: the FP32 factor reads A_ki, i >= k).
+template <typename AT, bool kUpper>
 __global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me, DMat mat,
                                                              const double* __restrict__ x,
                                                              const double* __restrict__ diag, AT* Abase) {
@@ -243,7 +244,8 @@ __global__ void __launch_bounds__(kSwAsmBlock) k_sw_assemble(SwArgs s, DMesh me,
         const uint32_t w = cvs[cj];
         if (s.domain_of[w] != d) continue;
         const uint32_t lw = s.local_of[w];
-        for (int b = 0; b < 3; b++) Ar[lw * 3 + b] += (AT)h9[cj * 3 + b];
+        for (int b = 0; b < 3; b++)
+          if (!kUpper || lw * 3 + b >= r) Ar[lw * 3 + b] += (AT)h9[cj * 3 + b];
       }
     }
     Ar[r] += (AT)diag[(size_t)v * 3 + ra];
@@ -1000,9 +1002,9 @@ void Schwarz::factor(const DMesh& me, const DMat& mat, const double* x, const do
   if (!(fp32 && nb <= 12) && !S.A.ptr) S.A.alloc((size_t)S.D * S.DW * S.DW);
   const SwArgs a = sw_args(S);
   if (fp32 && nb <= 12)
-    hipLaunchKernelGGL(k_sw_assemble<float>, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A32.ptr);
+    hipLaunchKernelGGL((k_sw_assemble<float, true>), dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A32.ptr);
   else
-    hipLaunchKernelGGL(k_sw_assemble<double>, dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A.ptr);
+    hipLaunchKernelGGL((k_sw_assemble<double, false>), dim3(S.D), dim3(kSwAsmBlock), 0, S.st, a, me, mat, x, diag, S.A.ptr);
   if (fp32 && nb <= 12) {
 #define SW_RL32(C, NB)                                                                                       \
   case C:                                                                                                    \
