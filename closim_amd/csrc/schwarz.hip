@@ -387,7 +387,8 @@ __device__ __forceinline__ float readlane_f32(float x, uint32_t l) {
 // subtractions for s = 0, 1, ..., k - 1 (mul, then sub), then the division by
 // L_kk (on the diagonal, the square root), so it is bit-identical to the
 // left-looking loop of schwarz.ts:212-235.
-// kF32 = true: the FP32 factor of the mixed solve (reciprocal pivots).
+// kF32 = true: the FP32 factor of the mixed solve (reciprocal pivots) from
+// the FP32 assembly (A_ki for i >= k), written as the tiles of k_sw_apply32t.
 //
 // Elements above the diagonal and rows or columns past the domain are
 // updated too, with whatever the column buffer holds: they are never read
