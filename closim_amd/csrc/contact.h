@@ -32,6 +32,9 @@ class ContactSystem {
   void set_segment(const double* xa, const double* xb);
   void update_active_set(const double* x);                 // updateActiveSet :164-220
   void energy_grad(const double* x, double* ceg);          // energyGrad :262-285
+  // energy_grad's gradient alone: no energy sums, no host read, the
+  // diagnostics untouched (for callers that stay on the stream)
+  void grad_only(const double* x, double* ceg);
   void add_diag_estimate(double* diag, const double* x);   // addDiagEstimate :436-471
   double merit_energy(const double* x);                    // meritEnergy :419-428
   void add_rejection();                                    // :83

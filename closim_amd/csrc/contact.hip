@@ -2300,16 +2300,18 @@ struct ContactImpl {
 
   CEval cev() { return CEval{inr.ptr, cg.ptr, fr.ptr, coeff.ptr}; }
 
-  void energy_grad(const double* x, double* ceg) {
+  void energy_grad(const double* x, double* ceg, bool energies = true) {
     const DGeom G = geom();
     if (C) {
       hipLaunchKernelGGL(k_contact_eval, dim3(red_grid(C)), dim3(kBlock), 0, st, G, x, xstep, act.ptr, anke.ptr, C,
                          lag_keys.ptr, lag_idx.ptr, lag_L, lag_vals.ptr, cev(), ddpart.ptr, cs.ptr);
       CS_LAUNCH_CHECK();
-      FinishJobs f{};
-      f.j[0] = FinishJob{ddpart.ptr, red_grid(C), FIN_DD, 0, &cs.ptr->red[2]};
-      f.j[1] = FinishJob{ddpart.ptr + kMaxRedBlocks, red_grid(C), FIN_DD, 0, &cs.ptr->red[3]};
-      launch_finish(st, f, 2);
+      if (energies) {
+        FinishJobs f{};
+        f.j[0] = FinishJob{ddpart.ptr, red_grid(C), FIN_DD, 0, &cs.ptr->red[2]};
+        f.j[1] = FinishJob{ddpart.ptr + kMaxRedBlocks, red_grid(C), FIN_DD, 0, &cs.ptr->red[3]};
+        launch_finish(st, f, 2);
+      }
     }
     CS_HIP(hipMemsetAsync(ceg, 0, (size_t)n * 3 * sizeof(double), st));
     if (E) {
@@ -2319,6 +2321,7 @@ struct ContactImpl {
                          hterm.ptr, ceg);
     }
     CS_LAUNCH_CHECK();
+    if (!energies) return;
     if (C) {
       sync_cs();
       diag.barrier_energy = hcs->red[2];
@@ -2595,6 +2598,7 @@ bool ContactSystem::check_trial(const double* xt) { return im_->check_trial(xt);
 void ContactSystem::set_segment(const double* xa, const double* xb) { im_->set_segment(xa, xb); }
 void ContactSystem::update_active_set(const double* x) { im_->update_active_set(x); }
 void ContactSystem::energy_grad(const double* x, double* ceg) { im_->energy_grad(x, ceg); }
+void ContactSystem::grad_only(const double* x, double* ceg) { im_->energy_grad(x, ceg, false); }
 void ContactSystem::add_diag_estimate(double* diag, const double*) { im_->add_diag_estimate(diag); }
 double ContactSystem::merit_energy(const double* x) { return im_->merit_energy(x); }
 void ContactSystem::probe_pairs(const double* x, double pad, std::vector<uint32_t>& self2,

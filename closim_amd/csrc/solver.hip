@@ -31,6 +31,15 @@ thread_local std::string g_last_error;
 
 constexpr int kArmijoBatch = 8;  // largest batch of line-search trials evaluated before one host read
 
+// Scalars of the adjoint's block-Jacobi PCG, kept on the device so that its
+// iterations run without a host read between them.
+struct AdjPcg {
+  double rz[2];  // r.z of the current iterate and of the next, alternating with the iteration's parity
+  double pap, rr, bnorm2, rel, tol;
+  int32_t it;    // iterations taken (each with p.Ap > 0)
+  int32_t done;  // 0 running, 1 p.Ap <= 0 (not positive definite along p), 2 converged or rhs = 0
+};
+
 struct NewtonScalars {
   PcgScalars pcg;
   double grad_sumsq;
@@ -39,7 +48,8 @@ struct NewtonScalars {
   double gtdx;
   double maxdx;
   double max_strain;
-  double adj[4];  // adjoint PCG: r.z, p.Ap, r.r, a.a
+  double adj[4];  // adjoint dot products: r.z, p.Ap, r.r, a.a
+  AdjPcg apcg;    // the adjoint PCG's scalars (full_pcg)
   // device Newton control (CS_FLAG_DEVICE_NEWTON)
   double tscale;                   // trust-region scale applied to dx
   int32_t fallback;                // 1 when the descent safeguard took -g / diag
@@ -118,14 +128,64 @@ __global__ void k_velocity(const double* x, const double* x0, double h, double d
 
 // ---- adjoint kernels (step_adjoint) ----------------------------------------
 
-__global__ void k_adj_shift(const double* x, const double* p, double s, double* out, size_t n3) {
+// out = x + (eps / |p|_inf) p, or x where p = 0 (|p|_inf read on the device)
+__global__ void k_adj_shift(const double* x, const double* p, const double* pmax, double eps, double* out,
+                            size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const double m = *pmax;
+  const double s = m > 0 ? eps / m : 0.0;
   if (i < n3) out[i] = x[i] + s * p[i];
 }
 
-__global__ void k_adj_fd_add(const double* gp, const double* gm, double scale, double* out, size_t n3) {
+// out += (gp - gm) |p|_inf / (2 eps), nothing where p = 0
+__global__ void k_adj_fd_add(const double* gp, const double* gm, const double* pmax, double two_eps, double* out,
+                             size_t n3) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const double m = *pmax;
+  if (!(m > 0)) return;
+  const double scale = m / two_eps;
   if (i < n3) out[i] += (gp[i] - gm[i]) * scale;
+}
+
+// PCG start: nothing to do for a zero right-hand side
+__global__ void k_adj_pcg_init(AdjPcg* a, double tol) {
+  a->tol = tol;
+  a->it = 0;
+  a->done = a->bnorm2 > 0 ? 0 : 2;
+  a->rel = a->bnorm2 > 0 ? 1.0 : 0.0;
+}
+
+// sol += alpha p, r -= alpha Ap, alpha = r.z / p.Ap; skipped once the solve
+// has ended or where p.Ap <= 0
+__global__ void k_adj_pcg_step(const AdjPcg* a, int cur, double* sol, double* r, const double* p, const double* ap,
+                               size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (a->done || !(a->pap > 0)) return;
+  const double alpha = a->rz[cur] / a->pap;
+  if (i < n3) {
+    sol[i] += alpha * p[i];
+    r[i] += -alpha * ap[i];
+  }
+}
+
+// after r.r: count the iteration, its relative residual, and the stops
+__global__ void k_adj_pcg_decide(AdjPcg* a) {
+  if (a->done) return;
+  if (!(a->pap > 0)) {
+    a->done = 1;
+    return;
+  }
+  a->it++;
+  a->rel = sqrt(a->rr / a->bnorm2);
+  if (a->rel <= a->tol) a->done = 2;
+}
+
+// p = z + beta p, beta = (r.z)_next / (r.z)_current
+__global__ void k_adj_pcg_dir(const AdjPcg* a, int cur, double* p, const double* z, size_t n3) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (a->done) return;
+  const double beta = a->rz[cur ^ 1] / a->rz[cur];
+  if (i < n3) p[i] = z[i] + beta * p[i];
 }
 
 // z = B^-1 r per vertex with the block-Jacobi inverses (pins zeroed)
@@ -670,23 +730,23 @@ struct Solver {
       launch_bend_hvp(st, dmesh(), kb, base, p, bend_term.ptr, out, true);
     }
     if (!contact) return;
+    // |p|_inf stays on the device (k_adj_shift / k_adj_fd_add read it)
     launch_max_abs(st, p, n3, mpartials.ptr, &ds()->maxdx);
-    const double pmax = read(&ds()->maxdx);
-    if (!(pmax > 0)) return;
     const double eps = 1e-6;  // metres along p / |p|_inf
-    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, eps / pmax, adj_xs.ptr, n3);
+    const double* pmax = &ds()->maxdx;
+    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, pmax, eps, adj_xs.ptr, n3);
     contact_gradient(adj_xs.ptr, adj_gp.ptr);
-    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, -eps / pmax, adj_xs.ptr, n3);
+    hipLaunchKernelGGL(k_adj_shift, dim3(grid_for(n3)), dim3(kBlock), 0, st, base, p, pmax, -eps, adj_xs.ptr, n3);
     contact_gradient(adj_xs.ptr, adj_gm.ptr);
-    hipLaunchKernelGGL(k_adj_fd_add, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr,
-                       pmax / (2 * eps), out, n3);
+    hipLaunchKernelGGL(k_adj_fd_add, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_gp.ptr, adj_gm.ptr, pmax, 2 * eps,
+                       out, n3);
     CS_LAUNCH_CHECK();
   }
 
   // the gradient term full_hvp differentiates numerically: contact alone
   // (the internal terms with every modulus zero), pins zeroed
   void contact_gradient(const double* xx, double* out) {
-    contact->energy_grad(xx, ceg.ptr);
+    contact->grad_only(xx, ceg.ptr);
     const DMat none{mat.thickness, 0, 0, 0, 0, 0};
     launch_gradient(st, dmesh(), none, xx, yhat.ptr, 0.0, ceg.ptr, out, partials.ptr, &ds()->grad_sumsq);
   }
@@ -701,9 +761,10 @@ struct Solver {
   // `rhs` must be zero on pinned DOFs.  Uses adj_r/z/p/ap/xs/gp/gm.
   double full_pcg(const double* base, const double* rhs, double* sol, double tol, int max_iters, int* iters) {
     const size_t n3 = (size_t)n * 3;
+    AdjPcg* A = &ds()->apcg;
     CS_HIP(hipMemsetAsync(sol, 0, n3 * sizeof(double), st));
     CS_HIP(hipMemcpyAsync(adj_r.ptr, rhs, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    const double bnorm2 = adj_dot(rhs, rhs, &ds()->adj[3]);
+    launch_dot(st, rhs, rhs, n3, partials.ptr, &A->bnorm2);
     // block-Jacobi: the 3x3 membrane vertex blocks at `base` plus the step's
     // diagonal, inverted (Jacobi where a block is not positive definite)
     adj_binv.alloc((size_t)n * 9);
@@ -714,26 +775,33 @@ struct Solver {
     };
     precondition();
     CS_HIP(hipMemcpyAsync(adj_p.ptr, adj_z.ptr, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double rz = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
-    int it = 0;
-    double rel = bnorm2 > 0 ? 1.0 : 0.0;
-    while (bnorm2 > 0 && it < max_iters) {
-      full_hvp(base, adj_p.ptr, adj_ap.ptr);
-      const double pap = adj_dot(adj_p.ptr, adj_ap.ptr, &ds()->adj[1]);
-      if (!(pap > 0)) break;  // not positive definite along p
-      const double alpha = rz / pap;
-      hipLaunchKernelGGL(k_adj_axpy, dim3(grid_for(n3)), dim3(kBlock), 0, st, sol, alpha, adj_p.ptr, n3);
-      hipLaunchKernelGGL(k_adj_axpy, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_r.ptr, -alpha, adj_ap.ptr, n3);
-      it++;
-      rel = std::sqrt(adj_dot(adj_r.ptr, adj_r.ptr, &ds()->adj[2]) / bnorm2);
-      if (rel <= tol) break;
-      precondition();
-      const double rz_new = adj_dot(adj_r.ptr, adj_z.ptr, &ds()->adj[0]);
-      hipLaunchKernelGGL(k_adj_xpby, dim3(grid_for(n3)), dim3(kBlock), 0, st, adj_p.ptr, adj_z.ptr, rz_new / rz, n3);
-      rz = rz_new;
+    launch_dot(st, adj_r.ptr, adj_z.ptr, n3, partials.ptr, &A->rz[0]);
+    hipLaunchKernelGGL(k_adj_pcg_init, dim3(1), dim3(1), 0, st, A, tol);
+    CS_LAUNCH_CHECK();
+    // Iterations are enqueued in chunks and the host reads the stop flag once
+    // per chunk; an iteration enqueued after the stop changes nothing (its
+    // update kernels return at once).
+    const int kChunk = 16;
+    int cur = 0, queued = 0;
+    while (queued < max_iters && read(&A->done) == 0) {
+      const int m = std::min(kChunk, max_iters - queued);
+      for (int k = 0; k < m; k++, queued++, cur ^= 1) {
+        full_hvp(base, adj_p.ptr, adj_ap.ptr);
+        launch_dot(st, adj_p.ptr, adj_ap.ptr, n3, partials.ptr, &A->pap);
+        hipLaunchKernelGGL(k_adj_pcg_step, dim3(grid_for(n3)), dim3(kBlock), 0, st, A, cur, sol, adj_r.ptr,
+                           adj_p.ptr, adj_ap.ptr, n3);
+        launch_dot(st, adj_r.ptr, adj_r.ptr, n3, partials.ptr, &A->rr);
+        hipLaunchKernelGGL(k_adj_pcg_decide, dim3(1), dim3(1), 0, st, A);
+        precondition();
+        launch_dot(st, adj_r.ptr, adj_z.ptr, n3, partials.ptr, &A->rz[cur ^ 1]);
+        hipLaunchKernelGGL(k_adj_pcg_dir, dim3(grid_for(n3)), dim3(kBlock), 0, st, A, cur, adj_p.ptr, adj_z.ptr,
+                           n3);
+        CS_LAUNCH_CHECK();
+      }
     }
-    if (iters) *iters = it;
-    return rel;
+    const AdjPcg a = read(A);
+    if (iters) *iters = a.it;
+    return a.rel;
   }
 
   void step_adjoint(double h, const double* hx0, const double* hv0, const double* hx1, const double* hgx,
