@@ -549,7 +549,10 @@ def main():
     ap.add_argument("--hvp-reps", type=int, default=200)
     ap.add_argument("--host-newton", action="store_true",
                     help="decide the descent safeguard, trust region and each Armijo trial on the host "
-                         "(default: CS_FLAG_DEVICE_NEWTON, the same results with one host read per trial batch)")
+                         "(default: CS_FLAG_DEVICE_NEWTON, the same results with one host read per trial)")
+    ap.add_argument("--speculate", action="store_true",
+                    help="with the device Newton control, evaluate Armijo trials in speculative batches of "
+                         "1, 2, 4, 8 (CS_FLAG_ARMIJO_SPECULATE; the same results)")
     ap.add_argument("--descent-gate", choices=["webgpu", "cpusolver"], default="webgpu",
                     help="Newton's descent gate: the reference WebGPU Newton's (steepest descent also after a PCG "
                          "breakdown, CS_FLAG_BREAKDOWN_DESCENT) or CpuSolver's (only for a non-descent direction)")
@@ -584,6 +587,8 @@ def main():
     h = 1.0 / 60.0
     popts = precond_options(args.workload)
     base_flags = (_abi.CS_FLAG_EAGER if args.eager else 0) | (0 if args.host_newton else _abi.CS_FLAG_DEVICE_NEWTON)
+    if args.speculate and not args.host_newton:
+        base_flags |= _abi.CS_FLAG_ARMIJO_SPECULATE
     flags = base_flags | (_abi.CS_FLAG_BREAKDOWN_DESCENT if args.descent_gate == "webgpu" else 0)
     print(f"[bench] device: {device_info(0)}", file=sys.stderr, flush=True)
     sc_m = make_scene(args.workload)
@@ -674,7 +679,8 @@ def main():
                    "vertices": mesh.count, "triangles": mesh.tri_count, "h": h,
                    "preconditioner": WORKLOADS[args.workload]["precond"],
                    "pcg_launch": "eager" if args.eager else "graph replay",
-                   "newton_control": "host" if args.host_newton else "device (batched Armijo trials)",
+                   "newton_control": "host" if args.host_newton else (
+                       "device (speculative Armijo batches)" if args.speculate else "device (one Armijo trial per host read)"),
                    "descent_gate": ("reference WebGPU Newton: steepest descent after a non-descent direction or a "
                                     "PCG breakdown" if args.descent_gate == "webgpu" else
                                     "CpuSolver: steepest descent after a non-descent direction"),

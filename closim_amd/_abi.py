@@ -30,6 +30,7 @@ CS_FLAG_POLISH_ROOT = 2
 CS_FLAG_EAGER = 4
 CS_FLAG_PSD_NEWTON = 8
 CS_FLAG_DEVICE_NEWTON = 32
+CS_FLAG_ARMIJO_SPECULATE = 128
 CS_FLAG_BREAKDOWN_DESCENT = 64
 
 CS_COPY_HOST_TO_DEVICE = 1

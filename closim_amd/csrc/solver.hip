@@ -1084,7 +1084,9 @@ struct Solver {
   bool armijo_batched(double h, int lsIters, double& energy, double& alpha, double& gtdx, int& lsTrials) {
     const size_t n3 = (size_t)n * 3;
     // the full step is usually accepted, and a trial computed past the
-    // accepted one costs a whole contact check: the first batch is one trial
+    // accepted one costs a whole contact check: the first batch is one trial,
+    // and later batches grow only when speculation is asked for
+    const bool speculate = (opt.flags & CS_FLAG_ARMIJO_SPECULATE) != 0;
     int li = 0, batch = 1;
     bool first = true;
     while (li < lsIters) {
@@ -1131,7 +1133,7 @@ struct Solver {
         alpha *= 0.5;
       }
       li += K;
-      batch *= 2;
+      if (speculate) batch *= 2;
     }
     return false;
   }

@@ -87,12 +87,19 @@ enum { CS_COARSE_NONE = 0, CS_COARSE_JACOBI = 1, CS_COARSE_BLOCK = 2, CS_COARSE_
 #define CS_FLAG_PSD_NEWTON 8u
 /* Device Newton control (no reference counterpart; the reference's GPU path
  * does the same, src/backend/webgpu/gpu-newton.ts:1889-2200): the descent
- * safeguard and the trust region are decided on the device, and the Armijo
- * trials are evaluated in batches of 1, 2, 4, 8 with one host read per batch
- * instead of two to three per trial.  The trials are judged in the
- * reference's order and rules, so positions, energies, trial counts and
- * contact diagnostics are those of the sequential line search. */
+ * safeguard and the trust region are decided on the device, and each Armijo
+ * trial's contact check and energies come back in one host read instead of
+ * two to three.  The trials are judged in the reference's order and rules,
+ * so positions, energies, trial counts and contact diagnostics are those of
+ * the sequential line search. */
 #define CS_FLAG_DEVICE_NEWTON 32u
+/* With CS_FLAG_DEVICE_NEWTON: evaluate the Armijo trials speculatively in
+ * batches of 1, 2, 4, 8 (the reference's batched candidates, shaders/
+ * armijo.wgsl:48-238) with one host read per batch; the same results.  Off
+ * by default: a trial costs a whole contact check, so the trials computed
+ * past the accepted one cost more than the host reads they save (the
+ * 512x512 fold runs 13.0 steps/s without, 11.6 with). */
+#define CS_FLAG_ARMIJO_SPECULATE 128u
 /* The descent gate of the reference's WebGPU Newton (src/backend/webgpu/
  * gpu-solver.ts:1168-1188, shaders/newton-control.wgsl:58-73) instead of
  * CpuSolver's (src/solver/newton.ts:154-160): the Jacobi-scaled steepest

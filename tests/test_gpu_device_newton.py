@@ -31,13 +31,16 @@ def drive(scene, solver, k):
             solver.pinVertex(i, p)
 
 
+@pytest.mark.parametrize("speculate", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_device_control_equals_sequential_line_search(name, precision):
+def test_device_control_equals_sequential_line_search(name, precision, speculate):
+    # one trial per host read, or speculative batches of 1, 2, 4, 8
     make, kw, steps = SCENES[name]
     sa, sb = make(), make()
     a = HipSolver(precision, **kw)
-    b = HipSolver(precision, **kw, flags=_abi.CS_FLAG_DEVICE_NEWTON)
+    spec = _abi.CS_FLAG_ARMIJO_SPECULATE if speculate else 0
+    b = HipSolver(precision, **kw, flags=_abi.CS_FLAG_DEVICE_NEWTON | spec)
     a.initialize(sa)
     b.initialize(sb)
     trials = 0
